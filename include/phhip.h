@@ -1,0 +1,346 @@
+/*
+ * phhip.h — C-ABI of the MI355X (gfx950) backend for ph-core's NArray
+ * data-parallel path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, never
+ * throws, and returns an int status (PH_OK = 0, negative on error; the text of
+ * the last error is in ph_last_error()).  This is exactly the surface a Crystal
+ * `lib LibPhHip` block binds: validation (region canonicalisation, shape
+ * checks, exception types and messages) stays on the host side, above this
+ * boundary, as in ph-core, where it is done before any element is touched.
+ *
+ * Device memory is owned by the caller (ph_alloc / ph_free).  Nothing here
+ * retains a pointer after it returns.  All kernels run on one HIP stream per
+ * process (ph_get_stream), at the default priority; calls are asynchronous
+ * with respect to the host unless the entry point says otherwise.
+ */
+#ifndef PHHIP_H
+#define PHHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH_MAX_RANK 8
+#define PH_MAX_STAGES 2
+#define PH_MAX_INSN 48
+#define PH_MAX_INPUTS 8
+#define PH_NREG 8
+
+/* Element types.  BF16 is a storage type only: it is widened to Float32 on
+ * load (ph-core has no BF16; a BF16 array behaves as Float32 data). BOOL is
+ * one byte, 0 or 1. */
+typedef enum {
+  PH_I32 = 0,
+  PH_F32 = 1,
+  PH_F64 = 2,
+  PH_BF16 = 3,
+  PH_BOOL = 4
+} ph_dtype;
+
+/* Status codes, one per exception class the reference raises on this path
+ * (src/exceptions/exceptions.cr:4-22 and Crystal's IndexError, ArgumentError,
+ * OverflowError, DivisionByZeroError). */
+enum {
+  PH_OK = 0,
+  PH_E_INDEX = -1,
+  PH_E_SHAPE = -2,
+  PH_E_DIMENSION = -3,
+  PH_E_ARGUMENT = -4,
+  PH_E_OVERFLOW = -5,
+  PH_E_DIVZERO = -6,
+  PH_E_HIP = -10,
+  PH_E_NOINIT = -11,
+  PH_E_UNSUPPORTED = -12,
+  PH_E_OOM = -13
+};
+
+/* Per-launch element flags reported by ph_map / ph_reduce (Crystal raises on
+ * these; the host side turns them into the matching exception). */
+enum {
+  PH_FLAG_OVERFLOW = 1, /* checked Int32 + - * ** or unary - overflowed   */
+  PH_FLAG_DIVZERO = 2,  /* Int32 // or % by zero                          */
+  PH_FLAG_NEGEXP = 4    /* Int32 ** with a negative exponent (ArgumentError)*/
+};
+
+/* ---------------------------------------------------------------------------
+ * View descriptor.
+ *
+ * A logical coordinate c (rank `view.rank`, extents `view.shape`) maps to a
+ * buffer element index by
+ *     idx = view.offset + sum_i c_i * view.stride[i]
+ *     for s in stage[0 .. nstage-1]:
+ *         z   = delinearize(idx, s.shape[0..s.rank-1])   (row-major)
+ *         idx = s.offset + sum_j z_j * s.stride[j]
+ * Strides are signed element strides; 0 repeats an element (broadcast).
+ * Region, Permute and Reverse transforms (src/view_util/transforms.cr:
+ * 392-505) fold into the affine map; Reshape (:319-390) and tile
+ * (src/multi_indexable.cr:818-827) need a stage.  `extent` is the element
+ * count of the underlying buffer; the library rejects a descriptor that can
+ * address outside [0, extent).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t rank;
+  int32_t pad_;
+  int64_t offset;
+  int64_t shape[PH_MAX_RANK];
+  int64_t stride[PH_MAX_RANK];
+} ph_affine;
+
+typedef struct {
+  int32_t dtype;
+  int32_t nstage;
+  int64_t extent;
+  ph_affine view;
+  ph_affine stage[PH_MAX_STAGES];
+} ph_view_desc;
+
+/* ---------------------------------------------------------------------------
+ * Element program: the compiled form of an elementwise expression (the
+ * operator macros of src/multi_indexable.cr:931-985, scalar operands
+ * src/multi_indexable.cr:947-951 and src/patches/number.cr:5-15, and a
+ * ProcView block src/proc_view.cr:156-158).  Registers hold one value each
+ * (Int32, Float32, Float64 or Bool, tagged by the instruction's dtype);
+ * results round to the instruction's dtype exactly as Crystal does, with no
+ * contraction into fused multiply-adds.
+ * ------------------------------------------------------------------------- */
+typedef enum {
+  PH_OP_LOAD = 0,  /* r[dst] = input[a] widened to dt (BF16 -> F32)       */
+  PH_OP_CONST,     /* r[dst] = imm (dt)                                    */
+  PH_OP_CAST,      /* r[dst] = (dt) r[a], source dtype in c                */
+  PH_OP_ADD,       /* dt I32: checked; F32/F64: IEEE                        */
+  PH_OP_SUB,
+  PH_OP_MUL,
+  PH_OP_WADD,      /* Int32 &+ &- &* (wrapping)                             */
+  PH_OP_WSUB,
+  PH_OP_WMUL,
+  PH_OP_DIV,       /* F32/F64 true division                                 */
+  PH_OP_FLOORDIV,  /* // : I32 floored (div by 0 flagged), F floor(a/b)     */
+  PH_OP_MOD,       /* %  : I32 floored modulo, F a - b*floor(a/b)           */
+  PH_OP_POWI,      /* F ** Int32 by binary powering (llvm.powi)             */
+  PH_OP_IPOW,      /* Int32 ** Int32, checked                               */
+  PH_OP_WIPOW,     /* Int32 &** Int32, wrapping                             */
+  PH_OP_POW,       /* F ** F (libm pow)                                     */
+  PH_OP_NEG,       /* unary - (checked for I32)                             */
+  PH_OP_BNOT,      /* unary ~ (I32)                                         */
+  PH_OP_AND,       /* & | ^ (I32, BOOL)                                     */
+  PH_OP_OR,
+  PH_OP_XOR,
+  PH_OP_GT,        /* comparisons of two dt operands -> BOOL                */
+  PH_OP_LT,
+  PH_OP_GE,
+  PH_OP_LE,
+  PH_OP_EQ,
+  PH_OP_NE,
+  PH_OP_CMP,       /* <=> -> I32 (-1, 0, 1)                                 */
+  PH_OP_SQRT,      /* F sqrt (correctly rounded)                            */
+  PH_OP_ABS,
+  PH_OP_SELECT,    /* r[dst] = r[c] ? r[a] : r[b]                           */
+  PH_OP__COUNT
+} ph_opcode;
+
+typedef struct {
+  uint8_t op;
+  uint8_t dt;
+  uint8_t dst;
+  uint8_t a;
+  uint8_t b;
+  uint8_t c;
+  uint16_t pad_;
+  union {
+    int64_t i;
+    double f;
+  } imm;
+} ph_insn;
+
+typedef struct {
+  int32_t n_insn;
+  int32_t n_in;
+  int32_t out_reg;
+  int32_t out_dtype;
+  ph_insn insn[PH_MAX_INSN];
+} ph_prog;
+
+/* ---------------------------------------------------------------------------
+ * Runtime
+ * ------------------------------------------------------------------------- */
+
+/* Select `device` and create the library's stream.  Replaces nothing in the
+ * reference (ph-core is CPU-only, src/n_array.cr:20 owns a host Slice). */
+int ph_init(int device);
+int ph_device_count(int *n);
+const char *ph_last_error(void);
+
+/* Device buffers backing an NArray (src/n_array.cr:20 `@buffer : Slice(T)`). */
+int ph_alloc(void **ptr, size_t bytes);
+int ph_free(void *ptr);
+/* Synchronous copies (the stream is drained first). */
+int ph_h2d(void *dst, const void *src, size_t bytes);
+int ph_d2h(void *dst, const void *src, size_t bytes);
+/* Asynchronous device-to-device copy on the library stream. */
+int ph_d2d(void *dst, const void *src, size_t bytes);
+int ph_sync(void);
+/* The hipStream_t every ph_* kernel is launched on. */
+int ph_get_stream(void **stream);
+/* HIP events on the library stream, for timing. */
+int ph_event_create(void **ev);
+int ph_event_record(void *ev);
+int ph_event_elapsed(float *ms, void *start, void *stop);
+int ph_event_destroy(void *ev);
+
+/* ---------------------------------------------------------------------------
+ * Data generation.  Counter-based splitmix64 stream: draw k of stream `seed`
+ * is splitmix64 output number k+1 from state seed; element i takes draw
+ * offset+i (normal: draws offset+2*(i/2), +1 feed one Box-Muller pair).
+ * dist: 0 uniform [a,b) (F: top 53 bits; I32: a + (draw>>11) mod (b-a)),
+ *       1 normal(a = mean, b = stddev), 2 Int32 uniform [0, 2^31) (draw>>33).
+ * No reference counterpart (ph-core's specs use literals); inputs of the
+ * benchmark workloads are generated on the device with it.
+ * ------------------------------------------------------------------------- */
+int ph_fill_random(void *buf, int32_t dtype, int64_t n, uint64_t seed,
+                   uint64_t offset, int32_t dist, double a, double b);
+
+/* ---------------------------------------------------------------------------
+ * Fill every element addressed by `dst` with one scalar (`value` points to one
+ * element of dst->dtype).
+ * Replaces NArray.fill (src/n_array.cr:230-232), NArray#unsafe_set_chunk(
+ * region, value) (src/n_array.cr:495-500) and MultiWritable#unsafe_set_chunk(
+ * region, value) (src/multi_writable.cr:40-44).
+ * ------------------------------------------------------------------------- */
+int ph_fill(void *buf, const ph_view_desc *dst, const void *value);
+
+/* ---------------------------------------------------------------------------
+ * Strided copy over a common logical shape: for every coordinate k,
+ *     dst[dd(k)] = src[sd(k)]      (same dtype; dd and sd have equal shapes)
+ * Gather: sd is a view, dd contiguous  — NArray#unsafe_fetch_chunk
+ *   (src/n_array.cr:450-453, odometer src/buffered/indexed/lex_iterator.cr:
+ *   7-21), View#to_narr (src/view.cr:123-126), MultiIndexable#reshape/
+ *   permute/reverse (src/multi_indexable.cr:795-803).
+ * Scatter: dd is a view, sd contiguous — NArray#unsafe_set_chunk(region,
+ *   src) (src/n_array.cr:484-492), MultiWritable#unsafe_set_chunk
+ *   (src/multi_writable.cr:29-36) through MutableView#unsafe_set_element
+ *   (src/mutable_view.cr:16-18).
+ * flags & PH_COPY_ZERO_REST: additionally write 0 to every element of dst's
+ *   buffer (dd->extent elements) that dd does not address — NArray.fill(
+ *   shape, 0) followed by the scatter, in one pass.  dd must be affine (no
+ *   stages) and its strides must address distinct elements.
+ * ------------------------------------------------------------------------- */
+#define PH_COPY_ZERO_REST 1
+int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src,
+                    const ph_view_desc *sd, int32_t flags);
+
+/* The same copy performed strictly in lexicographic order of the logical
+ * shape (one device thread), for a source that aliases the destination: each
+ * source element is read after every write that precedes it, which is what
+ * the reference's element-by-element loop does when a view of an array is
+ * written into the same array (src/n_array.cr:484-492 iterating `src.each`
+ * lazily, src/multi_writable.cr:29-36). */
+int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd,
+                               const void *src, const ph_view_desc *sd);
+
+/* ---------------------------------------------------------------------------
+ * Fused fill + scatter over a whole row-major destination buffer of shape
+ * dst_shape[rank]: element b gets src[sd(c)] if, on every axis a,
+ *     b_a = first[a] + c_{axis[a]} * step[a],  0 <= c_{axis[a]} < count[a]
+ * (axis[a] = -1: the axis is fixed at first[a]); every other element gets
+ * *fill.  One pass writes the destination once and reads the source once.
+ * Replaces NArray.fill(shape, value) (src/n_array.cr:230-232) followed by a
+ * scatter through a region / MutableView (src/multi_writable.cr:55-63,
+ * src/mutable_view.cr:16-18) into that fresh array.
+ * ------------------------------------------------------------------------- */
+int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
+                    int32_t rank, const int64_t *first, const int64_t *step,
+                    const int64_t *count, const int32_t *axis,
+                    const void *src, const ph_view_desc *sd,
+                    const void *fill);
+
+/* ---------------------------------------------------------------------------
+ * Elementwise map: out[k] = prog(in_0[sd_0(k)], ..., in_{n-1}[sd_{n-1}(k)])
+ * for every coordinate k of out_desc's logical shape; every in_desc has that
+ * shape (broadcast is expressed by stride 0 / tile stages).  `flags_out`
+ * (host pointer, may be NULL) receives the OR of PH_FLAG_* raised by any
+ * element; the call then synchronises.
+ * Replaces MultiIndexable#map_with / each_with (src/multi_indexable.cr:
+ * 1026-1102), the operator macros (:931-985), NArray#map / map! (src/
+ * n_array.cr:589-628), scalar-first patches (src/patches/number.cr:5-21),
+ * and ProcView#to_narr (src/proc_view.cr:156-158 + src/view.cr:123-126).
+ * In-place map (map!) passes out == in_0 with equal contiguous descriptors.
+ * ------------------------------------------------------------------------- */
+int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins,
+           const ph_view_desc *in_descs, void *out,
+           const ph_view_desc *out_desc, int32_t *flags_out);
+
+/* ---------------------------------------------------------------------------
+ * Fused map + reduction of prog's value over a view of rank >= 1.
+ * keep_axis = -1: one result (Enumerable#sum / #max over `each`,
+ *   src/multi_indexable.cr:18, :656-658).
+ * keep_axis = k : shape[k] results, result j reducing the slice with
+ *   coordinate k fixed at j (`slices(k).map(&.sum)`, src/multi_indexable.cr:
+ *   742-786).
+ * ops: bit 0 sum, bit 1 max.  acc: accumulation mode for float values,
+ *   PH_ACC_F32C (FP32 Neumaier-compensated), PH_ACC_F64 (FP64 Neumaier-
+ *   compensated).  Int32/Bool values are summed exactly in 64 bits; an Int32
+ *   sum whose sequential (lexicographic) running total ever leaves the Int32
+ *   range sets PH_FLAG_OVERFLOW, as Crystal's checked `+` would raise.
+ * out_sum / out_max: device arrays of doubles (count results each; may be
+ *   NULL when the op is not requested).
+ * ------------------------------------------------------------------------- */
+#define PH_RED_SUM 1
+#define PH_RED_MAX 2
+#define PH_ACC_F32C 0
+#define PH_ACC_F64 1
+int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins,
+              const ph_view_desc *in_descs, int32_t keep_axis, int32_t ops,
+              int32_t acc, double *out_sum, double *out_max,
+              int32_t *flags_out);
+
+/* One-pass 2-D variant: for a rows x cols view, writes in one read of the
+ * data full sum/max (1 each), per-row sum/max (slices(0), `rows` each) and
+ * per-column sum/max (slices(1), `cols` each).  Any output may be NULL. */
+int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                    const ph_view_desc *in_descs, int32_t acc,
+                    double *full_sum, double *full_max, double *row_sum,
+                    double *row_max, double *col_sum, double *col_max,
+                    int32_t *flags_out);
+
+/* ---------------------------------------------------------------------------
+ * Explicit 2-D heat equation, 5-point FTCS stencil, Dirichlet-zero outside
+ * the ny x nx grid (cells beyond the edge read 0):
+ *   u'[i,j] = u[i,j] + ((((uN - 4u) + uS) + uW) + uE) * r
+ * the 2-D generalisation of examples/heat_equation.cr:38-51 (same operation
+ * order as :47, `(s[i-1] - 2*s[i] + s[i+1]) * COEFF`, then `el + diff` :50).
+ * dtype PH_F32 or PH_F64; u holds the initial field and receives the final
+ * one; tmp is scratch of the same size.  If energy_every > 0, energy_out
+ * (device, steps/energy_every + 1 doubles) receives the FP64 sum of u at
+ * steps 0, energy_every, 2*energy_every, ...
+ * ------------------------------------------------------------------------- */
+int ph_heat2d(int32_t dtype, void *u, void *tmp, int64_t ny, int64_t nx,
+              double r, int64_t steps, int64_t energy_every,
+              double *energy_out);
+
+/* Sum of squares of (a - b) and of b, both in FP64 (a, b of dtype F32 or
+ * F64, n elements): the relative L2 distance report of a reduced-precision
+ * field against an FP64 one.  out: device, 2 doubles. */
+int ph_l2_diff(int32_t dtype_a, const void *a, int32_t dtype_b, const void *b,
+               int64_t n, double *out);
+
+/* ---------------------------------------------------------------------------
+ * Diagnostics (no reference counterpart).
+ * ph_compile_check generates and compiles, without launching, the fused
+ * kernels ph_map (kind 0) and ph_reduce2d_all (kind 1) would use for `prog`
+ * over contiguous inputs of the given dtypes and a rows x cols shape.  It
+ * needs no GPU, so program lowering can be checked on a build host.
+ * ------------------------------------------------------------------------- */
+int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes,
+                     int64_t rows, int64_t cols, int32_t kind);
+/* Number of fused-kernel modules compiled (not loaded from cache) so far. */
+long ph_jit_compiles(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHHIP_H */

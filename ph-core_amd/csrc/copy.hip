@@ -1,0 +1,547 @@
+// Strided copies, fills and seeded random fills (hand-written, ahead-of-time).
+//
+// Layout of the work: a copy over a logical shape is first collapsed
+// (ph::collapse) to `rows x cols` with the innermost axis as `cols`.  One
+// 256-thread workgroup owns a tile of kTile consecutive columns of one row;
+// lane t of the group handles columns tile0 + e*256 + t for e < EPT, so for
+// every unrolled e the 64 lanes of a wavefront touch 64 consecutive elements
+// of the destination (fully coalesced) and consecutive source elements
+// `s_inner` apart (coalesced for |s_inner| <= 2 within 512-byte spans).  All
+// EPT loads are issued before the stores, keeping EPT x 256 B per wavefront in
+// flight.  Row offsets are decoded once per workgroup, in scalar registers.
+#include <cstring>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct RowArgs {
+  int64_t rows;
+  int64_t cols;
+  int64_t ntiles;
+  int64_t nblocks;
+  int32_t outer_rank;
+  int32_t pad_;
+  int64_t outer_shape[PH_MAX_RANK];
+  int64_t d_off, s_off;
+  int64_t d_outer[PH_MAX_RANK];
+  int64_t s_outer[PH_MAX_RANK];
+  int64_t d_inner, s_inner;
+};
+
+__device__ __forceinline__ void decode_row(const RowArgs &a, int64_t row, int64_t *d_base,
+                                           int64_t *s_base) {
+  int64_t d = a.d_off, s = a.s_off;
+  for (int k = a.outer_rank - 1; k >= 0; --k) {
+    int64_t n = a.outer_shape[k];
+    int64_t q = row / n;
+    int64_t c = row - q * n;
+    row = q;
+    d += c * a.d_outer[k];
+    s += c * a.s_outer[k];
+  }
+  *d_base = d;
+  *s_base = s;
+}
+
+template <typename E, int EPT>
+__global__ __launch_bounds__(kBlock) void k_copy_rows(E *__restrict__ dst,
+                                                      const E *__restrict__ src, RowArgs a) {
+  constexpr int64_t kTile = (int64_t)kBlock * EPT;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    const int64_t row = b / a.ntiles;
+    const int64_t tile = b - row * a.ntiles;
+    int64_t db, sb;
+    decode_row(a, row, &db, &sb);
+    const int64_t c0 = tile * kTile + threadIdx.x;
+    E v[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t c = c0 + (int64_t)e * kBlock;
+      if (c < a.cols) v[e] = src[sb + c * a.s_inner];
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t c = c0 + (int64_t)e * kBlock;
+      if (c < a.cols) dst[db + c * a.d_inner] = v[e];
+    }
+  }
+}
+
+template <typename E, int EPT>
+__global__ __launch_bounds__(kBlock) void k_fill_rows(E *__restrict__ dst, E value, RowArgs a) {
+  constexpr int64_t kTile = (int64_t)kBlock * EPT;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    const int64_t row = b / a.ntiles;
+    const int64_t tile = b - row * a.ntiles;
+    int64_t db, sb;
+    decode_row(a, row, &db, &sb);
+    const int64_t c0 = tile * kTile + threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t c = c0 + (int64_t)e * kBlock;
+      if (c < a.cols) dst[db + c * a.d_inner] = value;
+    }
+  }
+}
+
+// Narrow rows (cols small): one element per thread, grid-stride, full decode.
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_copy_flat(E *__restrict__ dst,
+                                                      const E *__restrict__ src, RowArgs a,
+                                                      int64_t total, int has_src, E value) {
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * kBlock) {
+    const int64_t row = g / a.cols;
+    const int64_t c = g - row * a.cols;
+    int64_t db, sb;
+    decode_row(a, row, &db, &sb);
+    dst[db + c * a.d_inner] = has_src ? src[sb + c * a.s_inner] : value;
+  }
+}
+
+// General path: descriptors with reshape/tile stages.  Element g of the
+// common logical shape is decoded into a coordinate and mapped through each
+// descriptor in full.
+__device__ __forceinline__ int64_t map_index(const ph_view_desc &d, const int64_t *coord) {
+  int64_t idx = d.view.offset;
+  for (int i = 0; i < d.view.rank; ++i) idx += coord[i] * d.view.stride[i];
+  for (int s = 0; s < d.nstage; ++s) {
+    const ph_affine &st = d.stage[s];
+    int64_t out = st.offset;
+    for (int j = st.rank - 1; j >= 0; --j) {
+      const int64_t n = st.shape[j];
+      const int64_t q = idx / n;
+      out += (idx - q * n) * st.stride[j];
+      idx = q;
+    }
+    idx = out;
+  }
+  return idx;
+}
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_copy_general(E *__restrict__ dst, ph_view_desc dd,
+                                                         const E *__restrict__ src,
+                                                         ph_view_desc sd, int64_t total,
+                                                         int has_src, E value) {
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * kBlock) {
+    int64_t coord[PH_MAX_RANK];
+    int64_t r = g;
+    for (int i = dd.view.rank - 1; i >= 0; --i) {
+      const int64_t n = dd.view.shape[i];
+      const int64_t q = r / n;
+      coord[i] = r - q * n;
+      r = q;
+    }
+    const int64_t di = map_index(dd, coord);
+    dst[di] = has_src ? src[map_index(sd, coord)] : value;
+  }
+}
+
+// One thread, lexicographic order: the aliasing-safe copy.
+template <typename E>
+__global__ void k_copy_sequential(E *dst, ph_view_desc dd, const E *src, ph_view_desc sd,
+                                  int64_t total) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t coord[PH_MAX_RANK];
+  for (int i = 0; i < dd.view.rank; ++i) coord[i] = 0;
+  for (int64_t g = 0; g < total; ++g) {
+    dst[map_index(dd, coord)] = src[map_index(sd, coord)];
+    for (int i = dd.view.rank - 1; i >= 0; --i) {
+      if (++coord[i] < dd.view.shape[i]) break;
+      coord[i] = 0;
+    }
+  }
+}
+
+template <typename E>
+int launch_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd) {
+  const int64_t total = ph::desc_count(dd);
+  if (total == 0) return PH_OK;
+  int rc = ph::launch_gate("sequential copy");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy_sequential<E>, dim3(1), dim3(64), 0, ph::stream(), (E *)dst, *dd,
+                     (const E *)src, *sd, total);
+  return ph::check_launch("k_copy_sequential");
+}
+
+int64_t grid_for(int64_t blocks) {
+  const int64_t cap = 1 << 20;
+  return blocks < 1 ? 1 : (blocks > cap ? cap : blocks);
+}
+
+void make_row_args(const ph::Layout &L, int d_op, int s_op, int ept, RowArgs *a) {
+  std::memset(a, 0, sizeof(*a));
+  if (L.rank == 0) {
+    a->rows = 1;
+    a->cols = 1;
+    a->d_off = L.offset[d_op];
+    a->s_off = s_op >= 0 ? L.offset[s_op] : 0;
+  } else {
+    a->cols = L.shape[L.rank - 1];
+    a->d_inner = L.stride[d_op][L.rank - 1];
+    a->s_inner = s_op >= 0 ? L.stride[s_op][L.rank - 1] : 0;
+    a->outer_rank = L.rank - 1;
+    a->rows = 1;
+    for (int k = 0; k < L.rank - 1; ++k) {
+      a->outer_shape[k] = L.shape[k];
+      a->rows *= L.shape[k];
+      a->d_outer[k] = L.stride[d_op][k];
+      a->s_outer[k] = s_op >= 0 ? L.stride[s_op][k] : 0;
+    }
+    a->d_off = L.offset[d_op];
+    a->s_off = s_op >= 0 ? L.offset[s_op] : 0;
+  }
+  const int64_t tile = (int64_t)kBlock * ept;
+  a->ntiles = (a->cols + tile - 1) / tile;
+  a->nblocks = a->rows * a->ntiles;
+}
+
+template <typename E>
+int launch_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
+                const void *value_bits) {
+  E value{};
+  if (value_bits) std::memcpy(&value, value_bits, sizeof(E));
+  const int64_t total = ph::desc_count(dd);
+  if (total == 0) return PH_OK;
+  int rc = ph::launch_gate("strided copy");
+  if (rc) return rc;
+  hipStream_t st = ph::stream();
+  const bool staged = dd->nstage > 0 || (src && sd->nstage > 0);
+  if (staged) {
+    const ph_view_desc &s_desc = src ? *sd : *dd;
+    hipLaunchKernelGGL(k_copy_general<E>, dim3((unsigned)grid_for((total + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, st, (E *)dst, *dd, (const E *)src, s_desc, total,
+                       src ? 1 : 0, value);
+    return ph::check_launch("k_copy_general");
+  }
+  const ph_view_desc *descs[2] = {dd, src ? sd : dd};
+  ph::Layout L;
+  ph::collapse(descs, 2, &L);
+  constexpr int EPT = sizeof(E) >= 8 ? 4 : 8;
+  RowArgs a;
+  make_row_args(L, 0, src ? 1 : -1, EPT, &a);
+  if (a.cols * 4 < (int64_t)kBlock * EPT && a.rows > 1) {
+    hipLaunchKernelGGL(k_copy_flat<E>, dim3((unsigned)grid_for((total + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, st, (E *)dst, (const E *)src, a, total, src ? 1 : 0,
+                       value);
+    return ph::check_launch("k_copy_flat");
+  }
+  if (src) {
+    hipLaunchKernelGGL((k_copy_rows<E, EPT>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock), 0,
+                       st, (E *)dst, (const E *)src, a);
+    return ph::check_launch("k_copy_rows");
+  }
+  hipLaunchKernelGGL((k_fill_rows<E, EPT>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock), 0,
+                     st, (E *)dst, value, a);
+  return ph::check_launch("k_fill_rows");
+}
+
+int dispatch_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
+                  const void *value) {
+  switch (ph::dtype_size(dd->dtype)) {
+    case 1: return launch_copy<uint8_t>(dst, dd, src, sd, value);
+    case 2: return launch_copy<uint16_t>(dst, dd, src, sd, value);
+    case 4: return launch_copy<uint32_t>(dst, dd, src, sd, value);
+    case 8: return launch_copy<uint64_t>(dst, dd, src, sd, value);
+  }
+  return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
+}
+
+// ---------------------------------------------------------------------------
+// Fused fill + scatter over a whole row-major destination buffer.
+// ---------------------------------------------------------------------------
+struct FillScatterArgs {
+  int32_t rank;        // destination rank
+  int32_t last_axis;   // view axis of the innermost destination axis, -1 fixed
+  int64_t cols;        // innermost destination extent
+  int64_t rows;        // product of the outer destination extents
+  int64_t ntiles, nblocks;
+  int64_t shape[PH_MAX_RANK];
+  int64_t first[PH_MAX_RANK];
+  int64_t step[PH_MAX_RANK];
+  int64_t count[PH_MAX_RANK];
+  int32_t axis[PH_MAX_RANK];
+  int64_t src_stride[PH_MAX_RANK];  // per view axis
+  int64_t src_off;
+  // innermost axis: member iff (c - first) = q*step, 0 <= q < count
+  int64_t in_first, in_step, in_count, in_src_stride;
+};
+
+template <typename E, int EPT>
+__global__ __launch_bounds__(kBlock) void k_fill_scatter(E *__restrict__ dst,
+                                                         const E *__restrict__ src,
+                                                         FillScatterArgs a, E fill) {
+  constexpr int64_t kTile = (int64_t)kBlock * EPT;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    const int64_t row = b / a.ntiles;
+    const int64_t tile = b - row * a.ntiles;
+    // decode the outer coordinate; decide row membership (uniform)
+    bool member = true;
+    int64_t soff = a.src_off;
+    int64_t r = row;
+    for (int k = a.rank - 2; k >= 0; --k) {
+      const int64_t n = a.shape[k];
+      const int64_t q = r / n;
+      const int64_t x = r - q * n;
+      r = q;
+      const int64_t rel = x - a.first[k];
+      const int64_t st = a.step[k];
+      if (a.axis[k] < 0) {
+        member = member && (rel == 0);
+      } else {
+        const int64_t j = rel / st;
+        member = member && (j * st == rel) && j >= 0 && j < a.count[k];
+        soff += j * a.src_stride[a.axis[k]];
+      }
+    }
+    const int64_t dbase = row * a.cols;
+    const int64_t c0 = tile * kTile + threadIdx.x;
+    if (!member) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int64_t c = c0 + (int64_t)e * kBlock;
+        if (c < a.cols) dst[dbase + c] = fill;
+      }
+      continue;
+    }
+    E v[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t c = c0 + (int64_t)e * kBlock;
+      v[e] = fill;
+      if (c < a.cols) {
+        const int64_t rel = c - a.in_first;
+        if (a.last_axis < 0) {
+          if (rel == 0) v[e] = src[soff];
+        } else {
+          const int64_t j = rel / a.in_step;
+          if (j * a.in_step == rel && j >= 0 && j < a.in_count) v[e] = src[soff + j * a.in_src_stride];
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t c = c0 + (int64_t)e * kBlock;
+      if (c < a.cols) dst[dbase + c] = v[e];
+    }
+  }
+}
+
+template <typename E>
+int launch_fill_scatter(void *dst, FillScatterArgs a, const void *src, const void *fill_bits) {
+  E fill{};
+  std::memcpy(&fill, fill_bits, sizeof(E));
+  constexpr int EPT = sizeof(E) >= 8 ? 4 : 8;
+  const int64_t tile = (int64_t)kBlock * EPT;
+  a.ntiles = (a.cols + tile - 1) / tile;
+  a.nblocks = a.rows * a.ntiles;
+  if (a.nblocks == 0) return PH_OK;
+  int rc = ph::launch_gate("fill scatter");
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_fill_scatter<E, EPT>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock),
+                     0, ph::stream(), (E *)dst, (const E *)src, a, fill);
+  return ph::check_launch("k_fill_scatter");
+}
+
+// ---------------------------------------------------------------------------
+// Seeded random fill (counter-based splitmix64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t draw(uint64_t seed, uint64_t k) {
+  return mix64(seed + (k + 1) * 0x9E3779B97F4A7C15ULL);
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ __launch_bounds__(kBlock) void k_random(void *buf, int dtype, int64_t n, uint64_t seed,
+                                                   uint64_t offset, int dist, double a, double b) {
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint64_t k = offset + (uint64_t)i;
+    double x = 0.0;
+    int64_t xi = 0;
+    if (dist == 0) {
+      const uint64_t d = draw(seed, k);
+      if (dtype == PH_I32) {
+        const int64_t span = (int64_t)(b - a);
+        xi = (int64_t)a + (int64_t)((d >> 11) % (uint64_t)span);
+      } else {
+        const double u = (double)(d >> 11) * 0x1.0p-53;
+        x = __dadd_rn(a, __dmul_rn(__dsub_rn(b, a), u));
+      }
+    } else if (dist == 1) {
+      const uint64_t kp = k & ~1ULL;
+      const double u1 = (double)(draw(seed, kp) >> 11) * 0x1.0p-53;
+      const double u2 = (double)(draw(seed, kp + 1) >> 11) * 0x1.0p-53;
+      const double r = sqrt(-2.0 * log(1.0 - u1));
+      const double t = __dmul_rn(two_pi, u2);
+      const double z = (k & 1) ? __dmul_rn(r, sin(t)) : __dmul_rn(r, cos(t));
+      x = __dadd_rn(a, __dmul_rn(b, z));
+    } else {
+      xi = (int64_t)(draw(seed, k) >> 33);
+    }
+    switch (dtype) {
+      case PH_I32: ((int32_t *)buf)[i] = (int32_t)xi; break;
+      case PH_F32: ((float *)buf)[i] = (float)x; break;
+      case PH_F64: ((double *)buf)[i] = x; break;
+      case PH_BF16: ((uint16_t *)buf)[i] = f32_to_bf16_rne((float)x); break;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ph_fill_random(void *buf, int32_t dtype, int64_t n, uint64_t seed, uint64_t offset,
+                   int32_t dist, double a, double b) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (n <= 0) return PH_OK;
+  if (dtype == PH_BOOL || ph::dtype_size(dtype) == 0)
+    return ph::set_error(PH_E_ARGUMENT, "ph_fill_random: unsupported dtype %d", dtype);
+  if (dist < 0 || dist > 2) return ph::set_error(PH_E_ARGUMENT, "ph_fill_random: dist %d", dist);
+  if (dist == 0 && dtype == PH_I32 && !(b > a))
+    return ph::set_error(PH_E_ARGUMENT, "ph_fill_random: empty integer range");
+  int rc = ph::launch_gate("random");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_random, dim3((unsigned)grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, ph::stream(), buf, dtype, n, seed, offset, dist, a, b);
+  return ph::check_launch("k_random");
+}
+
+int ph_fill(void *buf, const ph_view_desc *dst, const void *value) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  int rc = ph::validate_desc(dst, "fill destination");
+  if (rc) return rc;
+  if (!value) return ph::set_error(PH_E_ARGUMENT, "ph_fill: null value");
+  return dispatch_copy(buf, dst, nullptr, nullptr, value);
+}
+
+int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
+                    int32_t flags) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  int rc = ph::validate_desc(dd, "copy destination");
+  if (rc) return rc;
+  rc = ph::validate_desc(sd, "copy source");
+  if (rc) return rc;
+  if (dd->dtype != sd->dtype && ph::dtype_size(dd->dtype) != ph::dtype_size(sd->dtype))
+    return ph::set_error(PH_E_ARGUMENT, "copy between dtypes %d and %d", sd->dtype, dd->dtype);
+  if (dd->view.rank != sd->view.rank)
+    return ph::set_error(PH_E_SHAPE, "copy between ranks %d and %d", sd->view.rank, dd->view.rank);
+  for (int i = 0; i < dd->view.rank; ++i)
+    if (dd->view.shape[i] != sd->view.shape[i])
+      return ph::set_error(PH_E_SHAPE, "copy shapes differ on axis %d (%lld vs %lld)", i,
+                           (long long)sd->view.shape[i], (long long)dd->view.shape[i]);
+  if (flags & PH_COPY_ZERO_REST)
+    return ph::set_error(PH_E_UNSUPPORTED,
+                         "PH_COPY_ZERO_REST needs the region form: use ph_fill_scatter");
+  return dispatch_copy(dst, dd, src, sd, nullptr);
+}
+
+int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd, const void *src,
+                               const ph_view_desc *sd) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  int rc = ph::validate_desc(dd, "copy destination");
+  if (rc) return rc;
+  rc = ph::validate_desc(sd, "copy source");
+  if (rc) return rc;
+  if (ph::dtype_size(dd->dtype) != ph::dtype_size(sd->dtype))
+    return ph::set_error(PH_E_ARGUMENT, "copy between dtypes %d and %d", sd->dtype, dd->dtype);
+  if (dd->view.rank != sd->view.rank)
+    return ph::set_error(PH_E_SHAPE, "copy between ranks %d and %d", sd->view.rank, dd->view.rank);
+  for (int i = 0; i < dd->view.rank; ++i)
+    if (dd->view.shape[i] != sd->view.shape[i])
+      return ph::set_error(PH_E_SHAPE, "copy shapes differ on axis %d", i);
+  switch (ph::dtype_size(dd->dtype)) {
+    case 1: return launch_sequential<uint8_t>(dst, dd, src, sd);
+    case 2: return launch_sequential<uint16_t>(dst, dd, src, sd);
+    case 4: return launch_sequential<uint32_t>(dst, dd, src, sd);
+    case 8: return launch_sequential<uint64_t>(dst, dd, src, sd);
+  }
+  return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
+}
+
+int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape, int32_t rank,
+                    const int64_t *first, const int64_t *step, const int64_t *count,
+                    const int32_t *axis, const void *src, const ph_view_desc *sd,
+                    const void *fill) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (rank < 1 || rank > PH_MAX_RANK)
+    return ph::set_error(PH_E_DIMENSION, "ph_fill_scatter: rank %d", rank);
+  int rc = ph::validate_desc(sd, "fill_scatter source");
+  if (rc) return rc;
+  if (sd->nstage != 0)
+    return ph::set_error(PH_E_UNSUPPORTED, "ph_fill_scatter: source must be affine");
+  if (ph::dtype_size(dtype) != ph::dtype_size(sd->dtype))
+    return ph::set_error(PH_E_ARGUMENT, "ph_fill_scatter: dtype mismatch");
+  FillScatterArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.rank = rank;
+  a.rows = 1;
+  int used[PH_MAX_RANK] = {0};
+  for (int k = 0; k < rank; ++k) {
+    if (dst_shape[k] < 0) return ph::set_error(PH_E_DIMENSION, "negative extent");
+    a.shape[k] = dst_shape[k];
+    a.first[k] = first[k];
+    a.step[k] = step[k];
+    a.count[k] = count[k];
+    a.axis[k] = axis[k];
+    if (axis[k] >= 0) {
+      if (axis[k] >= sd->view.rank || used[axis[k]])
+        return ph::set_error(PH_E_ARGUMENT, "ph_fill_scatter: bad view axis %d", axis[k]);
+      used[axis[k]] = 1;
+      if (count[k] != sd->view.shape[axis[k]])
+        return ph::set_error(PH_E_SHAPE, "ph_fill_scatter: axis %d count %lld vs source %lld", k,
+                             (long long)count[k], (long long)sd->view.shape[axis[k]]);
+      if (count[k] > 0) {
+        if (step[k] == 0) return ph::set_error(PH_E_ARGUMENT, "ph_fill_scatter: zero step");
+        const int64_t lastc = first[k] + (count[k] - 1) * step[k];
+        if (first[k] < 0 || first[k] >= dst_shape[k] || lastc < 0 || lastc >= dst_shape[k])
+          return ph::set_error(PH_E_INDEX, "ph_fill_scatter: region leaves axis %d", k);
+      }
+    } else if (first[k] < 0 || first[k] >= dst_shape[k]) {
+      if (dst_shape[k] > 0) return ph::set_error(PH_E_INDEX, "ph_fill_scatter: fixed index outside axis %d", k);
+    }
+    if (k < rank - 1) a.rows *= dst_shape[k];
+  }
+  for (int i = 0; i < sd->view.rank; ++i) {
+    if (!used[i] && sd->view.shape[i] != 1)
+      return ph::set_error(PH_E_SHAPE, "ph_fill_scatter: source axis %d not mapped", i);
+    a.src_stride[i] = sd->view.stride[i];
+  }
+  a.src_off = sd->view.offset;
+  a.cols = dst_shape[rank - 1];
+  a.last_axis = axis[rank - 1];
+  a.in_first = first[rank - 1];
+  a.in_step = step[rank - 1] == 0 ? 1 : step[rank - 1];
+  a.in_count = count[rank - 1];
+  a.in_src_stride = axis[rank - 1] >= 0 ? sd->view.stride[axis[rank - 1]] : 0;
+  switch (ph::dtype_size(dtype)) {
+    case 1: return launch_fill_scatter<uint8_t>(dst, a, src, fill);
+    case 2: return launch_fill_scatter<uint16_t>(dst, a, src, fill);
+    case 4: return launch_fill_scatter<uint32_t>(dst, a, src, fill);
+    case 8: return launch_fill_scatter<uint64_t>(dst, a, src, fill);
+  }
+  return ph::set_error(PH_E_ARGUMENT, "ph_fill_scatter: dtype %d", dtype);
+}
+
+}  // extern "C"

@@ -1,0 +1,1028 @@
+// Fused elementwise programs: ph_map, ph_reduce, ph_reduce2d_all.
+//
+// A ph_prog (the compiled form of an elementwise expression or a ProcView
+// block) is turned into HIP source with every operand's collapsed shape and
+// strides baked in as constants, compiled once with hipRTC (jit.cpp) and
+// launched on the library stream.  Each instruction becomes one typed C++
+// statement, so the kernel does exactly the roundings the program asks for
+// (compiled with -ffp-contract=off: no fused multiply-adds).
+//
+// Map kernel: `rows x cols` after collapsing; a 256-thread workgroup owns
+// 256*EPT consecutive columns of one row, lane t handles columns
+// tile0 + e*256 + t (coalesced for every unrolled e); all loads of a tile
+// are issued before any arithmetic.
+//
+// Reduction kernel (2-D view rows x cols): a workgroup owns TR rows x 1024
+// columns; wavefront w owns columns [256w, 256w+256), lane l owns VEC=4
+// consecutive columns.  Per element the program's value feeds (a) the
+// thread's column states, accumulated row after row, and (b) a per-row lane
+// state that an ordered shuffle tree folds into one state per wavefront and
+// row.  Second pass: per-row and per-column states are folded across tiles in
+// tile order, then the full result across rows in row order.  Every fold is
+// in lexicographic order, so Int32 running-sum overflow is detected exactly as
+// a sequential checked sum would, and every result is deterministic.
+#include <cinttypes>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "jit.hpp"
+
+namespace {
+
+const char *ctype(int dt) {
+  switch (dt) {
+    case PH_I32: return "int";
+    case PH_F32: return "float";
+    case PH_F64: return "double";
+    case PH_BOOL: return "bool";
+  }
+  return "void";
+}
+
+const char *stype(int dt) {  // storage type
+  switch (dt) {
+    case PH_I32: return "int";
+    case PH_F32: return "float";
+    case PH_F64: return "double";
+    case PH_BF16: return "unsigned short";
+    case PH_BOOL: return "unsigned char";
+  }
+  return "void";
+}
+
+int compute_dt(int storage) { return storage == PH_BF16 ? PH_F32 : storage; }
+
+std::string fmt(const char *f, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, f);
+  std::vsnprintf(buf, sizeof(buf), f, ap);
+  va_end(ap);
+  return buf;
+}
+
+std::string lit_i64(int64_t v) {
+  if (v == INT64_MIN) return "(-9223372036854775807LL-1)";
+  return fmt("%" PRId64 "LL", v);
+}
+
+std::string const_expr(int dt, const ph_insn &in) {
+  switch (dt) {
+    case PH_I32: {
+      int32_t v = (int32_t)in.imm.i;
+      if (v == INT32_MIN) return "(-2147483647-1)";
+      return fmt("%d", v);
+    }
+    case PH_F32: return fmt("((float)%a)", (double)(float)in.imm.f);
+    case PH_F64: return fmt("(%a)", in.imm.f);
+    case PH_BOOL: return in.imm.i ? "true" : "false";
+  }
+  return "0";
+}
+
+bool is_float(int dt) { return dt == PH_F32 || dt == PH_F64; }
+
+// Validates `p` against the input storage dtypes and checks every operand's
+// dtype; fills reg dtypes on the way.
+int validate_prog(const ph_prog *p, int n_in, const int *in_dt) {
+  if (!p) return ph::set_error(PH_E_ARGUMENT, "null program");
+  if (p->n_insn < 1 || p->n_insn > PH_MAX_INSN)
+    return ph::set_error(PH_E_ARGUMENT, "program length %d", p->n_insn);
+  if (p->n_in != n_in) return ph::set_error(PH_E_ARGUMENT, "program expects %d inputs, got %d", p->n_in, n_in);
+  int rdt[PH_NREG];
+  for (int i = 0; i < PH_NREG; ++i) rdt[i] = -1;
+  for (int k = 0; k < p->n_insn; ++k) {
+    const ph_insn &in = p->insn[k];
+    if (in.dst >= PH_NREG) return ph::set_error(PH_E_ARGUMENT, "insn %d: register %d", k, in.dst);
+    const int dt = in.dt;
+    if (dt != PH_I32 && dt != PH_F32 && dt != PH_F64 && dt != PH_BOOL)
+      return ph::set_error(PH_E_ARGUMENT, "insn %d: dtype %d", k, dt);
+    auto reg = [&](int r) { return (r < PH_NREG) ? rdt[r] : -2; };
+    int out_dt = dt;
+    switch (in.op) {
+      case PH_OP_LOAD:
+        if (in.a >= n_in) return ph::set_error(PH_E_ARGUMENT, "insn %d: input %d", k, in.a);
+        if (compute_dt(in_dt[in.a]) != dt)
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: input %d loads as %d, not %d", k, in.a,
+                               compute_dt(in_dt[in.a]), dt);
+        break;
+      case PH_OP_CONST: break;
+      case PH_OP_CAST:
+        if (reg(in.a) != in.c) return ph::set_error(PH_E_ARGUMENT, "insn %d: cast source dtype", k);
+        break;
+      case PH_OP_ADD: case PH_OP_SUB: case PH_OP_MUL:
+        if (dt == PH_BOOL) return ph::set_error(PH_E_ARGUMENT, "insn %d: arithmetic on Bool", k);
+        if (reg(in.a) != dt || reg(in.b) != dt) return ph::set_error(PH_E_ARGUMENT, "insn %d: operand dtype", k);
+        break;
+      case PH_OP_WADD: case PH_OP_WSUB: case PH_OP_WMUL: case PH_OP_IPOW: case PH_OP_WIPOW:
+      case PH_OP_BNOT:
+        if (dt != PH_I32) return ph::set_error(PH_E_ARGUMENT, "insn %d: Int32-only op", k);
+        if (reg(in.a) != dt || (in.op != PH_OP_BNOT && reg(in.b) != dt))
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: operand dtype", k);
+        break;
+      case PH_OP_DIV: case PH_OP_POW: case PH_OP_SQRT:
+        if (!is_float(dt)) return ph::set_error(PH_E_ARGUMENT, "insn %d: float-only op", k);
+        if (reg(in.a) != dt || (in.op != PH_OP_SQRT && reg(in.b) != dt))
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: operand dtype", k);
+        break;
+      case PH_OP_FLOORDIV: case PH_OP_MOD:
+        if (dt == PH_BOOL) return ph::set_error(PH_E_ARGUMENT, "insn %d: // or %% on Bool", k);
+        if (reg(in.a) != dt || reg(in.b) != dt) return ph::set_error(PH_E_ARGUMENT, "insn %d: operand dtype", k);
+        break;
+      case PH_OP_POWI:
+        if (!is_float(dt) || reg(in.a) != dt || reg(in.b) != PH_I32)
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: powi operands", k);
+        break;
+      case PH_OP_NEG: case PH_OP_ABS:
+        if (dt == PH_BOOL || reg(in.a) != dt) return ph::set_error(PH_E_ARGUMENT, "insn %d: operand dtype", k);
+        break;
+      case PH_OP_AND: case PH_OP_OR: case PH_OP_XOR:
+        if (!(dt == PH_I32 || dt == PH_BOOL) || reg(in.a) != dt || reg(in.b) != dt)
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: bitwise operands", k);
+        break;
+      case PH_OP_GT: case PH_OP_LT: case PH_OP_GE: case PH_OP_LE: case PH_OP_EQ: case PH_OP_NE:
+        if (reg(in.a) != dt || reg(in.b) != dt) return ph::set_error(PH_E_ARGUMENT, "insn %d: comparison operands", k);
+        out_dt = PH_BOOL;
+        break;
+      case PH_OP_CMP:
+        if (reg(in.a) != dt || reg(in.b) != dt) return ph::set_error(PH_E_ARGUMENT, "insn %d: <=> operands", k);
+        out_dt = PH_I32;
+        break;
+      case PH_OP_SELECT:
+        if (reg(in.c) != PH_BOOL || reg(in.a) != dt || reg(in.b) != dt)
+          return ph::set_error(PH_E_ARGUMENT, "insn %d: select operands", k);
+        break;
+      default:
+        return ph::set_error(PH_E_ARGUMENT, "insn %d: opcode %d", k, in.op);
+    }
+    rdt[in.dst] = out_dt;
+  }
+  if (p->out_reg < 0 || p->out_reg >= PH_NREG || rdt[p->out_reg] < 0)
+    return ph::set_error(PH_E_ARGUMENT, "program result register %d undefined", p->out_reg);
+  if (rdt[p->out_reg] != p->out_dtype)
+    return ph::set_error(PH_E_ARGUMENT, "program result dtype %d, declared %d", rdt[p->out_reg], p->out_dtype);
+  return PH_OK;
+}
+
+// Emits the statements computing the program for element `e`; inputs are
+// the expressions `inexpr[j]` (already of their compute type).  Returns the
+// name of the result variable.
+std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, std::string *code,
+                      const std::string &sfx) {
+  std::string regv[PH_NREG];
+  for (int k = 0; k < p->n_insn; ++k) {
+    const ph_insn &in = p->insn[k];
+    const int dt = in.dt;
+    const std::string v = fmt("v%d%s", k, sfx.c_str());
+    const std::string A = in.a < PH_NREG ? regv[in.a] : "";
+    const std::string B = in.b < PH_NREG ? regv[in.b] : "";
+    const std::string C = in.c < PH_NREG ? regv[in.c] : "";
+    const char *T = ctype(dt);
+    std::string s;
+    switch (in.op) {
+      case PH_OP_LOAD: s = fmt("const %s %s = %s;", T, v.c_str(), inexpr[in.a].c_str()); break;
+      case PH_OP_CONST: s = fmt("const %s %s = %s;", T, v.c_str(), const_expr(dt, in).c_str()); break;
+      case PH_OP_CAST: s = fmt("const %s %s = (%s)(%s);", T, v.c_str(), T, A.c_str()); break;
+      case PH_OP_ADD: case PH_OP_SUB: case PH_OP_MUL: {
+        const char *o = in.op == PH_OP_ADD ? "+" : in.op == PH_OP_SUB ? "-" : "*";
+        if (dt == PH_I32)
+          s = fmt("int %s; { const long long t_ = (long long)%s %s (long long)%s; fl |= (t_ != (long long)(int)t_) ? 1 : 0; %s = (int)t_; }",
+                  v.c_str(), A.c_str(), o, B.c_str(), v.c_str());
+        else
+          s = fmt("const %s %s = %s %s %s;", T, v.c_str(), A.c_str(), o, B.c_str());
+        break;
+      }
+      case PH_OP_WADD: case PH_OP_WSUB: case PH_OP_WMUL: {
+        const char *o = in.op == PH_OP_WADD ? "+" : in.op == PH_OP_WSUB ? "-" : "*";
+        s = fmt("const int %s = (int)((unsigned)%s %s (unsigned)%s);", v.c_str(), A.c_str(), o, B.c_str());
+        break;
+      }
+      case PH_OP_DIV: s = fmt("const %s %s = %s / %s;", T, v.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_FLOORDIV:
+        if (dt == PH_I32)
+          s = fmt("const int %s = ph_ifloordiv(%s, %s, fl);", v.c_str(), A.c_str(), B.c_str());
+        else
+          s = fmt("const %s %s = ph_floor(%s / %s);", T, v.c_str(), A.c_str(), B.c_str());
+        break;
+      case PH_OP_MOD:
+        if (dt == PH_I32)
+          s = fmt("const int %s = ph_imod(%s, %s, fl);", v.c_str(), A.c_str(), B.c_str());
+        else
+          s = fmt("const %s %s = ph_fmod_floor(%s, %s);", T, v.c_str(), A.c_str(), B.c_str());
+        break;
+      case PH_OP_POWI: s = fmt("const %s %s = ph_powi(%s, %s);", T, v.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_IPOW: s = fmt("const int %s = ph_ipow(%s, %s, fl, true);", v.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_WIPOW: s = fmt("const int %s = ph_ipow(%s, %s, fl, false);", v.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_POW: s = fmt("const %s %s = ph_pow(%s, %s);", T, v.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_NEG:
+        if (dt == PH_I32)
+          s = fmt("const int %s = ph_ineg(%s, fl);", v.c_str(), A.c_str());
+        else
+          s = fmt("const %s %s = -%s;", T, v.c_str(), A.c_str());
+        break;
+      case PH_OP_BNOT: s = fmt("const int %s = ~%s;", v.c_str(), A.c_str()); break;
+      case PH_OP_AND: case PH_OP_OR: case PH_OP_XOR: {
+        const char *o = in.op == PH_OP_AND ? "&" : in.op == PH_OP_OR ? "|" : "^";
+        if (dt == PH_BOOL)
+          s = fmt("const bool %s = (bool)((int)%s %s (int)%s);", v.c_str(), A.c_str(), o, B.c_str());
+        else
+          s = fmt("const int %s = %s %s %s;", v.c_str(), A.c_str(), o, B.c_str());
+        break;
+      }
+      case PH_OP_GT: case PH_OP_LT: case PH_OP_GE: case PH_OP_LE: case PH_OP_EQ: case PH_OP_NE: {
+        static const char *ops[] = {">", "<", ">=", "<=", "==", "!="};
+        s = fmt("const bool %s = %s %s %s;", v.c_str(), A.c_str(), ops[in.op - PH_OP_GT], B.c_str());
+        break;
+      }
+      case PH_OP_CMP: s = fmt("const int %s = (int)(%s > %s) - (int)(%s < %s);", v.c_str(), A.c_str(), B.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_SQRT: s = fmt("const %s %s = ph_sqrt(%s);", T, v.c_str(), A.c_str()); break;
+      case PH_OP_ABS:
+        if (dt == PH_I32)
+          s = fmt("const int %s = ph_iabs(%s, fl);", v.c_str(), A.c_str());
+        else
+          s = fmt("const %s %s = ph_fabs(%s);", T, v.c_str(), A.c_str());
+        break;
+      case PH_OP_SELECT: s = fmt("const %s %s = %s ? %s : %s;", T, v.c_str(), C.c_str(), A.c_str(), B.c_str()); break;
+    }
+    *code += "        " + s + "\n";
+    regv[in.dst] = v;
+  }
+  return regv[p->out_reg];
+}
+
+const char *kPrelude = R"PH(
+typedef long long i64;
+__device__ __forceinline__ float ph_ld(const unsigned short *p, i64 i) { return __uint_as_float(((unsigned)p[i]) << 16); }
+__device__ __forceinline__ bool ph_ld(const unsigned char *p, i64 i) { return p[i] != 0; }
+__device__ __forceinline__ int ph_ld(const int *p, i64 i) { return p[i]; }
+__device__ __forceinline__ float ph_ld(const float *p, i64 i) { return p[i]; }
+__device__ __forceinline__ double ph_ld(const double *p, i64 i) { return p[i]; }
+__device__ __forceinline__ void ph_st(unsigned char *p, i64 i, bool v) { p[i] = v ? 1 : 0; }
+__device__ __forceinline__ void ph_st(int *p, i64 i, int v) { p[i] = v; }
+__device__ __forceinline__ void ph_st(float *p, i64 i, float v) { p[i] = v; }
+__device__ __forceinline__ void ph_st(double *p, i64 i, double v) { p[i] = v; }
+__device__ __forceinline__ float ph_floor(float x) { return __builtin_floorf(x); }
+__device__ __forceinline__ double ph_floor(double x) { return __builtin_floor(x); }
+__device__ __forceinline__ float ph_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double ph_sqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float ph_fabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double ph_fabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float ph_pow(float a, float b) { return __builtin_powf(a, b); }
+__device__ __forceinline__ double ph_pow(double a, double b) { return __builtin_pow(a, b); }
+__device__ __forceinline__ float ph_fmod_floor(float a, float b) { const float q = __builtin_floorf(a / b); const float m = b * q; return a - m; }
+__device__ __forceinline__ double ph_fmod_floor(double a, double b) { const double q = __builtin_floor(a / b); const double m = b * q; return a - m; }
+template <typename F> __device__ __forceinline__ F ph_powi(F a, int b) {
+  const bool recip = b < 0; long long e = b; if (e < 0) e = -e; F r = (F)1;
+  for (;;) { if (e & 1) r *= a; e /= 2; if (e == 0) break; a *= a; }
+  return recip ? (F)1 / r : r;
+}
+__device__ __forceinline__ int ph_ipow(int base, int e, int &fl, bool checked) {
+  if (e < 0) { fl |= 4; return 0; }
+  long long result = 1, k = base; long long ex = e;
+  while (ex > 0) {
+    if (ex & 1) { const long long t = result * k; if (checked && t != (long long)(int)t) fl |= 1; result = (int)t; }
+    ex >>= 1;
+    if (ex > 0) { const long long t = k * k; if (checked && t != (long long)(int)t) fl |= 1; k = (int)t; }
+  }
+  return (int)result;
+}
+__device__ __forceinline__ int ph_ifloordiv(int a, int b, int &fl) {
+  if (b == 0) { fl |= 2; return 0; }
+  if (a == (-2147483647 - 1) && b == -1) { fl |= 1; return a; }
+  int q = a / b; if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1; return q;
+}
+__device__ __forceinline__ int ph_imod(int a, int b, int &fl) {
+  if (b == 0) { fl |= 2; return 0; }
+  if (b == -1) return 0;
+  int m = a % b; if (m != 0 && ((m < 0) != (b < 0))) m += b; return m;
+}
+__device__ __forceinline__ int ph_ineg(int a, int &fl) { if (a == (-2147483647 - 1)) fl |= 1; return (int)(0u - (unsigned)a); }
+__device__ __forceinline__ int ph_iabs(int a, int &fl) { if (a == (-2147483647 - 1)) fl |= 1; return a < 0 ? (int)(0u - (unsigned)a) : a; }
+)PH";
+
+// ---------------------------------------------------------------------------
+// Map
+// ---------------------------------------------------------------------------
+struct MapArgs {
+  const void *in[PH_MAX_INPUTS];
+  int64_t in_off[PH_MAX_INPUTS];
+  void *out;
+  int64_t out_off;
+  int *flags;
+};
+
+constexpr int kBlock = 256;
+
+std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::Layout &L,
+                       int out_dt, int ept, bool flat, int64_t *nblocks) {
+  // operands 0..n_in-1 inputs, n_in output
+  const int outer = L.rank > 0 ? L.rank - 1 : 0;
+  const int64_t cols = L.rank > 0 ? L.shape[L.rank - 1] : 1;
+  int64_t rows = 1;
+  for (int k = 0; k < outer; ++k) rows *= L.shape[k];
+  const int64_t tile = (int64_t)kBlock * ept;
+  std::string s = kPrelude;
+  s += fmt("struct MapArgs { const void *in[%d]; i64 in_off[%d]; void *out; i64 out_off; int *flags; };\n",
+           PH_MAX_INPUTS, PH_MAX_INPUTS);
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_map_kernel(MapArgs A) {\n";
+  s += "  int fl = 0;\n";
+  auto inner_stride = [&](int op) { return L.rank > 0 ? L.stride[op][L.rank - 1] : 0; };
+  auto decode = [&](const char *rowvar) {
+    std::string d;
+    d += fmt("    i64 rr = %s;\n", rowvar);
+    for (int o = 0; o <= n_in; ++o)
+      d += fmt("    i64 base%d = %s;\n", o, o < n_in ? fmt("A.in_off[%d]", o).c_str() : "A.out_off");
+    for (int k = outer - 1; k >= 0; --k) {
+      d += fmt("    { const i64 q = rr / %s; const i64 c = rr - q * %s; rr = q;\n", lit_i64(L.shape[k]).c_str(),
+               lit_i64(L.shape[k]).c_str());
+      for (int o = 0; o <= n_in; ++o)
+        if (L.stride[o][k] != 0) d += fmt("      base%d += c * %s;\n", o, lit_i64(L.stride[o][k]).c_str());
+      d += "    }\n";
+    }
+    return d;
+  };
+  std::vector<std::string> inexpr;
+  for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[e]", j));
+  std::string body;
+  std::string res = emit_prog(p, inexpr, &body, "");
+  if (!flat) {
+    const int64_t ntiles = (cols + tile - 1) / tile;
+    *nblocks = rows * ntiles;
+    s += fmt("  for (i64 b = blockIdx.x; b < %s; b += gridDim.x) {\n", lit_i64(*nblocks).c_str());
+    s += fmt("    const i64 row = b / %s; const i64 tile = b - row * %s;\n", lit_i64(ntiles).c_str(),
+             lit_i64(ntiles).c_str());
+    s += decode("row");
+    s += fmt("    const i64 c0 = tile * %s + threadIdx.x;\n", lit_i64(tile).c_str());
+    for (int j = 0; j < n_in; ++j) s += fmt("    %s x%d[%d];\n", ctype(compute_dt(in_dt[j])), j, ept);
+    s += fmt("#pragma unroll\n    for (int e = 0; e < %d; ++e) {\n", ept);
+    s += fmt("      const i64 c = c0 + (i64)e * 256;\n      if (c < %s) {\n", lit_i64(cols).c_str());
+    for (int j = 0; j < n_in; ++j)
+      s += fmt("        x%d[e] = ph_ld((const %s *)A.in[%d], base%d + c * %s);\n", j, stype(in_dt[j]), j, j,
+               lit_i64(inner_stride(j)).c_str());
+    s += "      }\n    }\n";
+    s += fmt("#pragma unroll\n    for (int e = 0; e < %d; ++e) {\n", ept);
+    s += fmt("      const i64 c = c0 + (i64)e * 256;\n      if (c < %s) {\n", lit_i64(cols).c_str());
+    s += body;
+    s += fmt("        ph_st((%s *)A.out, base%d + c * %s, %s);\n", stype(out_dt), n_in,
+             lit_i64(inner_stride(n_in)).c_str(), res.c_str());
+    s += "      }\n    }\n  }\n";
+  } else {
+    const int64_t total = rows * cols;
+    *nblocks = (total + tile - 1) / tile;
+    s += fmt("  for (i64 b = blockIdx.x; b < %s; b += gridDim.x) {\n", lit_i64(*nblocks).c_str());
+    s += fmt("#pragma unroll\n    for (int e = 0; e < %d; ++e) {\n", ept);
+    s += fmt("      const i64 g = b * %s + (i64)e * 256 + threadIdx.x;\n", lit_i64(tile).c_str());
+    s += fmt("      if (g < %s) {\n", lit_i64(total).c_str());
+    s += fmt("        const i64 row = g / %s; const i64 c = g - row * %s;\n", lit_i64(cols).c_str(),
+             lit_i64(cols).c_str());
+    std::string d = decode("row");
+    s += d;
+    for (int j = 0; j < n_in; ++j)
+      s += fmt("        %s x%d[1]; x%d[0] = ph_ld((const %s *)A.in[%d], base%d + c * %s);\n",
+               ctype(compute_dt(in_dt[j])), j, j, stype(in_dt[j]), j, j, lit_i64(inner_stride(j)).c_str());
+    s += "        const int e0 = 0; (void)e0;\n";
+    std::string body0;
+    std::vector<std::string> inexpr0;
+    for (int j = 0; j < n_in; ++j) inexpr0.push_back(fmt("x%d[0]", j));
+    std::string res0 = emit_prog(p, inexpr0, &body0, "");
+    s += body0;
+    s += fmt("        ph_st((%s *)A.out, base%d + c * %s, %s);\n", stype(out_dt), n_in,
+             lit_i64(inner_stride(n_in)).c_str(), res0.c_str());
+    s += "      }\n    }\n  }\n";
+  }
+  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  return s;
+}
+
+int64_t grid_for(int64_t blocks) {
+  const int64_t cap = 1 << 20;
+  return blocks < 1 ? 1 : (blocks > cap ? cap : blocks);
+}
+
+int get_flags_buffer(int **dflags) {
+  void *p;
+  int rc = ph::scratch(64, &p, 3);
+  if (rc) return rc;
+  *dflags = (int *)p;
+  return PH_OK;
+}
+
+int read_flags(int *dflags, int32_t *flags_out) {
+  if (!flags_out) return PH_OK;
+  int v = 0;
+  PH_HIP(hipMemcpyAsync(&v, dflags, sizeof(int), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  *flags_out = v;
+  return PH_OK;
+}
+
+// Materialises staged (reshape / tile) inputs into contiguous scratch so the
+// fused kernels only see affine operands.
+int materialize(const ph_view_desc *d, const void *src, int slot, ph_view_desc *nd, const void **np) {
+  const int64_t n = ph::desc_count(d);
+  void *buf;
+  int rc = ph::scratch((size_t)(n > 0 ? n : 1) * ph::dtype_size(d->dtype), &buf, slot);
+  if (rc) return rc;
+  std::memset(nd, 0, sizeof(*nd));
+  nd->dtype = d->dtype;
+  nd->extent = n > 0 ? n : 1;
+  nd->view.rank = d->view.rank;
+  int64_t st = 1;
+  for (int i = d->view.rank - 1; i >= 0; --i) {
+    nd->view.shape[i] = d->view.shape[i];
+    nd->view.stride[i] = st;
+    st *= d->view.shape[i];
+  }
+  rc = ph_strided_copy(buf, nd, src, d, 0);
+  if (rc) return rc;
+  *np = buf;
+  return PH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+const char *kRedPrelude = R"PH(
+struct SF { float s, c, m; };
+struct SD { double s, c, m; };
+struct SI { i64 s, hi, lo; int m; int n; };
+__device__ __forceinline__ void st_init(SF &a) { a.s = 0.f; a.c = 0.f; a.m = -__builtin_inff(); }
+__device__ __forceinline__ void st_init(SD &a) { a.s = 0.0; a.c = 0.0; a.m = -__builtin_inf(); }
+__device__ __forceinline__ void st_init(SI &a) { a.s = 0; a.hi = -(1LL << 62); a.lo = (1LL << 62); a.m = 0; a.n = 0; }
+template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, F v) {
+  const F t = a.s + v;
+  const F d = (__builtin_fabs((double)a.s) >= __builtin_fabs((double)v)) ? ((a.s - t) + v) : ((v - t) + a.s);
+  a.c += d; a.s = t; a.m = (v > a.m) ? v : a.m;
+}
+__device__ __forceinline__ void st_add(SF &a, float v) { st_addf<SF, float>(a, v); }
+__device__ __forceinline__ void st_add(SD &a, double v) { st_addf<SD, double>(a, v); }
+__device__ __forceinline__ void st_add(SI &a, i64 v) {
+  a.s += v; a.hi = a.s > a.hi ? a.s : a.hi; a.lo = a.s < a.lo ? a.s : a.lo;
+  a.m = (a.n == 0 || (int)v > a.m) ? (int)v : a.m; a.n = 1;
+}
+template <typename S> __device__ __forceinline__ S st_combf(const S &a, const S &b) {
+  S r; const auto t = a.s + b.s; const auto bb = t - a.s; const auto err = (a.s - (t - bb)) + (b.s - bb);
+  r.s = t; r.c = (a.c + b.c) + err; r.m = (b.m > a.m) ? b.m : a.m; return r;
+}
+__device__ __forceinline__ SF st_comb(const SF &a, const SF &b) { return st_combf<SF>(a, b); }
+__device__ __forceinline__ SD st_comb(const SD &a, const SD &b) { return st_combf<SD>(a, b); }
+__device__ __forceinline__ SI st_comb(const SI &a, const SI &b) {
+  if (!b.n) return a; if (!a.n) return b;
+  SI r; r.s = a.s + b.s; const i64 h = a.s + b.hi, l = a.s + b.lo;
+  r.hi = a.hi > h ? a.hi : h; r.lo = a.lo < l ? a.lo : l; r.m = b.m > a.m ? b.m : a.m; r.n = 1; return r;
+}
+__device__ __forceinline__ SF st_shfl_down(const SF &a, int off) { SF r; r.s = __shfl_down(a.s, off, 64); r.c = __shfl_down(a.c, off, 64); r.m = __shfl_down(a.m, off, 64); return r; }
+__device__ __forceinline__ SD st_shfl_down(const SD &a, int off) { SD r; r.s = __shfl_down(a.s, off, 64); r.c = __shfl_down(a.c, off, 64); r.m = __shfl_down(a.m, off, 64); return r; }
+__device__ __forceinline__ SI st_shfl_down(const SI &a, int off) { SI r; r.s = __shfl_down(a.s, off, 64); r.hi = __shfl_down(a.hi, off, 64); r.lo = __shfl_down(a.lo, off, 64); r.m = __shfl_down(a.m, off, 64); r.n = __shfl_down(a.n, off, 64); return r; }
+__device__ __forceinline__ double st_sum(const SF &a) { return (double)a.s + (double)a.c; }
+__device__ __forceinline__ double st_sum(const SD &a) { return a.s + a.c; }
+__device__ __forceinline__ double st_sum(const SI &a) { return (double)a.s; }
+__device__ __forceinline__ double st_max(const SF &a) { return (double)a.m; }
+__device__ __forceinline__ double st_max(const SD &a) { return a.m; }
+__device__ __forceinline__ double st_max(const SI &a) { return (double)a.m; }
+__device__ __forceinline__ int st_ovf(const SF &) { return 0; }
+__device__ __forceinline__ int st_ovf(const SD &) { return 0; }
+__device__ __forceinline__ int st_ovf(const SI &a) { return (a.n && (a.hi > 2147483647LL || a.lo < -2147483648LL)) ? 1 : 0; }
+// Ordered fold of a wavefront: lane l ends up holding lanes [l, 64) folded in
+// lane order; lane 0 holds the whole wavefront.
+template <typename S> __device__ __forceinline__ S wave_fold(S v) {
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    const S o = st_shfl_down(v, off);
+    if ((lane & (2 * off - 1)) == 0) v = st_comb(v, o);
+  }
+  return v;
+}
+struct RedArgs {
+  const void *in[8]; i64 in_off[8];
+  void *rowpart; void *colpart; void *rowfinal;
+  double *full_sum, *full_max, *row_sum, *row_max, *col_sum, *col_max;
+  int *flags;
+};
+)PH";
+
+struct RedArgs {
+  const void *in[PH_MAX_INPUTS];
+  int64_t in_off[PH_MAX_INPUTS];
+  void *rowpart, *colpart, *rowfinal;
+  double *full_sum, *full_max, *row_sum, *row_max, *col_sum, *col_max;
+  int *flags;
+};
+
+struct RedPlan {
+  int64_t rows, cols;
+  int tr;         // rows per tile
+  int64_t ntr, ntc;
+  int vec;        // columns per lane
+  bool do_rows, do_cols, do_full;
+  const char *state;   // SF / SD / SI
+  size_t state_bytes;
+  const char *acc_t;   // float / double / i64
+  bool vec_ok[PH_MAX_INPUTS];  // whole-vector loads of the lane's VEC columns
+};
+
+std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
+                       const int64_t *cs, const RedPlan &P) {
+  std::string s = kPrelude;
+  s += kRedPrelude;
+  const int V = P.vec;
+  // --- tile kernel
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
+  s += fmt("  typedef %s S;\n", P.state);
+  s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
+           lit_i64(P.ntc).c_str());
+  s += "  const int lane = threadIdx.x & 63; const int wave = threadIdx.x >> 6;\n";
+  s += fmt("  const i64 col0 = ct * %d + wave * %d + lane * %d;\n", 256 * V, 64 * V, V);
+  s += fmt("  const i64 r0 = rt * %d;\n", P.tr);
+  s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
+           lit_i64(P.rows).c_str());
+  s += "  int fl = 0;\n";
+  s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
+  if (P.do_rows) s += fmt("  __shared__ S rowsh[%d][4];\n", P.tr);
+  s += "  for (i64 r = r0; r < r1; ++r) {\n";
+  for (int j = 0; j < n_in; ++j)
+    s += fmt("    const i64 rb%d = A.in_off[%d] + r * %s;\n", j, j, lit_i64(rs[j]).c_str());
+  for (int j = 0; j < n_in; ++j) s += fmt("    %s x%d[%d];\n", ctype(compute_dt(in_dt[j])), j, V);
+  for (int j = 0; j < n_in; ++j) {
+    if (P.vec_ok[j]) {
+      // whole-vector load when the lane's VEC columns are in range and aligned
+      const char *vt = in_dt[j] == PH_F64 ? "double4" : in_dt[j] == PH_BF16 ? "ushort4"
+                       : in_dt[j] == PH_I32 ? "int4" : in_dt[j] == PH_BOOL ? "uchar4" : "float4";
+      s += fmt("    if (col0 + %d <= %s) {\n", V, lit_i64(P.cols).c_str());
+      s += fmt("      const %s t = *(const %s *)((const %s *)A.in[%d] + rb%d + col0);\n", vt, vt,
+               stype(in_dt[j]), j, j);
+      s += fmt("      const %s tt[4] = {t.x, t.y, t.z, t.w};\n", stype(in_dt[j]));
+      s += fmt("#pragma unroll\n      for (int v = 0; v < 4; ++v) x%d[v] = ph_ld(tt, v);\n", j);
+      s += "    } else {\n";
+    } else {
+      s += "    {\n";
+    }
+    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) {\n        const i64 c = col0 + v;\n", V);
+    s += fmt("        if (c < %s) x%d[v] = ph_ld((const %s *)A.in[%d], rb%d + c * %s);\n", lit_i64(P.cols).c_str(),
+             j, stype(in_dt[j]), j, j, lit_i64(cs[j]).c_str());
+    s += "      }\n    }\n";
+  }
+  if (P.do_rows) s += "    S rst; st_init(rst);\n";
+  std::vector<std::string> inexpr;
+  for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[v]", j));
+  std::string body;
+  std::string res = emit_prog(p, inexpr, &body, "");
+  s += fmt("#pragma unroll\n    for (int v = 0; v < %d; ++v) {\n      const i64 c = col0 + v;\n", V);
+  s += fmt("      if (c < %s) {\n", lit_i64(P.cols).c_str());
+  s += body;
+  s += fmt("        const %s val = (%s)(%s);\n", P.acc_t, P.acc_t, res.c_str());
+  if (P.do_cols) s += "        st_add(cst[v], val);\n";
+  if (P.do_rows) s += "        st_add(rst, val);\n";
+  s += "      }\n    }\n";
+  if (P.do_rows) {
+    s += "    rst = wave_fold(rst);\n";
+    s += "    if (lane == 0) rowsh[r - r0][wave] = rst;\n";
+  }
+  s += "  }\n";
+  if (P.do_cols) {
+    s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = col0 + v;\n", V);
+    s += fmt("    if (c < %s) ((S *)A.colpart)[rt * %s + c] = cst[v];\n  }\n", lit_i64(P.cols).c_str(),
+             lit_i64(P.cols).c_str());
+  }
+  if (P.do_rows) {
+    s += "  __syncthreads();\n";
+    s += "  for (i64 i = threadIdx.x; i < r1 - r0; i += 256) {\n";
+    s += "    S a = rowsh[i][0]; a = st_comb(a, rowsh[i][1]); a = st_comb(a, rowsh[i][2]); a = st_comb(a, rowsh[i][3]);\n";
+    s += fmt("    ((S *)A.rowpart)[ct * %s + r0 + i] = a;\n  }\n", lit_i64(P.rows).c_str());
+  }
+  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  // --- finish: per-row and per-column results
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_finish(RedArgs A) {\n";
+  s += fmt("  typedef %s S;\n", P.state);
+  s += "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n  int fl = 0;\n";
+  if (P.do_rows) {
+    s += fmt("  if (i < %s) {\n    S a = ((const S *)A.rowpart)[i];\n", lit_i64(P.rows).c_str());
+    s += fmt("    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
+             lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
+    s += "    if (A.row_sum) A.row_sum[i] = st_sum(a);\n    if (A.row_max) A.row_max[i] = st_max(a);\n";
+    s += "    if (A.row_sum) fl |= st_ovf(a);\n";
+    s += "    ((S *)A.rowfinal)[i] = a;\n  }\n";
+  }
+  if (P.do_cols) {
+    s += fmt("  if (i < %s) {\n    S a = ((const S *)A.colpart)[i];\n", lit_i64(P.cols).c_str());
+    s += fmt("    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + i]);\n",
+             lit_i64(P.ntr).c_str(), lit_i64(P.cols).c_str());
+    s += "    if (A.col_sum) { A.col_sum[i] = st_sum(a); fl |= st_ovf(a); }\n    if (A.col_max) A.col_max[i] = st_max(a);\n  }\n";
+  }
+  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  // --- finish: full result over the per-row states, in row order
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_full(RedArgs A) {\n";
+  s += fmt("  typedef %s S;\n", P.state);
+  s += "  __shared__ S sh[256];\n";
+  s += fmt("  const i64 n = %s; const i64 chunk = (n + 255) / 256;\n", lit_i64(P.rows).c_str());
+  s += "  const i64 b = threadIdx.x * chunk; const i64 e = (b + chunk < n) ? b + chunk : n;\n";
+  s += "  S a; st_init(a);\n  for (i64 i = b; i < e; ++i) a = (i == b) ? ((const S *)A.rowfinal)[i] : st_comb(a, ((const S *)A.rowfinal)[i]);\n";
+  s += "  sh[threadIdx.x] = a;\n  __syncthreads();\n";
+  s += "  for (int off = 1; off < 256; off <<= 1) {\n    S o = sh[threadIdx.x];\n";
+  s += "    if ((threadIdx.x & (2 * off - 1)) == 0 && threadIdx.x + off < 256) o = st_comb(o, sh[threadIdx.x + off]);\n";
+  s += "    __syncthreads();\n    sh[threadIdx.x] = o;\n    __syncthreads();\n  }\n";
+  s += "  if (threadIdx.x == 0) { const S r = sh[0]; if (A.full_sum) { A.full_sum[0] = st_sum(r); if (st_ovf(r)) atomicOr(A.flags, 1); } if (A.full_max) A.full_max[0] = st_max(r); }\n}\n";
+  return s;
+}
+
+// A 2-D view of every input: rows x cols with per-input row/col strides.
+struct View2D {
+  int64_t rows, cols;
+  int64_t rs[PH_MAX_INPUTS], cs[PH_MAX_INPUTS], off[PH_MAX_INPUTS];
+};
+
+// Runs the fused reduction over a 2-D arrangement (inputs affine).
+int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const int *in_dt,
+                 const View2D &V, int acc, double *full_sum, double *full_max, double *row_sum,
+                 double *row_max, double *col_sum, double *col_max, int32_t *flags_out) {
+  RedPlan P;
+  P.rows = V.rows;
+  P.cols = V.cols;
+  P.vec = 4;
+  P.do_cols = col_sum || col_max;
+  P.do_full = full_sum || full_max;
+  P.do_rows = row_sum || row_max || P.do_full;
+  const int odt = prog->out_dtype;
+  if (odt == PH_I32 || odt == PH_BOOL) {
+    P.state = "SI"; P.state_bytes = 40; P.acc_t = "i64";
+  } else if (acc == PH_ACC_F64) {
+    P.state = "SD"; P.state_bytes = 24; P.acc_t = "double";
+  } else {
+    P.state = "SF"; P.state_bytes = 12; P.acc_t = "float";
+  }
+  for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = false;
+  for (int j = 0; j < n_in; ++j) {
+    const size_t vbytes = 4 * ph::dtype_size(in_dt[j]);
+    P.vec_ok[j] = V.cs[j] == 1 && V.off[j] % 4 == 0 && (P.rows == 1 || V.rs[j] % 4 == 0) &&
+                  ((uintptr_t)ins[j] % vbytes) == 0;
+  }
+  P.ntc = (P.cols + 1023) / 1024;
+  // rows per tile: aim for >= ~2048 workgroups, between 8 and 256 rows
+  int64_t tr = P.rows * P.ntc / 2048;
+  if (tr < 8) tr = 8;
+  if (tr > 256) tr = 256;
+  if (tr > P.rows) tr = P.rows;
+  if (tr < 1) tr = 1;
+  P.tr = (int)tr;
+  P.ntr = (P.rows + P.tr - 1) / P.tr;
+  std::string src = red_source(prog, n_in, in_dt, V.rs, V.cs, P);
+  hipFunction_t ktile, kfin, kfull;
+  int rc = ph::jit_kernel(src, "ph_red_tile", &ktile);
+  if (rc) return rc;
+  rc = ph::jit_kernel(src, "ph_red_finish", &kfin);
+  if (rc) return rc;
+  rc = ph::jit_kernel(src, "ph_red_full", &kfull);
+  if (rc) return rc;
+  // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [rows]
+  const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
+  const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
+  const size_t rowfinal_b = P.do_rows ? (size_t)P.rows * P.state_bytes : 0;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  void *base;
+  rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + 256, &base, 2);
+  if (rc) return rc;
+  int *dflags;
+  rc = get_flags_buffer(&dflags);
+  if (rc) return rc;
+  PH_HIP(hipMemsetAsync(dflags, 0, sizeof(int), ph::stream()));
+  RedArgs A;
+  std::memset(&A, 0, sizeof(A));
+  for (int j = 0; j < n_in; ++j) {
+    A.in[j] = ins[j];
+    A.in_off[j] = V.off[j];
+  }
+  char *b = (char *)base;
+  A.rowpart = b;
+  A.colpart = b + al(rowpart_b);
+  A.rowfinal = b + al(rowpart_b) + al(colpart_b);
+  A.full_sum = full_sum;
+  A.full_max = full_max;
+  A.row_sum = row_sum;
+  A.row_max = row_max;
+  A.col_sum = col_sum;
+  A.col_max = col_max;
+  A.flags = dflags;
+  size_t asz = sizeof(A);
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &asz,
+                 HIP_LAUNCH_PARAM_END};
+  rc = ph::launch_gate("reduce");
+  if (rc) return rc;
+  PH_HIP(hipModuleLaunchKernel(ktile, (unsigned)(P.ntr * P.ntc), 1, 1, 256, 1, 1, 0, ph::stream(),
+                               nullptr, cfg));
+  const int64_t nfin = (P.rows > P.cols ? P.rows : P.cols);
+  PH_HIP(hipModuleLaunchKernel(kfin, (unsigned)((nfin + 255) / 256), 1, 1, 256, 1, 1, 0, ph::stream(),
+                               nullptr, cfg));
+  if (P.do_full)
+    PH_HIP(hipModuleLaunchKernel(kfull, 1, 1, 1, 256, 1, 1, 0, ph::stream(), nullptr, cfg));
+  return read_flags(dflags, flags_out);
+}
+
+int check_inputs(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                 const ph_view_desc *in_descs, int *in_dt) {
+  if (n_in < 1 || n_in > PH_MAX_INPUTS)
+    return ph::set_error(PH_E_ARGUMENT, "%d inputs (1..%d)", n_in, PH_MAX_INPUTS);
+  for (int j = 0; j < n_in; ++j) {
+    int rc = ph::validate_desc(&in_descs[j], "input");
+    if (rc) return rc;
+    if (!ins[j]) return ph::set_error(PH_E_ARGUMENT, "input %d: null pointer", j);
+    in_dt[j] = in_descs[j].dtype;
+    if (in_descs[j].view.rank != in_descs[0].view.rank)
+      return ph::set_error(PH_E_SHAPE, "input %d rank differs", j);
+    for (int i = 0; i < in_descs[0].view.rank; ++i)
+      if (in_descs[j].view.shape[i] != in_descs[0].view.shape[i])
+        return ph::set_error(PH_E_SHAPE, "input %d shape differs on axis %d", j, i);
+  }
+  return validate_prog(prog, n_in, in_dt);
+}
+
+}  // namespace
+
+extern "C" {
+
+long ph_jit_compiles(void) { return ph::jit_compiles(); }
+
+int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes, int64_t rows,
+                     int64_t cols, int32_t kind) {
+  PH_GUARD();
+  if (n_in < 1 || n_in > PH_MAX_INPUTS) return ph::set_error(PH_E_ARGUMENT, "%d inputs", n_in);
+  int in_dt[PH_MAX_INPUTS];
+  for (int j = 0; j < n_in; ++j) in_dt[j] = in_dtypes[j];
+  int rc = validate_prog(prog, n_in, in_dt);
+  if (rc) return rc;
+  if (rows < 1 || cols < 1) return ph::set_error(PH_E_ARGUMENT, "empty shape");
+  if (kind == 0) {
+    ph_view_desc d[PH_MAX_INPUTS + 1];
+    const ph_view_desc *dp[PH_MAX_INPUTS + 1];
+    for (int j = 0; j <= n_in; ++j) {
+      std::memset(&d[j], 0, sizeof(d[j]));
+      d[j].dtype = j < n_in ? in_dt[j] : prog->out_dtype;
+      d[j].extent = rows * cols;
+      d[j].view.rank = 2;
+      d[j].view.shape[0] = rows;
+      d[j].view.shape[1] = cols;
+      d[j].view.stride[0] = cols;
+      d[j].view.stride[1] = 1;
+      dp[j] = &d[j];
+    }
+    ph::Layout L;
+    ph::collapse(dp, n_in + 1, &L);
+    const int ept = ph::dtype_size(prog->out_dtype) >= 8 ? 4 : 8;
+    const int64_t c = L.rank > 0 ? L.shape[L.rank - 1] : 1;
+    int64_t nb;
+    std::string src = map_source(prog, n_in, in_dt, L, prog->out_dtype, ept, L.rank > 1 && c < 256, &nb);
+    return ph::jit_compile_only(src);
+  }
+  View2D V;
+  V.rows = rows;
+  V.cols = cols;
+  for (int j = 0; j < n_in; ++j) {
+    V.rs[j] = cols;
+    V.cs[j] = 1;
+    V.off[j] = 0;
+  }
+  RedPlan P;
+  P.rows = rows;
+  P.cols = cols;
+  P.vec = 4;
+  P.do_cols = P.do_rows = P.do_full = true;
+  const int odt = prog->out_dtype;
+  if (odt == PH_I32 || odt == PH_BOOL) {
+    P.state = "SI"; P.state_bytes = 40; P.acc_t = "i64";
+  } else {
+    P.state = "SF"; P.state_bytes = 12; P.acc_t = "float";
+  }
+  for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = j < n_in && cols % 4 == 0;
+  P.ntc = (cols + 1023) / 1024;
+  P.tr = 64;
+  P.ntr = (rows + 63) / 64;
+  return ph::jit_compile_only(red_source(prog, n_in, in_dt, V.rs, V.cs, P));
+}
+
+int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
+           void *out, const ph_view_desc *out_desc, int32_t *flags_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (flags_out) *flags_out = 0;
+  int in_dt[PH_MAX_INPUTS];
+  int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
+  if (rc) return rc;
+  rc = ph::validate_desc(out_desc, "output");
+  if (rc) return rc;
+  if (out_desc->nstage != 0) return ph::set_error(PH_E_UNSUPPORTED, "map output must be affine");
+  if (out_desc->dtype != prog->out_dtype)
+    return ph::set_error(PH_E_ARGUMENT, "output dtype %d, program yields %d", out_desc->dtype, prog->out_dtype);
+  if (out_desc->view.rank != in_descs[0].view.rank)
+    return ph::set_error(PH_E_SHAPE, "output rank differs from inputs");
+  for (int i = 0; i < out_desc->view.rank; ++i)
+    if (out_desc->view.shape[i] != in_descs[0].view.shape[i])
+      return ph::set_error(PH_E_SHAPE, "output shape differs on axis %d", i);
+  if (ph::desc_count(out_desc) == 0) return PH_OK;
+  // inputs with stages are gathered first
+  ph_view_desc local[PH_MAX_INPUTS];
+  const void *ptrs[PH_MAX_INPUTS];
+  int nstaged = 0;
+  for (int j = 0; j < n_in; ++j) nstaged += in_descs[j].nstage > 0 ? 1 : 0;
+  if (nstaged > 2) return ph::set_error(PH_E_UNSUPPORTED, "at most two reshaped/tiled inputs per map");
+  int slot = 0;
+  for (int j = 0; j < n_in; ++j) {
+    local[j] = in_descs[j];
+    ptrs[j] = ins[j];
+    if (in_descs[j].nstage > 0) {
+      rc = materialize(&in_descs[j], ins[j], slot++, &local[j], &ptrs[j]);
+      if (rc) return rc;
+    }
+  }
+  const ph_view_desc *descs[PH_MAX_INPUTS + 1];
+  for (int j = 0; j < n_in; ++j) descs[j] = &local[j];
+  descs[n_in] = out_desc;
+  ph::Layout L;
+  ph::collapse(descs, n_in + 1, &L);
+  const int esize = (int)ph::dtype_size(out_desc->dtype);
+  const int ept = esize >= 8 ? 4 : 8;
+  const int64_t cols = L.rank > 0 ? L.shape[L.rank - 1] : 1;
+  const bool flat = L.rank > 1 && cols < 256;
+  int64_t nblocks = 0;
+  std::string src = map_source(prog, n_in, in_dt, L, out_desc->dtype, ept, flat, &nblocks);
+  hipFunction_t fn;
+  rc = ph::jit_kernel(src, "ph_map_kernel", &fn);
+  if (rc) return rc;
+  int *dflags;
+  rc = get_flags_buffer(&dflags);
+  if (rc) return rc;
+  PH_HIP(hipMemsetAsync(dflags, 0, sizeof(int), ph::stream()));
+  MapArgs A;
+  std::memset(&A, 0, sizeof(A));
+  for (int j = 0; j < n_in; ++j) {
+    A.in[j] = ptrs[j];
+    A.in_off[j] = L.offset[j];
+  }
+  A.out = out;
+  A.out_off = L.offset[n_in];
+  A.flags = dflags;
+  size_t asz = sizeof(A);
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &asz,
+                 HIP_LAUNCH_PARAM_END};
+  rc = ph::launch_gate("map");
+  if (rc) return rc;
+  PH_HIP(hipModuleLaunchKernel(fn, (unsigned)grid_for(nblocks), 1, 1, kBlock, 1, 1, 0, ph::stream(),
+                               nullptr, cfg));
+  return read_flags(dflags, flags_out);
+}
+
+int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                    const ph_view_desc *in_descs, int32_t acc, double *full_sum, double *full_max,
+                    double *row_sum, double *row_max, double *col_sum, double *col_max,
+                    int32_t *flags_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (flags_out) *flags_out = 0;
+  int in_dt[PH_MAX_INPUTS];
+  int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
+  if (rc) return rc;
+  if (in_descs[0].view.rank != 2) return ph::set_error(PH_E_DIMENSION, "ph_reduce2d_all needs a 2-D view");
+  if (ph::desc_count(&in_descs[0]) == 0)
+    return ph::set_error(PH_E_ARGUMENT, "reduction over an empty view");
+  View2D V;
+  const void *ptrs[PH_MAX_INPUTS];
+  ph_view_desc local[PH_MAX_INPUTS];
+  V.rows = in_descs[0].view.shape[0];
+  V.cols = in_descs[0].view.shape[1];
+  for (int j = 0; j < n_in; ++j) {
+    local[j] = in_descs[j];
+    ptrs[j] = ins[j];
+    if (in_descs[j].nstage > 0) {
+      if (j > 1) return ph::set_error(PH_E_UNSUPPORTED, "at most two reshaped/tiled inputs");
+      rc = materialize(&in_descs[j], ins[j], j, &local[j], &ptrs[j]);
+      if (rc) return rc;
+    }
+    V.rs[j] = local[j].view.stride[0];
+    V.cs[j] = local[j].view.stride[1];
+    V.off[j] = local[j].view.offset;
+  }
+  return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, full_sum, full_max, row_sum, row_max, col_sum,
+                      col_max, flags_out);
+}
+
+int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
+              int32_t keep_axis, int32_t ops, int32_t acc, double *out_sum, double *out_max,
+              int32_t *flags_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (flags_out) *flags_out = 0;
+  int in_dt[PH_MAX_INPUTS];
+  int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
+  if (rc) return rc;
+  const int rank = in_descs[0].view.rank;
+  if (rank < 1) return ph::set_error(PH_E_DIMENSION, "reduction of a rank-0 view");
+  if (keep_axis < -1 || keep_axis >= rank)
+    return ph::set_error(PH_E_INDEX, "axis %d outside a %d-dimensional view", keep_axis, rank);
+  double *sum = (ops & PH_RED_SUM) ? out_sum : nullptr;
+  double *mx = (ops & PH_RED_MAX) ? out_max : nullptr;
+  if (ph::desc_count(&in_descs[0]) == 0) {
+    if (keep_axis >= 0 && in_descs[0].view.shape[keep_axis] == 0) return PH_OK;  // no results
+    return ph::set_error(PH_E_ARGUMENT, "reduction over an empty view");
+  }
+  // Arrange as 2-D: rows x cols in lexicographic order of the reduced data.
+  //   keep -1         : [prod(shape[:-1]), shape[-1]]      -> full
+  //   keep 0          : [shape[0], prod(shape[1:])]         -> per row
+  //   keep rank-1     : [prod(shape[:-1]), shape[-1]]       -> per column
+  //   keep k (middle) : axis k moved to the front (gathered) -> per row
+  ph_view_desc local[PH_MAX_INPUTS];
+  const void *ptrs[PH_MAX_INPUTS];
+  for (int j = 0; j < n_in; ++j) {
+    local[j] = in_descs[j];
+    ptrs[j] = ins[j];
+  }
+  bool middle = keep_axis > 0 && keep_axis < rank - 1;
+  bool col_mode = keep_axis == rank - 1 && rank > 1;
+  bool row_mode = keep_axis == 0 || middle || (keep_axis == 0 && rank == 1);
+  if (keep_axis == 0 && rank == 1) {
+    // each slice is one element: a map of the program
+    row_mode = true;
+  }
+  for (int j = 0; j < n_in; ++j) {
+    ph_view_desc d = local[j];
+    bool need_copy = d.nstage > 0;
+    if (middle && !need_copy) {
+      // permute axis k to the front (a pure view change)
+      ph_view_desc p = d;
+      p.view.shape[0] = d.view.shape[keep_axis];
+      p.view.stride[0] = d.view.stride[keep_axis];
+      int o = 1;
+      for (int i = 0; i < rank; ++i)
+        if (i != keep_axis) {
+          p.view.shape[o] = d.view.shape[i];
+          p.view.stride[o] = d.view.stride[i];
+          ++o;
+        }
+      d = p;
+    }
+    if (!need_copy) {
+      // can the needed 2-D split be expressed by strides?
+      int split = (keep_axis == -1 || col_mode) ? rank - 1 : 1;  // dims [0,split) -> rows
+      for (int i = 0; i + 1 < split; ++i)
+        if (d.view.shape[i + 1] != 1 && d.view.stride[i] != d.view.stride[i + 1] * d.view.shape[i + 1])
+          need_copy = true;
+      for (int i = split; i + 1 < rank; ++i)
+        if (d.view.shape[i + 1] != 1 && d.view.stride[i] != d.view.stride[i + 1] * d.view.shape[i + 1])
+          need_copy = true;
+    }
+    if (need_copy) {
+      if (j > 1) return ph::set_error(PH_E_UNSUPPORTED, "at most two non-collapsible inputs");
+      rc = materialize(&d, ptrs[j], j, &local[j], &ptrs[j]);
+      if (rc) return rc;
+    } else {
+      local[j] = d;
+    }
+  }
+  View2D V;
+  const int split = (keep_axis == -1 || col_mode) ? rank - 1 : 1;
+  V.rows = 1;
+  V.cols = 1;
+  for (int i = 0; i < split; ++i) V.rows *= local[0].view.shape[i];
+  for (int i = split; i < rank; ++i) V.cols *= local[0].view.shape[i];
+  for (int j = 0; j < n_in; ++j) {
+    const ph_affine &a = local[j].view;
+    // innermost non-unit stride of each group (any stride if the group has one element)
+    int64_t rs = 0, cs = 0;
+    for (int i = split - 1; i >= 0; --i)
+      if (a.shape[i] != 1) { rs = a.stride[i]; break; }
+    for (int i = rank - 1; i >= split; --i)
+      if (a.shape[i] != 1) { cs = a.stride[i]; break; }
+    if (rank == 1 && split == 0) { V.rows = 1; rs = 0; }
+    V.rs[j] = rs;
+    V.cs[j] = cs;
+    V.off[j] = a.offset;
+  }
+  if (rank == 1 && keep_axis == -1) {
+    // single row
+    V.rows = 1;
+    V.cols = local[0].view.shape[0];
+    for (int j = 0; j < n_in; ++j) {
+      V.rs[j] = 0;
+      V.cs[j] = local[j].view.stride[0];
+    }
+  }
+  if (rank == 1 && keep_axis == 0) {
+    V.rows = local[0].view.shape[0];
+    V.cols = 1;
+    for (int j = 0; j < n_in; ++j) {
+      V.rs[j] = local[j].view.stride[0];
+      V.cs[j] = 0;
+    }
+  }
+  if (keep_axis == -1)
+    return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, sum, mx, nullptr, nullptr, nullptr, nullptr,
+                        flags_out);
+  if (col_mode)
+    return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, nullptr, nullptr, nullptr, nullptr, sum, mx,
+                        flags_out);
+  (void)row_mode;
+  return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, nullptr, nullptr, sum, mx, nullptr, nullptr,
+                      flags_out);
+}
+
+}  // extern "C"

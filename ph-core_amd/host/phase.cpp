@@ -1,0 +1,2085 @@
+// Host side of the MI355X backend: see phase.hpp.
+#include "phase.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cinttypes>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+namespace phase {
+
+// ===========================================================================
+// Errors
+// ===========================================================================
+[[noreturn]] void raise_status(int code, const std::string &context) {
+  std::string msg = context;
+  const char *le = ph_last_error();
+  if (le && *le) msg += ": " + std::string(le);
+  switch (code) {
+    case PH_E_INDEX: throw IndexError(msg);
+    case PH_E_SHAPE: throw ShapeError(msg);
+    case PH_E_DIMENSION: throw DimensionError(msg);
+    case PH_E_ARGUMENT: throw ArgumentError(msg);
+    case PH_E_OVERFLOW: throw OverflowError(msg);
+    case PH_E_DIVZERO: throw DivisionByZeroError(msg);
+    default: throw DeviceError(msg);
+  }
+}
+
+void check(int code, const char *context) {
+  if (code != PH_OK) raise_status(code, context);
+}
+
+static void check_flags(int32_t flags, const char *what) {
+  if (flags & PH_FLAG_NEGEXP)
+    throw ArgumentError(std::string(what) + ": Cannot raise an integer to a negative integer power, use floats for that");
+  if (flags & PH_FLAG_DIVZERO) throw DivisionByZeroError(std::string(what) + ": Division by 0");
+  if (flags & PH_FLAG_OVERFLOW) throw OverflowError(std::string(what) + ": Arithmetic overflow");
+}
+
+static bool g_inited = false;
+
+void init(int device) {
+  if (g_inited) return;
+  check(ph_init(device), "ph_init");
+  g_inited = true;
+}
+
+bool device_available() {
+  int n = 0;
+  if (ph_device_count(&n) != PH_OK) return false;
+  return n > 0;
+}
+
+// ===========================================================================
+// DType / Scalar
+// ===========================================================================
+const char *dtype_name(DType d) {
+  switch (d) {
+    case DType::I32: return "Int32";
+    case DType::F32: return "Float32";
+    case DType::F64: return "Float64";
+    case DType::BF16: return "BFloat16";
+    case DType::Bool: return "Bool";
+  }
+  return "?";
+}
+
+size_t dtype_size(DType d) {
+  switch (d) {
+    case DType::I32: return 4;
+    case DType::F32: return 4;
+    case DType::F64: return 8;
+    case DType::BF16: return 2;
+    case DType::Bool: return 1;
+  }
+  return 0;
+}
+
+DType compute_dtype(DType s) { return s == DType::BF16 ? DType::F32 : s; }
+
+Scalar Scalar::i32(int64_t v) {
+  if (v < INT32_MIN || v > INT32_MAX) throw OverflowError("Int32 literal out of range: " + std::to_string(v));
+  Scalar s;
+  s.dt = DType::I32;
+  s.i = v;
+  return s;
+}
+Scalar Scalar::f32(double v) {
+  Scalar s;
+  s.dt = DType::F32;
+  s.f = (double)(float)v;
+  return s;
+}
+Scalar Scalar::f64(double v) {
+  Scalar s;
+  s.dt = DType::F64;
+  s.f = v;
+  return s;
+}
+Scalar Scalar::boolean(bool v) {
+  Scalar s;
+  s.dt = DType::Bool;
+  s.i = v ? 1 : 0;
+  return s;
+}
+std::string Scalar::to_s() const {
+  switch (dt) {
+    case DType::I32: return std::to_string(i);
+    case DType::Bool: return i ? "true" : "false";
+    default: {
+      char b[64];
+      std::snprintf(b, sizeof(b), "%.17g", f);
+      return b;
+    }
+  }
+}
+
+// bits of a scalar stored as `dt`
+static void scalar_bits(const Scalar &s, DType dt, void *out) {
+  switch (dt) {
+    case DType::I32: {
+      int32_t v = (int32_t)(s.dt == DType::I32 || s.dt == DType::Bool ? s.i : (int64_t)s.f);
+      std::memcpy(out, &v, 4);
+      break;
+    }
+    case DType::F32: {
+      float v = (float)(s.dt == DType::I32 || s.dt == DType::Bool ? (double)s.i : s.f);
+      std::memcpy(out, &v, 4);
+      break;
+    }
+    case DType::F64: {
+      double v = s.dt == DType::I32 || s.dt == DType::Bool ? (double)s.i : s.f;
+      std::memcpy(out, &v, 8);
+      break;
+    }
+    case DType::BF16: {
+      float f = (float)(s.dt == DType::I32 || s.dt == DType::Bool ? (double)s.i : s.f);
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      uint16_t h = (uint16_t)(u >> 16);
+      std::memcpy(out, &h, 2);
+      break;
+    }
+    case DType::Bool: {
+      uint8_t v = (s.dt == DType::Bool || s.dt == DType::I32) ? (s.i != 0) : (s.f != 0.0);
+      std::memcpy(out, &v, 1);
+      break;
+    }
+  }
+}
+
+static Scalar scalar_from_bits(DType dt, const void *p) {
+  switch (dt) {
+    case DType::I32: { int32_t v; std::memcpy(&v, p, 4); return Scalar::i32(v); }
+    case DType::F32: { float v; std::memcpy(&v, p, 4); return Scalar::f32(v); }
+    case DType::F64: { double v; std::memcpy(&v, p, 8); return Scalar::f64(v); }
+    case DType::BF16: {
+      uint16_t h; std::memcpy(&h, p, 2);
+      uint32_t u = (uint32_t)h << 16; float f; std::memcpy(&f, &u, 4);
+      return Scalar::f32(f);
+    }
+    case DType::Bool: { uint8_t v; std::memcpy(&v, p, 1); return Scalar::boolean(v != 0); }
+  }
+  return Scalar();
+}
+
+std::string shape_to_s(const Shape &s) {
+  std::string r = "[";
+  for (size_t i = 0; i < s.size(); ++i) r += (i ? ", " : "") + std::to_string(s[i]);
+  return r + "]";
+}
+
+// ===========================================================================
+// Range literal parsing (Crystal source forms)
+// ===========================================================================
+namespace {
+
+struct RNode {  // a parsed Crystal range expression
+  bool is_range = false;
+  bool has_value = false;  // Int (when !is_range)
+  int64_t value = 0;
+  std::shared_ptr<RNode> begin, end;
+  bool exclusive = false;
+};
+
+struct RParser {
+  std::string s;
+  size_t p = 0;
+  void ws() {
+    while (p < s.size() && std::isspace((unsigned char)s[p])) ++p;
+  }
+  [[noreturn]] void fail(const std::string &why) {
+    throw ArgumentError("poorly formatted range literal '" + s + "': " + why +
+                        " (should be a..c, a..b..c, (a..b)..c, or a..(b..c) where a,c are Int | Nil, b is Int)");
+  }
+  bool at_dots(bool *excl) {
+    ws();
+    if (s.compare(p, 3, "...") == 0) {
+      *excl = true;
+      return true;
+    }
+    if (s.compare(p, 2, "..") == 0) {
+      *excl = false;
+      return true;
+    }
+    return false;
+  }
+  std::shared_ptr<RNode> atom() {  // Int, (range), or empty
+    ws();
+    auto n = std::make_shared<RNode>();
+    if (p < s.size() && s[p] == '(') {
+      ++p;
+      n = expr();
+      ws();
+      if (p >= s.size() || s[p] != ')') fail("missing ')'");
+      ++p;
+      return n;
+    }
+    size_t b = p;
+    if (p < s.size() && (s[p] == '-' || s[p] == '+')) ++p;
+    while (p < s.size() && (std::isdigit((unsigned char)s[p]) || s[p] == '_')) ++p;
+    if (p == b) return n;  // empty operand (nil)
+    std::string num;
+    for (size_t i = b; i < p; ++i)
+      if (s[i] != '_') num += s[i];
+    if (num == "-" || num == "+") fail("sign without digits");
+    errno = 0;
+    long long v = std::strtoll(num.c_str(), nullptr, 10);
+    if (errno) fail("integer out of range");
+    n->has_value = true;
+    n->value = v;
+    return n;
+  }
+  std::shared_ptr<RNode> expr() {
+    auto left = atom();
+    bool excl;
+    while (at_dots(&excl)) {
+      p += excl ? 3 : 2;
+      auto right = atom();
+      auto r = std::make_shared<RNode>();
+      r->is_range = true;
+      r->begin = left;
+      r->end = right;
+      r->exclusive = excl;
+      left = r;
+    }
+    return left;
+  }
+};
+
+}  // namespace
+
+RangeLiteral RangeLiteral::of_index(int64_t i) {
+  RangeLiteral r;
+  r.is_index = true;
+  r.index = i;
+  r.text = std::to_string(i);
+  return r;
+}
+
+RangeLiteral RangeLiteral::full() { return parse(".."); }
+
+RangeLiteral RangeLiteral::parse(const std::string &text) {
+  RangeLiteral r;
+  r.text = text;
+  // Steppable::StepIterator form: a.step(by: b, to: c[, exclusive: true])
+  size_t sp = text.find(".step(");
+  if (sp != std::string::npos) {
+    auto num_after = [&](const std::string &key, bool *found) -> int64_t {
+      size_t k = text.find(key);
+      *found = k != std::string::npos;
+      if (!*found) return 0;
+      return std::strtoll(text.c_str() + k + key.size(), nullptr, 10);
+    };
+    bool fb, ft;
+    std::string head = text.substr(0, sp);
+    while (!head.empty() && (head[0] == '(' || std::isspace((unsigned char)head[0]))) head.erase(0, 1);
+    while (!head.empty() && (head.back() == ')' || std::isspace((unsigned char)head.back()))) head.pop_back();
+    r.first = std::strtoll(head.c_str(), nullptr, 10);
+    r.step = num_after("by:", &fb);
+    int64_t to = num_after("to:", &ft);
+    if (!fb) throw ArgumentError("step literal without by: '" + text + "'");
+    if (ft) r.last = to;
+    r.exclusive = text.find("exclusive: true") != std::string::npos ||
+                  text.find("exclusive:true") != std::string::npos;
+    if (*r.step == 0) throw ArgumentError("Zero step size");
+    return r;
+  }
+  RParser ps;
+  ps.s = text;
+  auto n = ps.expr();
+  ps.ws();
+  if (ps.p != text.size()) ps.fail("trailing characters");
+  if (!n->is_range) {
+    if (!n->has_value) ps.fail("empty literal");
+    r.is_index = true;
+    r.index = n->value;
+    return r;
+  }
+  // parse_range (range_syntax.cr:41-59)
+  auto intish = [](const std::shared_ptr<RNode> &x) { return !x->is_range; };
+  auto val = [](const std::shared_ptr<RNode> &x) -> std::optional<int64_t> {
+    if (x->has_value) return x->value;
+    return std::nullopt;
+  };
+  if (n->begin->is_range) {
+    if (!intish(n->end) || !intish(n->begin->begin) || !intish(n->begin->end)) ps.fail("nested ranges");
+    r.first = val(n->begin->begin);
+    r.step = val(n->begin->end);
+    r.last = val(n->end);
+    r.exclusive = n->exclusive;
+  } else if (n->end->is_range) {
+    if (!intish(n->end->begin) || !intish(n->end->end)) ps.fail("nested ranges");
+    r.first = val(n->begin);
+    r.step = val(n->end->begin);
+    r.last = val(n->end->end);
+    r.exclusive = n->end->exclusive;
+  } else {
+    r.first = val(n->begin);
+    r.last = val(n->end);
+    r.exclusive = n->exclusive;
+  }
+  return r;
+}
+
+// ===========================================================================
+// RangeSyntax / CoordUtil / ShapeUtil
+// ===========================================================================
+static int64_t sign_of(int64_t v) { return (v > 0) - (v < 0); }
+static int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+static int64_t floormod(int64_t a, int64_t b) {
+  int64_t m = a % b;
+  if (m != 0 && ((m < 0) != (b < 0))) m += b;
+  return m;
+}
+
+namespace RangeSyntax {
+
+int64_t get_size(int64_t first, int64_t last, int64_t step) {
+  int64_t cmp = (last > first) - (last < first);
+  if (last != first && sign_of(step) != cmp)
+    throw IndexError("Could not get size of range - step direction disagrees with first and last.");
+  if (step == 0) throw DivisionByZeroError("Division by 0");
+  if (last >= first) return floordiv(last - first, step) + 1;
+  return floordiv(first - last, -step) + 1;
+}
+
+static int64_t canon_unsafe(int64_t i, int64_t size) { return i < 0 ? size + i : i; }
+
+CanonicalRange infer_range(const RangeLiteral &lit, int64_t bound) {
+  if (lit.is_index) {
+    int64_t c = canon_unsafe(lit.index, bound);
+    return {c, 1, c, 1};
+  }
+  int64_t step, first, temp_last;
+  if (!lit.step) {
+    first = lit.first ? canon_unsafe(*lit.first, bound) : 0;
+    temp_last = lit.last ? canon_unsafe(*lit.last, bound) : bound - 1;
+    step = temp_last >= first ? 1 : -1;
+  } else {
+    step = *lit.step;
+    first = lit.first ? canon_unsafe(*lit.first, bound) : (step > 0 ? 0 : bound - 1);
+    temp_last = lit.last ? canon_unsafe(*lit.last, bound) : (step > 0 ? bound - 1 : 0);
+  }
+  if (lit.last && lit.exclusive) {
+    if (temp_last == first) return {0, 0, 0, 0};
+    temp_last -= sign_of(step);
+  }
+  if (first < 0 || temp_last < 0)
+    throw IndexError("Invalid index: At least one endpoint of " + lit.to_s() +
+                     " is negative after canonicalization.");
+  int64_t size;
+  try {
+    size = get_size(first, temp_last, step);
+  } catch (const IndexError &) {
+    throw IndexError("Could not canonicalize range: Conflict between implicit direction of " + lit.to_s() +
+                     " and provided step " + std::to_string(step));
+  }
+  return {first, step, first + step * (size - 1), size};
+}
+
+CanonicalRange canonicalize_range(const RangeLiteral &lit, int64_t bound) {
+  CanonicalRange r = infer_range(lit, bound);
+  bool valid = r.first >= 0 && r.first < bound && r.last >= 0 && r.last < bound;
+  if (!valid)
+    throw IndexError("Could not canonicalize range: " + lit.to_s() +
+                     " is not a sensible index range for axis of length " + std::to_string(bound) + ".");
+  return r;
+}
+
+bool bounded(const RangeLiteral &lit) {
+  if (lit.is_index) return true;
+  if (!lit.step || *lit.step >= 0) return lit.last.has_value();
+  return lit.first.has_value();
+}
+
+void ensure_nonnegative(const RangeLiteral &lit) {
+  bool neg = lit.is_index ? lit.index < 0 : ((lit.first && *lit.first < 0) || (lit.last && *lit.last < 0));
+  if (neg) throw IndexError("Negative indices have no meaning when a bounding shape is not provided.");
+}
+
+}  // namespace RangeSyntax
+
+namespace CoordUtil {
+bool has_index(int64_t index, int64_t size) { return index < size && index >= -size; }
+int64_t canonicalize_index(int64_t index, int64_t size) {
+  if (!has_index(index, size))
+    throw IndexError("Could not canonicalize index: " + std::to_string(index) +
+                     " is not a valid index for an axis of length " + std::to_string(size) + ".");
+  return index < 0 ? size + index : index;
+}
+Coord canonicalize_coord(const Coord &coord, const Shape &shape) {
+  if (coord.size() != shape.size())
+    throw DimensionError("Could not canonicalize coordinate " + shape_to_s(coord) + " to fit in shape " +
+                         shape_to_s(shape) + ": shape has " + std::to_string(shape.size()) +
+                         " dimensions, while coord has " + std::to_string(coord.size()) + ".");
+  Coord r(coord.size());
+  for (size_t i = 0; i < coord.size(); ++i) r[i] = canonicalize_index(coord[i], shape[i]);
+  return r;
+}
+}  // namespace CoordUtil
+
+namespace ShapeUtil {
+bool compatible_shapes(const Shape &a, const Shape &b) {
+  if (a.empty() || b.empty()) return a == b;
+  const Shape &larger = a.size() > b.size() ? a : b;
+  size_t shared = std::min(a.size(), b.size());
+  for (size_t i = 0; i < shared; ++i)
+    if (a[i] != b[i]) return false;
+  for (size_t i = shared; i < larger.size(); ++i)
+    if (larger[i] != 1) return false;
+  return true;
+}
+int64_t shape_to_size(const Shape &shape) {
+  if (shape.empty()) return 0;
+  int64_t n = 1;
+  for (int64_t s : shape) n *= s;
+  return n;
+}
+}  // namespace ShapeUtil
+
+// ===========================================================================
+// IndexRegion
+// ===========================================================================
+void IndexRegion::compute_reduced() {
+  if (drop) {
+    reduced_shape.clear();
+    for (size_t i = 0; i < proper_shape.size(); ++i)
+      if (!degeneracy[i]) reduced_shape.push_back(proper_shape[i]);
+    if (reduced_shape.empty()) reduced_shape.push_back(ShapeUtil::shape_to_size(proper_shape));
+  } else {
+    reduced_shape = proper_shape;
+  }
+}
+
+IndexRegion IndexRegion::from_literal(const RegionLiteral &lit, const Shape &bound, bool drop) {
+  if (lit.size() > bound.size())
+    throw DimensionError("The region literal had more dimensions than its bound shape " + shape_to_s(bound));
+  IndexRegion r;
+  const size_t n = bound.size();
+  r.first.assign(n, 0);
+  r.step.assign(n, 1);
+  r.last.assign(n, 0);
+  r.proper_shape.assign(n, 0);
+  r.degeneracy.assign(n, false);
+  r.drop = drop;
+  for (size_t i = 0; i < lit.size(); ++i) {
+    CanonicalRange c = RangeSyntax::canonicalize_range(lit[i], bound[i]);
+    r.first[i] = c.first;
+    r.step[i] = c.step;
+    r.last[i] = c.last;
+    r.proper_shape[i] = c.size;
+    if (lit[i].is_index) r.degeneracy[i] = drop;
+  }
+  for (size_t a = lit.size(); a < n; ++a) {
+    r.last[a] = bound[a] - 1;
+    r.proper_shape[a] = bound[a];
+  }
+  r.compute_reduced();
+  return r;
+}
+
+IndexRegion IndexRegion::absolute(const RegionLiteral &lit, bool drop) {
+  IndexRegion r;
+  const size_t n = lit.size();
+  r.first.assign(n, 0);
+  r.step.assign(n, 0);
+  r.last.assign(n, 0);
+  r.proper_shape.assign(n, 0);
+  r.degeneracy.assign(n, false);
+  r.drop = drop;
+  for (size_t i = 0; i < n; ++i) {
+    RangeSyntax::ensure_nonnegative(lit[i]);
+    if (!RangeSyntax::bounded(lit[i]))
+      throw PhaseError("Cannot create IndexRegion without an explicit upper bound unless you provide a bounding shape");
+    if (lit[i].is_index) r.degeneracy[i] = true;
+    CanonicalRange c = RangeSyntax::infer_range(lit[i], 0);
+    r.first[i] = c.first;
+    r.step[i] = c.step;
+    r.last[i] = c.last;
+    r.proper_shape[i] = c.size;
+  }
+  r.compute_reduced();
+  return r;
+}
+
+IndexRegion IndexRegion::trimmed(const RegionLiteral &lit, const Shape *bound, bool drop, const Shape &trim_to) {
+  IndexRegion r;
+  const size_t n = trim_to.size();
+  r.first.assign(n, 0);
+  r.step.assign(n, 1);
+  r.last.assign(n, 0);
+  r.proper_shape.assign(n, 0);
+  r.degeneracy.assign(n, false);
+  r.drop = drop;
+  const Shape &bs = bound ? *bound : trim_to;
+  for (size_t i = 0; i < lit.size(); ++i) {
+    if (!bound) RangeSyntax::ensure_nonnegative(lit[i]);
+    CanonicalRange c = RangeSyntax::infer_range(lit[i], bs.at(i));
+    r.first[i] = c.first;
+    r.step[i] = c.step;
+    r.last[i] = c.last;
+    r.proper_shape[i] = c.size;
+    if (lit[i].is_index) r.degeneracy[i] = drop;
+  }
+  for (size_t a = lit.size(); a < bs.size() && a < n; ++a) {
+    r.last[a] = bs[a] - 1;
+    r.proper_shape[a] = bs[a];
+  }
+  r.compute_reduced();
+  return r.trim(trim_to);
+}
+
+IndexRegion IndexRegion::cover(const Shape &shape, bool drop, const std::vector<bool> *degeneracy) {
+  IndexRegion r;
+  const size_t n = shape.size();
+  r.first.assign(n, 0);
+  r.step.resize(n);
+  r.last.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    r.step[i] = shape[i] == 0 ? 0 : 1;
+    r.last[i] = std::max<int64_t>(0, shape[i] - 1);
+  }
+  r.proper_shape = shape;
+  r.degeneracy = degeneracy ? *degeneracy : std::vector<bool>(n, false);
+  r.drop = drop;
+  r.compute_reduced();
+  return r;
+}
+
+IndexRegion IndexRegion::fitted_copy(const IndexRegion &src, const Shape &bound) {
+  if (src.fits_in(bound)) return src;
+  throw ShapeError("Region " + src.to_s() + " does not fit inside " + shape_to_s(bound));
+}
+
+IndexRegion &IndexRegion::trim(const Shape &bound) {
+  if ((int)bound.size() != proper_dimensions()) throw DimensionError("invalid error :)");
+  for (size_t a = 0; a < bound.size(); ++a) {
+    const int64_t nb = bound[a];
+    int64_t &f = first[a], &s = step[a], &l = last[a], &sz = proper_shape[a];
+    if (f >= nb) {
+      if (l >= nb) {
+        f = 0; s = 0; l = 0; sz = 0;
+      } else if (s < 0) {
+        const int64_t as = -s;
+        int64_t span = (nb - 1) - l;
+        const int64_t size = floordiv(span, as) + 1;
+        span -= floormod(span, as);
+        f = l + span;
+        sz = size;
+      }
+    } else if (s > 0 && l >= nb) {
+      int64_t span = (nb - 1) - f;
+      span -= floormod(span, s);
+      sz = floordiv(span, s) + 1;
+      l = f + span;
+    }
+  }
+  compute_reduced();
+  return *this;
+}
+
+IndexRegion &IndexRegion::reverse() {
+  std::swap(first, last);
+  for (auto &s : step) s = -s;
+  return *this;
+}
+
+IndexRegion &IndexRegion::unsafe_translate(const std::vector<int64_t> &offset) {
+  for (size_t a = 0; a < offset.size(); ++a) {
+    first[a] += offset[a];
+    last[a] += offset[a];
+  }
+  return *this;
+}
+
+IndexRegion &IndexRegion::translate(const std::vector<int64_t> &offset) {
+  for (size_t a = 0; a < offset.size(); ++a)
+    if (offset[a] < 0 && (first[a] < -offset[a] || last[a] < -offset[a]))
+      throw IndexError("Can't translate to negative indices");
+  return unsafe_translate(offset);
+}
+
+bool IndexRegion::includes(const Coord &coord) const {
+  if ((int)coord.size() != proper_dimensions()) return false;
+  for (size_t i = 0; i < coord.size(); ++i) {
+    int64_t lo = step[i] > 0 ? first[i] : last[i];
+    int64_t hi = step[i] > 0 ? last[i] : first[i];
+    if (coord[i] < lo || coord[i] > hi) return false;
+    if (step[i] == 0) throw DivisionByZeroError("Division by 0");
+    if (floormod(coord[i] - first[i], step[i]) != 0) return false;
+  }
+  return true;
+}
+
+bool IndexRegion::fits_in(const Shape &bound) const {
+  if ((int)bound.size() != proper_dimensions())
+    throw DimensionError("The bound shape provided had a different number of dimensions than this IndexRegion, so fits_in? is meaningless.");
+  for (size_t i = 0; i < bound.size(); ++i)
+    if (bound[i] <= std::max(first[i], last[i])) return false;
+  return true;
+}
+
+Coord IndexRegion::local_to_absolute_unsafe(const Coord &local) const {
+  Coord out(proper_shape.size());
+  if (drop) {
+    size_t la = 0;
+    for (size_t i = 0; i < out.size(); ++i) {
+      if (degeneracy[i]) out[i] = first[i];
+      else out[i] = first[i] + local.at(la++) * step[i];
+    }
+  } else {
+    for (size_t i = 0; i < out.size(); ++i) out[i] = first[i] + local.at(i) * step[i];
+  }
+  return out;
+}
+
+Coord IndexRegion::local_to_absolute(const Coord &local) const {
+  return local_to_absolute_unsafe(CoordUtil::canonicalize_coord(local, reduced_shape));
+}
+
+Coord IndexRegion::absolute_to_local_unsafe(const Coord &abs) const {
+  Coord local(abs.size());
+  for (size_t i = 0; i < abs.size(); ++i) local[i] = floordiv(abs[i] - first[i], step[i]);
+  if (!drop) return local;
+  Coord r;
+  for (size_t i = 0; i < local.size(); ++i)
+    if (!degeneracy[i]) r.push_back(local[i]);
+  if (r.empty()) r.push_back(0);
+  return r;
+}
+
+Coord IndexRegion::absolute_to_local(const Coord &abs) const {
+  if (!includes(abs)) throw IndexError("Could not convert coordinate: " + shape_to_s(abs) + " does not exist in region " + to_s());
+  return absolute_to_local_unsafe(abs);
+}
+
+std::string IndexRegion::to_s() const {
+  std::string r = "IndexRegion[";
+  for (size_t i = 0; i < first.size(); ++i) {
+    if (i) r += ", ";
+    if (degeneracy[i]) r += std::to_string(first[i]);
+    else if (step[i] == 1 || step[i] == -1) r += std::to_string(first[i]) + ".." + std::to_string(last[i]);
+    else r += std::to_string(first[i]) + ".." + std::to_string(step[i]) + ".." + std::to_string(last[i]);
+  }
+  return r + "]";
+}
+
+bool IndexRegion::operator==(const IndexRegion &o) const {
+  return first == o.first && step == o.step && last == o.last && degeneracy == o.degeneracy && drop == o.drop;
+}
+
+// ===========================================================================
+// Expressions
+// ===========================================================================
+struct ExprNode {
+  enum Kind { Input, Const, Op } kind = Const;
+  int op = 0;          // ph_opcode
+  DType dt = DType::I32;      // result dtype
+  DType opdt = DType::I32;    // dtype the operation computes in
+  int input = 0;
+  DType storage = DType::I32;
+  Scalar value;
+  std::shared_ptr<const ExprNode> a, b, c;
+  int nargs = 0;
+};
+
+static std::shared_ptr<const ExprNode> mk(ExprNode n) { return std::make_shared<const ExprNode>(std::move(n)); }
+
+Expr Expr::input(int k, DType storage) {
+  ExprNode n;
+  n.kind = ExprNode::Input;
+  n.input = k;
+  n.storage = storage;
+  n.dt = compute_dtype(storage);
+  return Expr{mk(n)};
+}
+
+Expr Expr::constant(const Scalar &s) {
+  ExprNode n;
+  n.kind = ExprNode::Const;
+  n.value = s;
+  n.dt = s.dt;
+  return Expr{mk(n)};
+}
+
+DType Expr::dtype() const { return node->dt; }
+
+static Expr cast_to(const Expr &e, DType dt) {
+  if (e.dtype() == dt) return e;
+  if (e.node->kind == ExprNode::Const) {  // fold literal conversions
+    Scalar s = e.node->value;
+    Scalar r;
+    if (dt == DType::F64) r = Scalar::f64(s.dt == DType::I32 || s.dt == DType::Bool ? (double)s.i : s.f);
+    else if (dt == DType::F32) r = Scalar::f32(s.dt == DType::I32 || s.dt == DType::Bool ? (double)s.i : s.f);
+    else if (dt == DType::I32) r = Scalar::i32(s.dt == DType::I32 || s.dt == DType::Bool ? s.i : (int64_t)s.f);
+    else r = Scalar::boolean(s.dt == DType::I32 || s.dt == DType::Bool ? s.i != 0 : s.f != 0.0);
+    return Expr::constant(r);
+  }
+  ExprNode n;
+  n.kind = ExprNode::Op;
+  n.op = PH_OP_CAST;
+  n.dt = dt;
+  n.opdt = e.dtype();
+  n.a = e.node;
+  n.nargs = 1;
+  return Expr{mk(n)};
+}
+
+static bool is_float(DType d) { return d == DType::F32 || d == DType::F64; }
+
+// S1: Int32 op Int32 -> Int32; any Float64 -> Float64; otherwise Float32.
+static DType arith_type(DType a, DType b) {
+  if (a == DType::I32 && b == DType::I32) return DType::I32;
+  if (a == DType::F64 || b == DType::F64) return DType::F64;
+  return DType::F32;
+}
+
+static Expr op_node(int op, DType dt, DType opdt, const Expr &a, const Expr *b, const Expr *c = nullptr) {
+  ExprNode n;
+  n.kind = ExprNode::Op;
+  n.op = op;
+  n.dt = dt;
+  n.opdt = opdt;
+  n.a = a.node;
+  n.nargs = 1;
+  if (b) {
+    n.b = b->node;
+    n.nargs = 2;
+  }
+  if (c) {
+    n.c = c->node;
+    n.nargs = 3;
+  }
+  return Expr{mk(n)};
+}
+
+static void no_bool(const std::string &op, const Expr &a, const Expr &b) {
+  if (a.dtype() == DType::Bool || b.dtype() == DType::Bool)
+    throw ArgumentError("undefined method '" + op + "' for Bool");
+}
+
+Expr Expr::binary(const std::string &op, const Expr &a, const Expr &b) {
+  if (op == "+" || op == "-" || op == "*") {
+    no_bool(op, a, b);
+    DType t = arith_type(a.dtype(), b.dtype());
+    int code = op == "+" ? PH_OP_ADD : op == "-" ? PH_OP_SUB : PH_OP_MUL;
+    Expr x = cast_to(a, t), y = cast_to(b, t);
+    return op_node(code, t, t, x, &y);
+  }
+  if (op == "&+" || op == "&-" || op == "&*" || op == "&**") {
+    if (a.dtype() != DType::I32 || b.dtype() != DType::I32)
+      throw ArgumentError("wrapping operator '" + op + "' needs Int32 operands");
+    int code = op == "&+" ? PH_OP_WADD : op == "&-" ? PH_OP_WSUB : op == "&*" ? PH_OP_WMUL : PH_OP_WIPOW;
+    return op_node(code, DType::I32, DType::I32, a, &b);
+  }
+  if (op == "/") {
+    no_bool(op, a, b);
+    DType t = arith_type(a.dtype(), b.dtype());
+    if (t == DType::I32) t = DType::F64;  // Int32 / Int32 -> Float64
+    Expr x = cast_to(a, t), y = cast_to(b, t);
+    return op_node(PH_OP_DIV, t, t, x, &y);
+  }
+  if (op == "//" || op == "%") {
+    no_bool(op, a, b);
+    DType t = arith_type(a.dtype(), b.dtype());
+    Expr x = cast_to(a, t), y = cast_to(b, t);
+    return op_node(op == "//" ? PH_OP_FLOORDIV : PH_OP_MOD, t, t, x, &y);
+  }
+  if (op == "**") {
+    no_bool(op, a, b);
+    if (a.dtype() == DType::I32 && b.dtype() == DType::I32) return op_node(PH_OP_IPOW, DType::I32, DType::I32, a, &b);
+    if (is_float(a.dtype()) && b.dtype() == DType::I32) return op_node(PH_OP_POWI, a.dtype(), a.dtype(), a, &b);
+    DType t = (a.dtype() == DType::F32 && b.dtype() == DType::F32) ? DType::F32 : DType::F64;
+    Expr x = cast_to(a, t), y = cast_to(b, t);
+    return op_node(PH_OP_POW, t, t, x, &y);
+  }
+  if (op == "&" || op == "|" || op == "^") {
+    int code = op == "&" ? PH_OP_AND : op == "|" ? PH_OP_OR : PH_OP_XOR;
+    if (a.dtype() == DType::I32 && b.dtype() == DType::I32) return op_node(code, DType::I32, DType::I32, a, &b);
+    if (a.dtype() == DType::Bool && b.dtype() == DType::Bool) return op_node(code, DType::Bool, DType::Bool, a, &b);
+    throw ArgumentError("undefined method '" + op + "' for these operand types");
+  }
+  if (op == ">" || op == "<" || op == ">=" || op == "<=" || op == "==" || op == "!=" || op == "<=>") {
+    DType t;
+    if (a.dtype() == DType::Bool || b.dtype() == DType::Bool) {
+      if (!(a.dtype() == DType::Bool && b.dtype() == DType::Bool) || !(op == "==" || op == "!="))
+        throw ArgumentError("undefined method '" + op + "' for Bool");
+      t = DType::Bool;
+    } else {
+      t = (is_float(a.dtype()) || is_float(b.dtype())) ? DType::F64 : DType::I32;
+    }
+    Expr x = cast_to(a, t), y = cast_to(b, t);
+    static const std::map<std::string, int> codes = {{">", PH_OP_GT}, {"<", PH_OP_LT}, {">=", PH_OP_GE},
+                                                     {"<=", PH_OP_LE}, {"==", PH_OP_EQ}, {"!=", PH_OP_NE},
+                                                     {"<=>", PH_OP_CMP}};
+    int code = codes.at(op);
+    return op_node(code, code == PH_OP_CMP ? DType::I32 : DType::Bool, t, x, &y);
+  }
+  throw ArgumentError("unknown operator '" + op + "'");
+}
+
+Expr Expr::unary(const std::string &op, const Expr &a) {
+  if (op == "+") return a;
+  if (op == "-") {
+    if (a.dtype() == DType::Bool) throw ArgumentError("undefined method '-' for Bool");
+    return op_node(PH_OP_NEG, a.dtype(), a.dtype(), a, nullptr);
+  }
+  if (op == "~") {
+    if (a.dtype() != DType::I32) throw ArgumentError("undefined method '~'");
+    return op_node(PH_OP_BNOT, DType::I32, DType::I32, a, nullptr);
+  }
+  if (op == "abs") {
+    if (a.dtype() == DType::Bool) throw ArgumentError("undefined method 'abs' for Bool");
+    return op_node(PH_OP_ABS, a.dtype(), a.dtype(), a, nullptr);
+  }
+  if (op == "sqrt") {  // Math.sqrt: Int -> Float64
+    Expr x = a.dtype() == DType::I32 ? cast_to(a, DType::F64) : a;
+    if (!is_float(x.dtype())) throw ArgumentError("sqrt of Bool");
+    return op_node(PH_OP_SQRT, x.dtype(), x.dtype(), x, nullptr);
+  }
+  if (op == "to_f32") return cast_to(a, DType::F32);
+  if (op == "to_f64" || op == "to_f") return cast_to(a, DType::F64);
+  if (op == "to_i32" || op == "to_i") {
+    if (a.dtype() != DType::I32 && a.dtype() != DType::Bool)
+      throw ArgumentError("to_i32 of a float is not supported on the device path");
+    return cast_to(a, DType::I32);
+  }
+  throw ArgumentError("unknown unary operator '" + op + "'");
+}
+
+Expr Expr::select(const Expr &cond, const Expr &a, const Expr &b) {
+  if (cond.dtype() != DType::Bool) throw ArgumentError("select condition must be Bool");
+  DType t = a.dtype() == b.dtype() ? a.dtype() : arith_type(a.dtype(), b.dtype());
+  Expr x = cast_to(a, t), y = cast_to(b, t);
+  return op_node(PH_OP_SELECT, t, t, x, &y, &cond);
+}
+
+static std::shared_ptr<const ExprNode> subst(const std::shared_ptr<const ExprNode> &n,
+                                             const std::vector<Expr> &ex) {
+  if (!n) return n;
+  if (n->kind == ExprNode::Input) {
+    if (n->input >= (int)ex.size()) throw ArgumentError("expression input out of range");
+    const Expr &r = ex[(size_t)n->input];
+    if (compute_dtype(n->storage) != r.dtype()) return cast_to(r, compute_dtype(n->storage)).node;
+    return r.node;
+  }
+  if (n->kind == ExprNode::Const) return n;
+  ExprNode c = *n;
+  c.a = subst(n->a, ex);
+  c.b = subst(n->b, ex);
+  c.c = subst(n->c, ex);
+  return mk(c);
+}
+
+Expr Expr::substitute(const std::vector<Expr> &exprs) const { return Expr{subst(node, exprs)}; }
+
+static int arity_of(const std::shared_ptr<const ExprNode> &n) {
+  if (!n) return 0;
+  if (n->kind == ExprNode::Input) return n->input + 1;
+  return std::max(arity_of(n->a), std::max(arity_of(n->b), arity_of(n->c)));
+}
+int Expr::arity() const { return arity_of(node); }
+
+static std::string to_s_node(const std::shared_ptr<const ExprNode> &n) {
+  if (n->kind == ExprNode::Input) return "x" + std::to_string(n->input);
+  if (n->kind == ExprNode::Const) return n->value.to_s();
+  std::string s = "(op" + std::to_string(n->op) + " " + to_s_node(n->a);
+  if (n->b) s += " " + to_s_node(n->b);
+  if (n->c) s += " " + to_s_node(n->c);
+  return s + ")";
+}
+std::string Expr::to_s() const { return to_s_node(node); }
+
+namespace {
+struct Lowering {
+  ph_prog prog;
+  bool used[PH_NREG] = {false};
+  int alloc() {
+    for (int r = 0; r < PH_NREG; ++r)
+      if (!used[r]) {
+        used[r] = true;
+        return r;
+      }
+    throw ArgumentError("element expression needs more than 8 registers");
+  }
+  void release(int r) { used[r] = false; }
+  void emit(const ph_insn &in) {
+    if (prog.n_insn >= PH_MAX_INSN) throw ArgumentError("element expression too long");
+    prog.insn[prog.n_insn++] = in;
+  }
+  static int need(const std::shared_ptr<const ExprNode> &n) {
+    if (n->kind != ExprNode::Op) return 1;
+    int na = need(n->a);
+    if (n->nargs == 1) return na;
+    int nb = need(n->b);
+    int nc = n->nargs == 3 ? need(n->c) : 0;
+    int m = std::max(na, std::max(nb, nc));
+    return (na == nb) ? m + 1 : m;
+  }
+  int lower(const std::shared_ptr<const ExprNode> &n) {
+    ph_insn in;
+    std::memset(&in, 0, sizeof(in));
+    if (n->kind == ExprNode::Input) {
+      in.op = PH_OP_LOAD;
+      in.dt = (uint8_t)compute_dtype(n->storage);
+      in.a = (uint8_t)n->input;
+      in.dst = (uint8_t)alloc();
+      emit(in);
+      return in.dst;
+    }
+    if (n->kind == ExprNode::Const) {
+      in.op = PH_OP_CONST;
+      in.dt = (uint8_t)n->dt;
+      if (n->dt == DType::I32 || n->dt == DType::Bool) in.imm.i = n->value.i;
+      else in.imm.f = n->value.f;
+      in.dst = (uint8_t)alloc();
+      emit(in);
+      return in.dst;
+    }
+    int ra = -1, rb = -1, rc = -1;
+    if (n->nargs == 1) {
+      ra = lower(n->a);
+    } else {
+      if (need(n->b) > need(n->a)) {
+        rb = lower(n->b);
+        ra = lower(n->a);
+      } else {
+        ra = lower(n->a);
+        rb = lower(n->b);
+      }
+      if (n->nargs == 3) rc = lower(n->c);
+    }
+    in.op = (uint8_t)n->op;
+    in.dt = (uint8_t)(n->op == PH_OP_CAST ? n->dt : n->opdt);
+    if (n->op == PH_OP_POWI) in.dt = (uint8_t)n->dt;
+    in.a = (uint8_t)ra;
+    in.b = (uint8_t)(rb < 0 ? 0 : rb);
+    in.c = (uint8_t)(rc < 0 ? 0 : rc);
+    if (n->op == PH_OP_CAST) in.c = (uint8_t)n->opdt;
+    release(ra);
+    if (rb >= 0) release(rb);
+    if (rc >= 0) release(rc);
+    in.dst = (uint8_t)alloc();
+    emit(in);
+    return in.dst;
+  }
+};
+}  // namespace
+
+ph_prog Expr::compile(int n_in) const {
+  Lowering L;
+  std::memset(&L.prog, 0, sizeof(L.prog));
+  int r = L.lower(node);
+  L.prog.n_in = n_in;
+  L.prog.out_reg = r;
+  L.prog.out_dtype = (int32_t)node->dt;
+  return L.prog;
+}
+
+// ===========================================================================
+// Storage / descriptors
+// ===========================================================================
+Storage::~Storage() {
+  if (ptr) ph_free(ptr);
+}
+
+static ph_view_desc contiguous_desc(DType dt, const Shape &shape, int64_t extent) {
+  ph_view_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.dtype = (int32_t)dt;
+  d.extent = extent;
+  d.view.rank = (int32_t)shape.size();
+  int64_t st = 1;
+  for (int i = (int)shape.size() - 1; i >= 0; --i) {
+    d.view.shape[i] = shape[(size_t)i];
+    d.view.stride[i] = st;
+    st *= shape[(size_t)i];
+  }
+  return d;
+}
+
+void Storage::materialize() {
+  if (!pending_fill) return;
+  Scalar v = *pending_fill;
+  pending_fill.reset();
+  if (elements == 0) return;
+  ph_view_desc d = contiguous_desc(dtype, {elements}, elements);
+  unsigned char bits[8];
+  scalar_bits(v, dtype, bits);
+  check(ph_fill(ptr, &d, bits), "NArray.fill");
+}
+
+static std::shared_ptr<Storage> new_storage(DType dt, int64_t elements) {
+  init();
+  auto s = std::make_shared<Storage>();
+  s->dtype = dt;
+  s->elements = elements;
+  s->bytes = (size_t)elements * dtype_size(dt);
+  check(ph_alloc(&s->ptr, s->bytes), "NArray allocation");
+  return s;
+}
+
+static void check_shape(const Shape &shape) {
+  for (int64_t d : shape)
+    if (d < 0) throw DimensionError("Cannot create NArray: One or more of the provided dimensions was negative.");
+  if (shape.size() > PH_MAX_RANK) throw DimensionError("at most 8 dimensions are supported on the device");
+}
+
+// host evaluation of a descriptor at one coordinate (element access)
+static int64_t desc_index(const ph_view_desc &d, const Coord &c) {
+  int64_t idx = d.view.offset;
+  for (int i = 0; i < d.view.rank; ++i) idx += c[(size_t)i] * d.view.stride[i];
+  for (int s = 0; s < d.nstage; ++s) {
+    const ph_affine &st = d.stage[s];
+    int64_t out = st.offset;
+    for (int j = st.rank - 1; j >= 0; --j) {
+      int64_t q = idx / st.shape[j];
+      out += (idx - q * st.shape[j]) * st.stride[j];
+      idx = q;
+    }
+    idx = out;
+  }
+  return idx;
+}
+
+static Scalar fetch_one(const std::shared_ptr<Storage> &st, int64_t idx) {
+  st->materialize();
+  unsigned char bits[8];
+  const size_t es = dtype_size(st->dtype);
+  check(ph_d2h(bits, (const char *)st->ptr + (size_t)idx * es, es), "element read");
+  return scalar_from_bits(st->dtype, bits);
+}
+
+// --- descriptor composition (a new transform applied to view coordinates
+// first, then the existing descriptor) --------------------------------------
+static void compose_region(ph_view_desc &d, const IndexRegion &r) {
+  ph_affine &v = d.view;
+  int64_t off = v.offset;
+  ph_affine nv;
+  std::memset(&nv, 0, sizeof(nv));
+  int k = 0;
+  for (int i = 0; i < v.rank; ++i) {
+    off += r.first[(size_t)i] * v.stride[i];
+    if (!(r.drop && r.degeneracy[(size_t)i])) {
+      nv.shape[k] = r.proper_shape[(size_t)i];
+      nv.stride[k] = r.step[(size_t)i] * v.stride[i];
+      ++k;
+    }
+  }
+  if (k == 0) {  // every axis dropped: one axis of 1 (or 0) elements
+    nv.shape[0] = r.reduced_shape[0];
+    nv.stride[0] = 0;
+    k = 1;
+  }
+  nv.rank = k;
+  nv.offset = off;
+  // an empty axis (step 0) addresses nothing; keep offsets in range
+  for (int i = 0; i < k; ++i)
+    if (nv.shape[i] == 0) nv.offset = 0;
+  v = nv;
+}
+
+static void compose_permute(ph_view_desc &d, const std::vector<int64_t> &pattern) {
+  ph_affine nv = d.view;
+  for (size_t i = 0; i < pattern.size(); ++i) {
+    nv.shape[i] = d.view.shape[pattern[i]];
+    nv.stride[i] = d.view.stride[pattern[i]];
+  }
+  d.view = nv;
+}
+
+static void compose_reverse(ph_view_desc &d) {
+  for (int i = 0; i < d.view.rank; ++i) {
+    if (d.view.shape[i] > 0) d.view.offset += (d.view.shape[i] - 1) * d.view.stride[i];
+    d.view.stride[i] = -d.view.stride[i];
+  }
+}
+
+static bool view_is_rowmajor(const ph_affine &v) {
+  int64_t st = 1;
+  for (int i = v.rank - 1; i >= 0; --i) {
+    if (v.shape[i] != 1 && v.stride[i] != st) return false;
+    st *= v.shape[i];
+  }
+  return true;
+}
+
+static void push_stage(ph_view_desc &d, const ph_affine &stage_map, const ph_affine &new_view) {
+  if (d.nstage >= PH_MAX_STAGES)
+    throw ArgumentError("view chain has more than " + std::to_string(PH_MAX_STAGES) +
+                        " reshape/tile stages; materialise it with to_narr first");
+  for (int s = d.nstage; s > 0; --s) d.stage[s] = d.stage[s - 1];
+  d.stage[0] = stage_map;
+  d.nstage += 1;
+  d.view = new_view;
+}
+
+static void compose_reshape(ph_view_desc &d, const Shape &new_shape) {
+  ph_affine nv;
+  std::memset(&nv, 0, sizeof(nv));
+  nv.rank = (int32_t)new_shape.size();
+  int64_t st = 1;
+  for (int i = nv.rank - 1; i >= 0; --i) {
+    nv.shape[i] = new_shape[(size_t)i];
+    nv.stride[i] = st;
+    st *= new_shape[(size_t)i];
+  }
+  if (view_is_rowmajor(d.view)) {  // linear index maps straight through
+    nv.offset = d.view.offset;
+    d.view = nv;
+    return;
+  }
+  push_stage(d, d.view, nv);
+}
+
+static void compose_tile(ph_view_desc &d, const Shape &src_shape, const std::vector<int64_t> &counts) {
+  bool need_mod = false;
+  for (size_t i = 0; i < src_shape.size(); ++i)
+    if (src_shape[i] != 1 && counts[i] != 1) need_mod = true;
+  ph_affine nv = d.view;
+  for (size_t i = 0; i < src_shape.size(); ++i) {
+    nv.shape[i] = src_shape[i] * counts[i];
+    if (src_shape[i] == 1) nv.stride[i] = 0;  // broadcast
+  }
+  if (!need_mod) {
+    d.view = nv;
+    return;
+  }
+  if (2 * src_shape.size() > PH_MAX_RANK) throw ArgumentError("tile of more than 4 axes needs a materialised copy");
+  // linearise the tiled coordinate, delinearise as [k0, s0, k1, s1, ...]
+  ph_affine stage;
+  std::memset(&stage, 0, sizeof(stage));
+  stage.rank = (int32_t)(2 * src_shape.size());
+  stage.offset = d.view.offset;
+  for (size_t i = 0; i < src_shape.size(); ++i) {
+    stage.shape[2 * i] = counts[i];
+    stage.shape[2 * i + 1] = src_shape[i];
+    stage.stride[2 * i] = 0;
+    stage.stride[2 * i + 1] = d.view.stride[i];
+  }
+  ph_affine lin;
+  std::memset(&lin, 0, sizeof(lin));
+  lin.rank = (int32_t)src_shape.size();
+  int64_t st = 1;
+  for (int i = lin.rank - 1; i >= 0; --i) {
+    lin.shape[i] = nv.shape[i];
+    lin.stride[i] = st;
+    st *= nv.shape[i];
+  }
+  push_stage(d, stage, lin);
+}
+
+// ===========================================================================
+// NArray
+// ===========================================================================
+NArray NArray::empty(const Shape &shape, DType dtype) {
+  check_shape(shape);
+  NArray a;
+  a.shape_ = shape;
+  a.storage = new_storage(dtype, ShapeUtil::shape_to_size(shape));
+  return a;
+}
+
+NArray NArray::fill(const Shape &shape, const Scalar &value, std::optional<DType> dtype) {
+  NArray a = empty(shape, dtype ? *dtype : value.dt);
+  a.storage->pending_fill = value;
+  return a;
+}
+
+NArray NArray::of_buffer(const Shape &shape, DType dtype, const void *host, size_t bytes) {
+  check_shape(shape);
+  const int64_t n = ShapeUtil::shape_to_size(shape);
+  if ((size_t)n * dtype_size(dtype) != bytes)
+    throw ShapeError("Cannot create NArray: Given shape does not match number of elements in buffer.");
+  NArray a = empty(shape, dtype);
+  check(ph_h2d(a.storage->ptr, host, bytes), "NArray.of_buffer");
+  return a;
+}
+
+NArray NArray::random(const Shape &shape, DType dtype, uint64_t seed, uint64_t offset, int dist, double a,
+                      double b) {
+  NArray r = empty(shape, dtype);
+  check(ph_fill_random(r.storage->ptr, (int32_t)dtype, r.size(), seed, offset, dist, a, b), "random fill");
+  return r;
+}
+
+void *NArray::device_ptr() const {
+  storage->materialize();
+  return storage->ptr;
+}
+
+ph_view_desc NArray::desc() const {
+  return contiguous_desc(dtype(), shape_, std::max<int64_t>(storage->elements, 1));
+}
+
+Operand NArray::operand() const {
+  Operand o;
+  o.storage = storage;
+  o.desc = desc();
+  o.shape = shape_;
+  return o;
+}
+
+DType Operand::value_dtype() const { return body ? body->dtype() : compute_dtype((DType)desc.dtype); }
+
+void NArray::to_host(void *out, size_t bytes) const {
+  if (bytes != (size_t)size() * dtype_size(dtype())) throw ArgumentError("to_host: wrong byte count");
+  storage->materialize();
+  check(ph_d2h(out, storage->ptr, bytes), "NArray#to_host");
+}
+
+Scalar NArray::get(const Coord &coord) const {
+  Coord c = CoordUtil::canonicalize_coord(coord, shape_);
+  return fetch_one(storage, desc_index(desc(), c));
+}
+
+void NArray::set_element(const Coord &coord, const Scalar &v) {
+  Coord c = CoordUtil::canonicalize_coord(coord, shape_);
+  storage->materialize();
+  unsigned char bits[8];
+  scalar_bits(v, dtype(), bits);
+  const size_t es = dtype_size(dtype());
+  check(ph_h2d((char *)storage->ptr + (size_t)desc_index(desc(), c) * es, bits, es), "set_element");
+}
+
+static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                       const NArray *out_in);
+
+// gather of an arbitrary operand (view / ProcView) into a new NArray
+static NArray materialize_operand(const Operand &o) {
+  if (o.body) {
+    // ProcView: a map of its block
+    return map_with({o}, [](const std::vector<Expr> &x) { return x[0]; });
+  }
+  NArray out = NArray::empty(o.shape, (DType)o.desc.dtype);
+  if (out.size() == 0) return out;
+  o.storage->materialize();
+  ph_view_desc dd = out.desc();
+  check(ph_strided_copy(out.storage->ptr, &dd, o.storage->ptr, &o.desc, 0), "gather");
+  return out;
+}
+
+NArray NArray::unsafe_fetch_chunk(const IndexRegion &region) const {
+  View v = view();
+  v.restrict_to(region);
+  return v.to_narr();
+}
+
+NArray NArray::get_chunk(const IndexRegion &region) const {
+  if (region.proper_dimensions() != dimensions())
+    throw DimensionError("'region' was " + std::to_string(region.proper_dimensions()) +
+                         "-dimensional, but this MultiIndexable is " + std::to_string(dimensions()) + "-dimensional.");
+  if (!region.fits_in(shape_))
+    throw ShapeError("'region' (" + region.to_s() + ") cannot fit into a MultiIndexable with shape " + shape_to_s(shape_) + ".");
+  return unsafe_fetch_chunk(region);
+}
+
+NArray NArray::get_chunk(const RegionLiteral &lit, bool drop) const {
+  return unsafe_fetch_chunk(IndexRegion::from_literal(lit, shape_, drop));
+}
+
+NArray NArray::get_available(const RegionLiteral &lit, bool drop) const {
+  return unsafe_fetch_chunk(IndexRegion::trimmed(lit, &shape_, drop, shape_));
+}
+
+View NArray::view() const {
+  View v;
+  v.storage = storage;
+  v.src_shape = shape_;
+  v.shape_ = shape_;
+  v.desc_ = desc();
+  return v;
+}
+
+View NArray::view(const RegionLiteral &lit) const {
+  View v = view();
+  v.restrict_to(lit);
+  return v;
+}
+
+MutableView NArray::mutable_view() const {
+  MutableView m;
+  static_cast<View &>(m) = view();
+  return m;
+}
+
+MutableView NArray::mutable_view(const RegionLiteral &lit) const {
+  MutableView m = mutable_view();
+  m.restrict_to(lit);
+  return m;
+}
+
+ProcView NArray::process(const Expr &body) const { return view().process(body); }
+
+View NArray::tile(const std::vector<int64_t> &counts) const {
+  View v = view();
+  v.tile_(counts);
+  return v;
+}
+
+NArray NArray::reshape(const Shape &s) const {
+  check_shape(s);
+  if (ShapeUtil::shape_to_size(s) != size())
+    throw ShapeError("Cannot create NArray: Given shape does not match number of elements in buffer.");
+  NArray r = *this;
+  r.shape_ = s;
+  return r;
+}
+
+NArray NArray::flatten() const { return reshape({size()}); }
+
+NArray NArray::permute(const std::vector<int64_t> *order) const { return view().permute(order).to_narr(); }
+NArray NArray::reverse() const { return view().reverse().to_narr(); }
+NArray NArray::clone() const { return view().to_narr(); }
+
+std::vector<NArray> NArray::slices(int axis) const {
+  if (axis < 0 || axis >= dimensions()) throw IndexError("axis out of range");
+  Shape chunk = shape_;
+  chunk[(size_t)axis] = 1;
+  std::vector<bool> degen((size_t)dimensions(), false);
+  degen[(size_t)axis] = true;
+  std::vector<NArray> out;
+  for (int64_t j = 0; j < shape_[(size_t)axis]; ++j) {
+    IndexRegion r = IndexRegion::cover(chunk, true, &degen);
+    std::vector<int64_t> off((size_t)dimensions(), 0);
+    off[(size_t)axis] = j;
+    r.translate(off);
+    out.push_back(unsafe_fetch_chunk(r));
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// scatter
+// ---------------------------------------------------------------------------
+
+// Reshape `src` to the destination's logical shape when the two are
+// compatible up to trailing ones (ShapeUtil.compatible_shapes?).
+static ph_view_desc fit_src_desc(const ph_view_desc &src, const Shape &dst_shape) {
+  ph_view_desc d = src;
+  if (d.nstage > 0) {
+    // re-express the (equal-size) source in the destination shape: reshape it
+    compose_reshape(d, dst_shape);
+    return d;
+  }
+  // same leading extents; drop or add trailing unit axes
+  ph_affine nv;
+  std::memset(&nv, 0, sizeof(nv));
+  nv.rank = (int32_t)dst_shape.size();
+  nv.offset = src.view.offset;
+  for (size_t i = 0; i < dst_shape.size(); ++i) {
+    nv.shape[i] = dst_shape[i];
+    nv.stride[i] = (int)i < src.view.rank ? src.view.stride[i] : 0;
+  }
+  d.view = nv;
+  return d;
+}
+
+// Whether the destination descriptor is a strided box of the row-major
+// buffer (a region, possibly permuted/reversed): per buffer axis a, first,
+// step, count and which view axis walks it (-1 fixed).
+struct BoxForm {
+  std::vector<int64_t> first, step, count;
+  std::vector<int32_t> axis;
+};
+static std::optional<BoxForm> box_form(const ph_view_desc &d, const Shape &buf_shape) {
+  if (d.nstage != 0) return std::nullopt;
+  const size_t R = buf_shape.size();
+  if (R == 0) return std::nullopt;
+  std::vector<int64_t> rm(R);
+  int64_t st = 1;
+  for (int i = (int)R - 1; i >= 0; --i) {
+    rm[(size_t)i] = st;
+    st *= buf_shape[(size_t)i];
+  }
+  BoxForm f;
+  f.first.assign(R, 0);
+  f.step.assign(R, 1);
+  f.count.assign(R, 1);
+  f.axis.assign(R, -1);
+  int64_t off = d.view.offset;
+  for (size_t a = 0; a < R; ++a) {
+    f.first[a] = off / rm[a];
+    off -= f.first[a] * rm[a];
+  }
+  for (int i = 0; i < d.view.rank; ++i) {
+    const int64_t n = d.view.shape[i], s = d.view.stride[i];
+    if (n == 1) continue;
+    if (n == 0 || s == 0) return std::nullopt;
+    bool found = false;
+    for (size_t a = 0; a < R && !found; ++a) {
+      if (f.axis[a] >= 0 || s % rm[a] != 0) continue;
+      const int64_t step = s / rm[a];
+      const int64_t last = f.first[a] + (n - 1) * step;
+      if (last < 0 || last >= buf_shape[a]) continue;
+      f.axis[a] = i;
+      f.step[a] = step;
+      f.count[a] = n;
+      found = true;
+    }
+    if (!found) return std::nullopt;
+  }
+  // unit view axes map onto fixed buffer axes; give them a view axis when a
+  // buffer axis is free so the source's unit axes are accounted for
+  for (int i = 0; i < d.view.rank; ++i) {
+    if (d.view.shape[i] != 1) continue;
+    for (size_t a = 0; a < R; ++a)
+      if (f.axis[a] < 0) {
+        f.axis[a] = i;
+        f.count[a] = 1;
+        f.step[a] = 1;
+        break;
+      }
+  }
+  // verify: the box must reproduce the descriptor
+  int64_t o = 0;
+  for (size_t a = 0; a < R; ++a) o += f.first[a] * rm[a];
+  if (o != d.view.offset) return std::nullopt;
+  for (int i = 0; i < d.view.rank; ++i) {
+    bool mapped = false;
+    for (size_t a = 0; a < R; ++a)
+      if (f.axis[a] == i) {
+        mapped = true;
+        if (d.view.shape[i] != 1 && f.step[a] * rm[a] != d.view.stride[i]) return std::nullopt;
+      }
+    if (!mapped) return std::nullopt;
+  }
+  return f;
+}
+
+static bool ranges_overlap(const ph_view_desc &a, const ph_view_desc &b) {
+  auto range = [](const ph_view_desc &d, int64_t *lo, int64_t *hi) {
+    if (d.nstage > 0) {
+      *lo = 0;
+      *hi = d.extent - 1;
+      return;
+    }
+    int64_t l = d.view.offset, h = d.view.offset;
+    for (int i = 0; i < d.view.rank; ++i) {
+      int64_t span = (d.view.shape[i] - 1) * d.view.stride[i];
+      if (span < 0) l += span;
+      else h += span;
+    }
+    *lo = l;
+    *hi = h;
+  };
+  int64_t al, ah, bl, bh;
+  range(a, &al, &ah);
+  range(b, &bl, &bh);
+  return !(ah < bl || bh < al);
+}
+
+// dst (storage + view descriptor over buf_shape) <- src, elementwise over the
+// destination's logical shape.
+static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc &dd, const Shape &buf_shape,
+                         const Operand &src_in) {
+  int64_t n = 1;
+  for (int i = 0; i < dd.view.rank; ++i) n *= dd.view.shape[i];
+  if (n == 0) return;
+  Shape dshape(dd.view.shape, dd.view.shape + dd.view.rank);
+  Operand src = src_in;
+  if (src.body || compute_dtype((DType)src.desc.dtype) != compute_dtype((DType)dd.dtype) ||
+      dtype_size((DType)src.desc.dtype) != dtype_size((DType)dd.dtype)) {
+    // a ProcView (or a different element type): evaluate into the target type
+    DType target = (DType)dd.dtype;
+    if (src.value_dtype() != compute_dtype(target))
+      throw ArgumentError(std::string("cannot store ") + dtype_name(src.value_dtype()) + " values into a " +
+                          dtype_name(target) + " array");
+    NArray tmp = materialize_operand(src);
+    src = tmp.operand();
+    src.storage = tmp.storage;
+  }
+  if (src.storage == dst) {
+    // same buffer: the reference reads each source element after the writes
+    // that precede it in lexicographic order; copy the source first unless
+    // the two address ranges are disjoint
+    ph_view_desc sfit = fit_src_desc(src.desc, dshape);
+    if (ranges_overlap(sfit, dd)) {
+      dst->materialize();
+      check(ph_strided_copy_sequential(dst->ptr, &dd, src.storage->ptr, &sfit), "scatter (aliased)");
+      return;
+    }
+  }
+  ph_view_desc sd = fit_src_desc(src.desc, dshape);
+  src.storage->materialize();
+  if (dst->pending_fill && src.storage != dst) {
+    if (auto f = box_form(dd, buf_shape)) {
+      unsigned char bits[8];
+      scalar_bits(*dst->pending_fill, dst->dtype, bits);
+      check(ph_fill_scatter(dst->ptr, (int32_t)dst->dtype, buf_shape.data(), (int32_t)buf_shape.size(),
+                            f->first.data(), f->step.data(), f->count.data(), f->axis.data(), src.storage->ptr,
+                            &sd, bits),
+            "fill + scatter");
+      dst->pending_fill.reset();
+      return;
+    }
+  }
+  dst->materialize();
+  check(ph_strided_copy(dst->ptr, &dd, src.storage->ptr, &sd, 0), "scatter");
+}
+
+void NArray::unsafe_set_chunk(const IndexRegion &region, const Operand &src) {
+  View v = view();
+  v.restrict_to(region);
+  scatter_into(storage, v.desc_, shape_, src);
+}
+
+void NArray::unsafe_set_chunk(const IndexRegion &region, const Scalar &value) {
+  View v = view();
+  v.restrict_to(region);
+  if (v.size() == 0) return;
+  storage->materialize();
+  unsigned char bits[8];
+  scalar_bits(value, dtype(), bits);
+  check(ph_fill(storage->ptr, &v.desc_, bits), "set_chunk (scalar)");
+}
+
+void NArray::set_chunk(const RegionLiteral &lit, const Operand &src) {
+  IndexRegion r = IndexRegion::from_literal(lit, shape_);
+  if (!ShapeUtil::compatible_shapes(src.shape, r.shape()))
+    throw ShapeError("Cannot substitute MultiIndexable: the given MultiIndexable has shape " + shape_to_s(src.shape) +
+                     ", but region " + r.to_s() + " has shape " + shape_to_s(r.shape()) + ".");
+  unsafe_set_chunk(r, src);
+}
+
+void NArray::set_chunk(const RegionLiteral &lit, const Scalar &value) {
+  unsafe_set_chunk(IndexRegion::from_literal(lit, shape_), value);
+}
+
+void NArray::set_mask(const Operand &mask, const Scalar &value) {
+  if (mask.shape != shape_) throw DimensionError("Cannot perform masking: mask shape does not match array shape.");
+  if (mask.value_dtype() != DType::Bool) throw ArgumentError("mask must be Bool");
+  // dst = mask ? value : dst   (one fused pass, in place)
+  Operand self = operand();
+  map_core({mask, self}, [&](const std::vector<Expr> &x) {
+    return Expr::select(x[0], Expr::constant(value), x[1]);
+  }, this);
+}
+
+void NArray::set_mask(const Operand &mask, const Operand &src) {
+  if (mask.shape != shape_) throw DimensionError("Cannot perform masking: mask shape does not match array shape.");
+  if (src.shape != shape_) throw ShapeError("Cannot perform masking: source shape " + shape_to_s(src.shape) +
+                                            " does not match array shape " + shape_to_s(shape_) + ".");
+  if (mask.value_dtype() != DType::Bool) throw ArgumentError("mask must be Bool");
+  Operand self = operand();
+  map_core({mask, src, self}, [&](const std::vector<Expr> &x) { return Expr::select(x[0], x[1], x[2]); }, this);
+}
+
+// ===========================================================================
+// View / MutableView / ProcView
+// ===========================================================================
+Operand View::operand() const {
+  Operand o;
+  o.storage = storage;
+  o.desc = desc_;
+  o.shape = shape_;
+  return o;
+}
+
+View &View::restrict_to(const IndexRegion &region) {
+  shape_ = region.shape();
+  compose_region(desc_, region);
+  Transform t;
+  t.kind = Transform::Region;
+  t.region = region;
+  transforms.insert(transforms.begin(), t);
+  return *this;
+}
+
+View &View::restrict_to(const RegionLiteral &lit) {
+  // ph-core prints "i'm getting called" here (src/view.cr:44); the debug
+  // print is not reproduced.
+  return restrict_to(IndexRegion::from_literal(lit, shape_));
+}
+
+View View::view(const RegionLiteral &lit) const {
+  View v = *this;
+  v.restrict_to(lit);
+  return v;
+}
+
+View &View::reshape_(const Shape &ns) {
+  if (ShapeUtil::shape_to_size(ns) != size())
+    throw ShapeError("Cannot change shape from " + shape_to_s(shape_) + " (" + std::to_string(size()) +
+                     " elements) to " + shape_to_s(ns) + " (" + std::to_string(ShapeUtil::shape_to_size(ns)) +
+                     " elements) because reshape cannot add or remove elements.");
+  if (ns.size() > PH_MAX_RANK) throw DimensionError("at most 8 dimensions");
+  compose_reshape(desc_, ns);
+  Transform t;
+  t.kind = Transform::Reshape;
+  t.a = shape_;
+  t.b = ns;
+  transforms.insert(transforms.begin(), t);
+  shape_ = ns;
+  return *this;
+}
+
+View &View::permute_(const std::vector<int64_t> *order) {
+  std::vector<int64_t> pattern;
+  if (order) {
+    for (int64_t ax : *order)
+      if (ax < 0 || ax >= dimensions())
+        throw IndexError("Could not use pattern " + shape_to_s(*order) + " to permute: Axis " + std::to_string(ax) +
+                         " is not present in a " + std::to_string(dimensions()) + "-dimensional MultiIndexable");
+    pattern = *order;
+    if ((int)pattern.size() != dimensions()) throw DimensionError("permutation pattern has the wrong number of axes");
+    std::vector<bool> seen((size_t)dimensions(), false);
+    for (int64_t ax : pattern) {
+      if (seen[(size_t)ax]) throw ArgumentError("permutation pattern repeats axis " + std::to_string(ax));
+      seen[(size_t)ax] = true;
+    }
+  } else {
+    for (int i = 0; i < dimensions(); ++i) pattern.push_back(dimensions() - i - 1);
+  }
+  Shape ns(pattern.size());
+  for (size_t i = 0; i < pattern.size(); ++i) ns[i] = shape_[(size_t)pattern[i]];
+  compose_permute(desc_, pattern);
+  Transform t;
+  t.kind = Transform::Permute;
+  t.a = pattern;
+  transforms.insert(transforms.begin(), t);
+  shape_ = ns;
+  return *this;
+}
+
+View &View::reverse_() {
+  compose_reverse(desc_);
+  Transform t;
+  t.kind = Transform::Reverse;
+  t.a = shape_;
+  transforms.insert(transforms.begin(), t);
+  return *this;
+}
+
+View &View::tile_(const std::vector<int64_t> &counts) {
+  if (counts.size() != shape_.size())
+    throw DimensionError("tile counts " + shape_to_s(counts) + " do not match a " + std::to_string(dimensions()) +
+                         "-dimensional MultiIndexable");
+  for (int64_t c : counts)
+    if (c < 0) throw ArgumentError("negative tile count");
+  Shape ns(shape_.size());
+  for (size_t i = 0; i < ns.size(); ++i) ns[i] = shape_[i] * counts[i];
+  compose_tile(desc_, shape_, counts);
+  Transform t;
+  t.kind = Transform::Tile;
+  t.a = shape_;
+  t.b = counts;
+  transforms.insert(transforms.begin(), t);
+  shape_ = ns;
+  return *this;
+}
+
+View View::reshape(const Shape &s) const {
+  View v = *this;
+  v.reshape_(s);
+  return v;
+}
+View View::permute(const std::vector<int64_t> *order) const {
+  View v = *this;
+  v.permute_(order);
+  return v;
+}
+View View::reverse() const {
+  View v = *this;
+  v.reverse_();
+  return v;
+}
+
+NArray View::to_narr() const { return materialize_operand(operand()); }
+
+ProcView View::process(const Expr &body) const {
+  ProcView p;
+  static_cast<View &>(p) = *this;
+  if (body.arity() > 1) throw ArgumentError("a ProcView block takes one element");
+  p.body = body;
+  return p;
+}
+
+Coord View::apply(const Coord &coord) const {
+  // ComposedTransform#apply (transforms.cr:77-91): newest first
+  Coord c = coord;
+  for (const Transform &t : transforms) {
+    switch (t.kind) {
+      case Transform::Region: c = t.region.local_to_absolute_unsafe(c); break;
+      case Transform::Reshape: {
+        Shape vs(t.b.size());
+        int64_t st = 1;
+        for (int i = (int)t.b.size() - 1; i >= 0; --i) {
+          vs[(size_t)i] = st;
+          st *= t.b[(size_t)i];
+        }
+        int64_t index = 0;
+        for (size_t i = 0; i < c.size(); ++i) index += c[i] * vs[i];
+        Coord o(t.a.size());
+        for (int d = (int)t.a.size() - 1; d >= 0; --d) {
+          o[(size_t)d] = floormod(index, t.a[(size_t)d]);
+          index = floordiv(index, t.a[(size_t)d]);
+        }
+        c = o;
+        break;
+      }
+      case Transform::Permute: {
+        Coord o = c;
+        for (size_t i = 0; i < t.a.size(); ++i) o[(size_t)t.a[i]] = c[i];
+        c = o;
+        break;
+      }
+      case Transform::Reverse:
+        for (size_t i = 0; i < c.size(); ++i) c[i] = t.a[i] - 1 - c[i];
+        break;
+      case Transform::Tile:
+        for (size_t i = 0; i < c.size(); ++i) c[i] = floormod(c[i], t.a[i]);
+        break;
+    }
+  }
+  return c;
+}
+
+Scalar View::get(const Coord &coord) const {
+  Coord c = CoordUtil::canonicalize_coord(coord, shape_);
+  return fetch_one(storage, desc_index(desc_, c));
+}
+
+MutableView MutableView::mutable_view(const RegionLiteral &lit) const {
+  MutableView m = *this;
+  m.restrict_to(lit);
+  return m;
+}
+
+void MutableView::unsafe_set_chunk(const IndexRegion &region, const Operand &src) {
+  View v = *this;
+  v.restrict_to(region);
+  scatter_into(storage, v.desc_, src_shape, src);
+}
+
+void MutableView::set_chunk(const RegionLiteral &lit, const Operand &src) {
+  IndexRegion r = IndexRegion::from_literal(lit, shape_);
+  if (!ShapeUtil::compatible_shapes(src.shape, r.shape()))
+    throw ShapeError("Cannot substitute MultiIndexable: the given MultiIndexable has shape " + shape_to_s(src.shape) +
+                     ", but region " + r.to_s() + " has shape " + shape_to_s(r.shape()) + ".");
+  unsafe_set_chunk(r, src);
+}
+
+void MutableView::set_chunk(const RegionLiteral &lit, const Scalar &value) {
+  View v = *this;
+  v.restrict_to(IndexRegion::from_literal(lit, shape_));
+  if (v.size() == 0) return;
+  storage->materialize();
+  unsigned char bits[8];
+  scalar_bits(value, dtype(), bits);
+  check(ph_fill(storage->ptr, &v.desc_, bits), "MutableView set_chunk (scalar)");
+}
+
+void MutableView::set_element(const Coord &coord, const Scalar &v) {
+  Coord c = CoordUtil::canonicalize_coord(coord, shape_);
+  storage->materialize();
+  unsigned char bits[8];
+  scalar_bits(v, dtype(), bits);
+  const size_t es = dtype_size(dtype());
+  check(ph_h2d((char *)storage->ptr + (size_t)desc_index(desc_, c) * es, bits, es), "set_element");
+}
+
+Operand ProcView::operand() const {
+  Operand o = View::operand();
+  o.body = body;
+  return o;
+}
+
+NArray ProcView::to_narr() const { return materialize_operand(operand()); }
+
+ProcView ProcView::process(const Expr &next) const {
+  ProcView p = *this;
+  p.body = next.substitute({body});
+  return p;
+}
+
+Scalar ProcView::get(const Coord &coord) const {
+  // evaluate the block on the device for one element (a 1-element map)
+  Coord c = CoordUtil::canonicalize_coord(coord, shape_);
+  View one = *this;
+  RegionLiteral lit;
+  for (int64_t x : c) lit.push_back(RangeLiteral::of_index(x));
+  one.restrict_to(IndexRegion::from_literal(lit, shape_));
+  Operand o = one.operand();
+  o.body = body;
+  NArray r = materialize_operand(o);
+  return r.get({0});
+}
+
+// ===========================================================================
+// Elementwise
+// ===========================================================================
+// Fused map over several operands of one shape, into `out` (a contiguous
+// NArray of the result type), or into a new NArray when out is null.
+static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                       const NArray *out_in) {
+  if (ops.empty()) throw ArgumentError("map_with needs an operand");
+  for (size_t i = 1; i < ops.size(); ++i)
+    if (ops[i].shape != ops[0].shape)
+      throw ShapeError("Could not simultaneously map MultiIndexables with shapes " + shape_to_s(ops[i].shape) +
+                       " and " + shape_to_s(ops[0].shape) + ".");
+  // kernel inputs: distinct (storage, descriptor) pairs
+  std::vector<const Operand *> inputs;
+  std::vector<Expr> leaf;
+  for (const Operand &o : ops) {
+    int idx = -1;
+    for (size_t k = 0; k < inputs.size(); ++k)
+      if (inputs[k]->storage == o.storage && std::memcmp(&inputs[k]->desc, &o.desc, sizeof(ph_view_desc)) == 0)
+        idx = (int)k;
+    if (idx < 0) {
+      if (inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
+      inputs.push_back(&o);
+      idx = (int)inputs.size() - 1;
+    }
+    Expr e = Expr::input(idx, (DType)o.desc.dtype);
+    if (o.body) e = o.body->substitute({e});
+    leaf.push_back(e);
+  }
+  Expr body = fn(leaf);
+  NArray out;
+  if (out_in) {
+    out = *out_in;
+    if (compute_dtype(out.dtype()) != body.dtype() || out.dtype() == DType::BF16)
+      throw ArgumentError(std::string("cannot store ") + dtype_name(body.dtype()) + " values into a " +
+                          dtype_name(out.dtype()) + " array");
+  } else {
+    out = NArray::empty(ops[0].shape, body.dtype());
+  }
+  if (out.size() == 0) return out;
+  ph_prog prog = body.compile((int)inputs.size());
+  std::vector<const void *> ptrs;
+  std::vector<ph_view_desc> descs;
+  for (const Operand *o : inputs) {
+    o->storage->materialize();
+    ptrs.push_back(o->storage->ptr);
+    descs.push_back(o->desc);
+  }
+  out.storage->materialize();
+  ph_view_desc od = out.desc();
+  int32_t flags = 0;
+  check(ph_map(&prog, (int32_t)inputs.size(), ptrs.data(), descs.data(), out.storage->ptr, &od, &flags), "map");
+  check_flags(flags, "elementwise");
+  return out;
+}
+
+NArray map_with(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn) {
+  return map_core(ops, fn, nullptr);
+}
+
+static void same_shape(const std::string &op, const Operand &a, const Operand &b) {
+  if (a.shape != b.shape) {
+    std::string m = "The shape of this MultiIndexable (" + shape_to_s(a.shape) +
+                    ") does not match the shape of the one provided (" + shape_to_s(b.shape) + "), so '" + op +
+                    "' cannot be applied element-wise.";
+    if (ShapeUtil::shape_to_size(a.shape) == 1 || ShapeUtil::shape_to_size(b.shape) == 1)
+      m.replace(m.size() - 1, 1, ". Did you mean to call to_scalar on one of the arguments?");
+    throw ShapeError(m);
+  }
+}
+
+NArray elementwise(const std::string &op, const Operand &a, const Operand &b) {
+  same_shape(op, a, b);
+  return map_with({a, b}, [&](const std::vector<Expr> &x) { return Expr::binary(op, x[0], x[1]); });
+}
+
+NArray elementwise(const std::string &op, const Operand &a, const Scalar &b) {
+  return map_with({a}, [&](const std::vector<Expr> &x) { return Expr::binary(op, x[0], Expr::constant(b)); });
+}
+
+NArray elementwise(const std::string &op, const Scalar &a, const Operand &b) {
+  return map_with({b}, [&](const std::vector<Expr> &x) { return Expr::binary(op, Expr::constant(a), x[0]); });
+}
+
+NArray elementwise_unary(const std::string &op, const Operand &a) {
+  return map_with({a}, [&](const std::vector<Expr> &x) { return Expr::unary(op, x[0]); });
+}
+
+NArray map(const Operand &a, const std::function<Expr(const Expr &)> &fn) {
+  return map_with({a}, [&](const std::vector<Expr> &x) { return fn(x[0]); });
+}
+
+NArray &NArray::map_inplace(const std::function<Expr(const Expr &)> &fn) {
+  Operand self = operand();
+  Expr body = fn(Expr::input(0, dtype()));
+  if (body.dtype() != compute_dtype(dtype()))
+    throw ArgumentError(std::string("map! cannot change the element type (") + dtype_name(dtype()) + " -> " +
+                        dtype_name(body.dtype()) + ")");
+  if (dtype() == DType::BF16) throw ArgumentError("map! on BFloat16 storage is not supported");
+  if (size() == 0) return *this;
+  storage->materialize();
+  ph_prog prog = body.compile(1);
+  ph_view_desc d = desc();
+  const void *ptrs[1] = {storage->ptr};
+  int32_t flags = 0;
+  check(ph_map(&prog, 1, ptrs, &d, storage->ptr, &d, &flags), "map!");
+  check_flags(flags, "map!");
+  return *this;
+}
+
+NArray eq(const Operand &a, const Operand &b) {
+  if (a.shape != b.shape)
+    throw DimensionError("Cannot compute the element-wise equality between this MultiIndexable (shape: " +
+                         shape_to_s(a.shape) + ") and the one provided (shape: " + shape_to_s(b.shape) + ").");
+  return map_with({a, b}, [](const std::vector<Expr> &x) { return Expr::binary("==", x[0], x[1]); });
+}
+
+// ===========================================================================
+// Reductions
+// ===========================================================================
+DeviceDoubles::DeviceDoubles(int64_t n_) : n(n_) {
+  init();
+  void *p = nullptr;
+  check(ph_alloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(double)), "result buffer");
+  ptr = (double *)p;
+}
+DeviceDoubles::~DeviceDoubles() {
+  if (ptr) ph_free(ptr);
+}
+std::vector<double> DeviceDoubles::to_host() const {
+  std::vector<double> h((size_t)n);
+  if (n) check(ph_d2h(h.data(), ptr, (size_t)n * sizeof(double)), "result copy");
+  return h;
+}
+
+struct Prepared {
+  ph_prog prog;
+  std::vector<const void *> ptrs;
+  std::vector<ph_view_desc> descs;
+  DType vdt;
+};
+
+static Prepared prepare(const Operand &a) {
+  Prepared p;
+  Expr e = Expr::input(0, (DType)a.desc.dtype);
+  if (a.body) e = a.body->substitute({e});
+  if (e.dtype() == DType::Bool) throw ArgumentError("undefined method '+' for Bool (sum of a Bool MultiIndexable)");
+  p.vdt = e.dtype();
+  p.prog = e.compile(1);
+  a.storage->materialize();
+  p.ptrs.push_back(a.storage->ptr);
+  p.descs.push_back(a.desc);
+  return p;
+}
+
+static int64_t count_of(const Operand &a) { return ShapeUtil::shape_to_size(a.shape); }
+
+Scalar sum(const Operand &a, Acc acc) {
+  if (count_of(a) == 0) {
+    Expr e = Expr::input(0, (DType)a.desc.dtype);
+    if (a.body) e = a.body->substitute({e});
+    return e.dtype() == DType::I32 ? Scalar::i32(0) : Scalar::f64(0.0);
+  }
+  Prepared p = prepare(a);
+  DeviceDoubles out(1);
+  int32_t flags = 0;
+  check(ph_reduce(&p.prog, 1, p.ptrs.data(), p.descs.data(), -1, PH_RED_SUM, (int32_t)acc, out.ptr, nullptr, &flags),
+        "sum");
+  check_flags(flags, "sum");
+  double v = out.to_host()[0];
+  if (p.vdt == DType::I32) return Scalar::i32((int64_t)v);
+  return Scalar::f64(v);
+}
+
+Scalar max(const Operand &a) {
+  if (count_of(a) == 0) throw EmptyError("Empty enumerable");
+  Prepared p = prepare(a);
+  DeviceDoubles out(1);
+  int32_t flags = 0;
+  check(ph_reduce(&p.prog, 1, p.ptrs.data(), p.descs.data(), -1, PH_RED_MAX, PH_ACC_F64, nullptr, out.ptr, &flags),
+        "max");
+  double v = out.to_host()[0];
+  if (p.vdt == DType::I32) return Scalar::i32((int64_t)v);
+  if (p.vdt == DType::F32) return Scalar::f32(v);
+  return Scalar::f64(v);
+}
+
+std::vector<double> slices_sum(const Operand &a, int axis, Acc acc) {
+  if (axis < 0 || axis >= (int)a.shape.size()) throw IndexError("axis out of range");
+  const int64_t n = a.shape[(size_t)axis];
+  if (n == 0) return {};
+  if (count_of(a) == 0) return std::vector<double>((size_t)n, 0.0);
+  Prepared p = prepare(a);
+  DeviceDoubles out(n);
+  int32_t flags = 0;
+  check(ph_reduce(&p.prog, 1, p.ptrs.data(), p.descs.data(), axis, PH_RED_SUM, (int32_t)acc, out.ptr, nullptr, &flags),
+        "slices sum");
+  check_flags(flags, "slices sum");
+  return out.to_host();
+}
+
+std::vector<double> slices_max(const Operand &a, int axis) {
+  if (axis < 0 || axis >= (int)a.shape.size()) throw IndexError("axis out of range");
+  const int64_t n = a.shape[(size_t)axis];
+  if (n == 0) return {};
+  if (count_of(a) == 0) throw EmptyError("Empty enumerable");
+  Prepared p = prepare(a);
+  DeviceDoubles out(n);
+  int32_t flags = 0;
+  check(ph_reduce(&p.prog, 1, p.ptrs.data(), p.descs.data(), axis, PH_RED_MAX, PH_ACC_F64, nullptr, out.ptr, &flags),
+        "slices max");
+  return out.to_host();
+}
+
+Reduce2D reduce2d_all(const Operand &a, Acc acc) {
+  if (a.shape.size() != 2) throw DimensionError("reduce2d_all needs a 2-dimensional MultiIndexable");
+  if (count_of(a) == 0) throw EmptyError("Empty enumerable");
+  Prepared p = prepare(a);
+  const int64_t R = a.shape[0], C = a.shape[1];
+  DeviceDoubles buf(2 + 2 * R + 2 * C);
+  int32_t flags = 0;
+  double *b = buf.ptr;
+  check(ph_reduce2d_all(&p.prog, 1, p.ptrs.data(), p.descs.data(), (int32_t)acc, b, b + 1, b + 2, b + 2 + R,
+                        b + 2 + 2 * R, b + 2 + 2 * R + C, &flags),
+        "reduce2d_all");
+  check_flags(flags, "reduce2d_all");
+  std::vector<double> h = buf.to_host();
+  Reduce2D r;
+  r.full_sum = h[0];
+  r.full_max = h[1];
+  r.row_sum.assign(h.begin() + 2, h.begin() + 2 + R);
+  r.row_max.assign(h.begin() + 2 + R, h.begin() + 2 + 2 * R);
+  r.col_sum.assign(h.begin() + 2 + 2 * R, h.begin() + 2 + 2 * R + C);
+  r.col_max.assign(h.begin() + 2 + 2 * R + C, h.end());
+  return r;
+}
+
+bool equals(const Operand &a, const Operand &b) {
+  if (a.shape != b.shape) return false;
+  if (count_of(a) == 0) return true;
+  NArray ne = map_with({a, b}, [](const std::vector<Expr> &x) {
+    return Expr::unary("to_i32", Expr::binary("!=", x[0], x[1]));
+  });
+  Scalar s = sum(ne.operand());
+  return s.i == 0;
+}
+
+// ===========================================================================
+// Heat
+// ===========================================================================
+std::vector<double> heat2d(NArray &u, double r, int64_t steps, int64_t every) {
+  if (u.dimensions() != 2) throw DimensionError("heat2d needs a 2-dimensional NArray");
+  if (u.dtype() != DType::F32 && u.dtype() != DType::F64) throw ArgumentError("heat2d needs Float32 or Float64");
+  NArray tmp = NArray::empty(u.shape(), u.dtype());
+  const int64_t ne = every > 0 ? steps / every + 1 : 0;
+  DeviceDoubles e(std::max<int64_t>(ne, 1));
+  check(ph_heat2d((int32_t)u.dtype(), u.device_ptr(), tmp.device_ptr(), u.shape()[0], u.shape()[1], r, steps,
+                  every, every > 0 ? e.ptr : nullptr),
+        "heat2d");
+  check(ph_sync(), "heat2d sync");
+  if (ne == 0) return {};
+  std::vector<double> h = e.to_host();
+  h.resize((size_t)ne);
+  return h;
+}
+
+}  // namespace phase

@@ -1,0 +1,397 @@
+// Host side of the MI355X backend: a C++ mirror of ph-core's public API for
+// the NArray data-parallel path, on top of the C-ABI in include/phhip.h.
+//
+// Names, argument meaning and exception behaviour follow the reference:
+//   RangeSyntax            src/range_syntax/range_syntax.cr
+//   CoordUtil / ShapeUtil  src/coord_util.cr, src/shape_util.cr
+//   IndexRegion            src/index_region.cr
+//   NArray                 src/n_array.cr + src/multi_indexable.cr +
+//                          src/multi_writable.cr (device-resident buffer)
+//   View / MutableView /   src/view.cr, src/mutable_view.cr, src/proc_view.cr,
+//   ProcView               src/view_util/transforms.cr
+// Validation (canonicalisation, shape checks) happens here, before any call
+// into the device library, so exception types and messages are decided on
+// the host exactly where ph-core decides them.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/phhip.h"
+
+namespace phase {
+
+using Shape = std::vector<int64_t>;
+using Coord = std::vector<int64_t>;
+
+// ---------------------------------------------------------------------------
+// Exceptions — src/exceptions/exceptions.cr:4-22 and Crystal's stdlib ones.
+// ---------------------------------------------------------------------------
+struct PhaseError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct IndexError : PhaseError {
+  using PhaseError::PhaseError;
+};
+struct ShapeError : PhaseError {
+  using PhaseError::PhaseError;
+};
+struct DimensionError : ShapeError {
+  using ShapeError::ShapeError;
+};
+struct ArgumentError : PhaseError {
+  using PhaseError::PhaseError;
+};
+struct OverflowError : PhaseError {
+  using PhaseError::PhaseError;
+};
+struct DivisionByZeroError : PhaseError {
+  using PhaseError::PhaseError;
+};
+struct EmptyError : PhaseError {  // Enumerable::EmptyError
+  using PhaseError::PhaseError;
+};
+struct DeviceError : PhaseError {  // a failure inside the HIP library
+  using PhaseError::PhaseError;
+};
+
+// Raises the exception matching a C-ABI status code.
+[[noreturn]] void raise_status(int code, const std::string &context);
+void check(int code, const char *context);
+
+enum class DType : int32_t { I32 = PH_I32, F32 = PH_F32, F64 = PH_F64, BF16 = PH_BF16, Bool = PH_BOOL };
+const char *dtype_name(DType d);
+size_t dtype_size(DType d);
+DType compute_dtype(DType storage);  // BF16 computes as Float32
+
+// A scalar operand: Crystal literal types (Int32, Float32, Float64, Bool).
+struct Scalar {
+  DType dt = DType::I32;
+  int64_t i = 0;
+  double f = 0.0;
+  static Scalar i32(int64_t v);
+  static Scalar f32(double v);
+  static Scalar f64(double v);
+  static Scalar boolean(bool v);
+  std::string to_s() const;
+};
+
+// ---------------------------------------------------------------------------
+// Range literals — the forms parse_range accepts (range_syntax.cr:41-69):
+// an Int index, `a..c`, `a...c`, `a..b..c` (= `(a..b)..c`, step b),
+// `a..(b..c)`, and `a.step(by: b, to: c, exclusive: e)`.  Written as Crystal
+// source text, e.g. "1..3..-1", "..-2..", "(5..-3)..1", "...", "-2".
+// ---------------------------------------------------------------------------
+struct RangeLiteral {
+  bool is_index = false;
+  int64_t index = 0;
+  std::optional<int64_t> first, last, step;
+  bool exclusive = false;
+  std::string text;
+
+  static RangeLiteral parse(const std::string &text);
+  static RangeLiteral of_index(int64_t i);
+  static RangeLiteral full();  // `..`
+  std::string to_s() const { return text; }
+};
+using RegionLiteral = std::vector<RangeLiteral>;
+
+struct CanonicalRange {
+  int64_t first, step, last, size;
+};
+
+namespace RangeSyntax {
+int64_t get_size(int64_t first, int64_t last, int64_t step);
+CanonicalRange infer_range(const RangeLiteral &lit, int64_t bound);
+CanonicalRange canonicalize_range(const RangeLiteral &lit, int64_t bound);
+bool bounded(const RangeLiteral &lit);
+void ensure_nonnegative(const RangeLiteral &lit);
+}  // namespace RangeSyntax
+
+namespace CoordUtil {
+bool has_index(int64_t index, int64_t size);
+int64_t canonicalize_index(int64_t index, int64_t size);
+Coord canonicalize_coord(const Coord &coord, const Shape &shape);
+}  // namespace CoordUtil
+
+namespace ShapeUtil {
+bool compatible_shapes(const Shape &a, const Shape &b);
+int64_t shape_to_size(const Shape &shape);
+}  // namespace ShapeUtil
+
+std::string shape_to_s(const Shape &s);
+
+// ---------------------------------------------------------------------------
+// IndexRegion — src/index_region.cr
+// ---------------------------------------------------------------------------
+class IndexRegion {
+ public:
+  std::vector<int64_t> first, step, last;
+  Shape proper_shape, reduced_shape;
+  std::vector<bool> degeneracy;
+  bool drop = true;
+
+  // IndexRegion.new(region_literal, bound_shape, drop) (:192-224)
+  static IndexRegion from_literal(const RegionLiteral &lit, const Shape &bound, bool drop = true);
+  // IndexRegion.new(region_literal, drop) — absolute, no bound (:248-275)
+  static IndexRegion absolute(const RegionLiteral &lit, bool drop = true);
+  // IndexRegion.new(region_literal, bound_shape?, drop, trim_to:) (:133-168)
+  static IndexRegion trimmed(const RegionLiteral &lit, const Shape *bound, bool drop, const Shape &trim_to);
+  // IndexRegion.cover (:232-238)
+  static IndexRegion cover(const Shape &shape, bool drop = true, const std::vector<bool> *degeneracy = nullptr);
+  // IndexRegion.new(region, bound_shape) copy that must fit (:109-115)
+  static IndexRegion fitted_copy(const IndexRegion &r, const Shape &bound);
+
+  const Shape &shape() const { return reduced_shape; }
+  int dimensions() const { return (int)reduced_shape.size(); }
+  int proper_dimensions() const { return (int)proper_shape.size(); }
+  int64_t size() const { return ShapeUtil::shape_to_size(reduced_shape); }
+
+  IndexRegion &trim(const Shape &bound);           // trim! (:502-515)
+  IndexRegion &reverse();                          // reverse! (:533-537)
+  IndexRegion &translate(const std::vector<int64_t> &offset);  // translate! (:577-585)
+  IndexRegion &unsafe_translate(const std::vector<int64_t> &offset);
+  bool includes(const Coord &coord) const;         // includes? (:443-456)
+  bool fits_in(const Shape &bound) const;          // fits_in? (:468-478)
+  Coord local_to_absolute_unsafe(const Coord &local) const;  // (:621-637)
+  Coord local_to_absolute(const Coord &local) const;
+  Coord absolute_to_local_unsafe(const Coord &abs) const;    // (:654-664)
+  Coord absolute_to_local(const Coord &abs) const;
+  std::string to_s() const;                        // (:593-604)
+  bool operator==(const IndexRegion &o) const;
+
+ private:
+  void compute_reduced();
+};
+
+// ---------------------------------------------------------------------------
+// Element expressions: the compiled form of a ProcView block or of an
+// elementwise operator, with Crystal's numeric semantics (type promotion,
+// checked Int32 arithmetic, floored // and %, powi for Float ** Int32).
+// ---------------------------------------------------------------------------
+struct ExprNode;
+class Expr {
+ public:
+  std::shared_ptr<const ExprNode> node;
+
+  static Expr input(int k, DType storage);  // element of operand k
+  static Expr constant(const Scalar &s);
+  DType dtype() const;
+  // Crystal operator names: "+", "-", "*", "/", "//", "%", "**", "&+", "&-",
+  // "&*", "&**", "&", "|", "^", ">", "<", ">=", "<=", "==", "!=", "<=>"
+  static Expr binary(const std::string &op, const Expr &a, const Expr &b);
+  // "-", "+", "~", "sqrt", "abs", "to_f32", "to_f64", "to_i32"
+  static Expr unary(const std::string &op, const Expr &a);
+  static Expr select(const Expr &cond, const Expr &a, const Expr &b);
+  // Replace input k by exprs[k] (composition of blocks / operand fusion).
+  Expr substitute(const std::vector<Expr> &exprs) const;
+  // Index of the highest input referenced + 1.
+  int arity() const;
+  std::string to_s() const;
+  // Lower to a ph_prog over `n_in` inputs (storage dtypes given).
+  ph_prog compile(int n_in) const;
+};
+
+// ---------------------------------------------------------------------------
+// Device storage shared by an NArray and the views over it.  A fill can be
+// left pending (NArray.fill): the first scatter into a pending-fill buffer
+// writes the fill and the scattered elements in one pass.
+// ---------------------------------------------------------------------------
+struct Storage {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int64_t elements = 0;
+  DType dtype = DType::F64;
+  std::optional<Scalar> pending_fill;
+  ~Storage();
+  void materialize();  // issue a pending fill
+};
+
+// Coordinate transforms of a view chain (src/view_util/transforms.cr).
+struct Transform {
+  enum Kind { Region, Reshape, Permute, Reverse, Tile } kind;
+  IndexRegion region;          // Region
+  std::vector<int64_t> a, b;   // Reshape: src_shape, new_shape; Permute: pattern;
+                               // Reverse: shape; Tile: src_shape, counts
+};
+
+class View;
+class MutableView;
+class ProcView;
+
+// An operand of a fused elementwise kernel: storage + descriptor (+ the
+// ProcView block applied to it, as an expression over input 0).
+struct Operand {
+  std::shared_ptr<Storage> storage;
+  ph_view_desc desc;
+  Shape shape;
+  std::optional<Expr> body;
+  DType value_dtype() const;
+};
+
+// Device-resident NArray (src/n_array.cr): row-major buffer + shape.
+class NArray {
+ public:
+  std::shared_ptr<Storage> storage;
+  Shape shape_;
+
+  NArray() = default;
+  // NArray.fill(shape, value) (:230-232)
+  static NArray fill(const Shape &shape, const Scalar &value, std::optional<DType> dtype = {});
+  // NArray.of_buffer(shape, buffer) (:76-79), from host memory
+  static NArray of_buffer(const Shape &shape, DType dtype, const void *host, size_t bytes);
+  // Seeded synthetic data (counter-based splitmix64; see ph_fill_random)
+  static NArray random(const Shape &shape, DType dtype, uint64_t seed, uint64_t offset, int dist,
+                       double a, double b);
+  static NArray empty(const Shape &shape, DType dtype);  // uninitialised
+
+  const Shape &shape() const { return shape_; }
+  int dimensions() const { return (int)shape_.size(); }
+  int64_t size() const { return ShapeUtil::shape_to_size(shape_); }
+  DType dtype() const { return storage->dtype; }
+  bool empty_() const { return size() == 0; }
+  bool scalar_() const { return size() == 1; }
+  void *device_ptr() const;  // materialises a pending fill
+  ph_view_desc desc() const;
+  Operand operand() const;
+
+  void to_host(void *out, size_t bytes) const;
+  Scalar get(const Coord &coord) const;           // MultiIndexable#get
+  void set_element(const Coord &coord, const Scalar &v);  // MultiWritable#set_element
+
+  // MultiIndexable
+  NArray get_chunk(const RegionLiteral &lit, bool drop = true) const;  // [] / get_chunk (:354-356)
+  NArray get_chunk(const IndexRegion &region) const;                   // (:338-349)
+  NArray unsafe_fetch_chunk(const IndexRegion &region) const;          // (n_array.cr:450-453)
+  NArray get_available(const RegionLiteral &lit, bool drop = true) const;  // (:411-413)
+  View view(const RegionLiteral &lit) const;                           // (:881-887)
+  View view() const;
+  MutableView mutable_view(const RegionLiteral &lit) const;            // (mutable.cr:6-12)
+  MutableView mutable_view() const;
+  ProcView process(const Expr &body) const;                            // (:889-895)
+  View tile(const std::vector<int64_t> &counts) const;                 // lazy tile (:818-827)
+  NArray reshape(const Shape &shape) const;                            // aliases (n_array.cr:429-433)
+  NArray flatten() const;
+  NArray permute(const std::vector<int64_t> *order = nullptr) const;   // (:795-803)
+  NArray reverse() const;
+  NArray clone() const;
+  std::vector<NArray> slices(int axis = 0) const;                      // (:784-786)
+
+  // MultiWritable
+  void set_chunk(const RegionLiteral &lit, const Operand &src);        // (multi_writable.cr:55-63)
+  void set_chunk(const RegionLiteral &lit, const Scalar &value);       // (:67-70)
+  void unsafe_set_chunk(const IndexRegion &region, const Operand &src);
+  void unsafe_set_chunk(const IndexRegion &region, const Scalar &value);
+  void set_mask(const Operand &mask, const Scalar &value);             // (n_array.cr:510-551)
+  void set_mask(const Operand &mask, const Operand &src);
+
+  // in-place map (map!, n_array.cr:622-628)
+  NArray &map_inplace(const std::function<Expr(const Expr &)> &fn);
+};
+
+// Lazy view (src/view.cr): source storage + a transform chain, folded into a
+// ph_view_desc.
+class View {
+ public:
+  std::shared_ptr<Storage> storage;
+  Shape src_shape;
+  Shape shape_;
+  std::vector<Transform> transforms;  // newest first (compose! unshifts)
+  ph_view_desc desc_;
+
+  const Shape &shape() const { return shape_; }
+  int dimensions() const { return (int)shape_.size(); }
+  int64_t size() const { return ShapeUtil::shape_to_size(shape_); }
+  DType dtype() const { return storage->dtype; }
+  Operand operand() const;
+  const ph_view_desc &desc() const { return desc_; }
+
+  View view(const RegionLiteral &lit) const;
+  View &restrict_to(const RegionLiteral &lit);   // (:43-49)
+  View &restrict_to(const IndexRegion &region);  // (:51-56)
+  View &reshape_(const Shape &shape);            // reshape! (:58-66)
+  View &permute_(const std::vector<int64_t> *order);  // permute! (:72-81)
+  View &reverse_();                              // reverse! (:96-99)
+  View &tile_(const std::vector<int64_t> &counts);
+  View reshape(const Shape &shape) const;
+  View permute(const std::vector<int64_t> *order = nullptr) const;
+  View reverse() const;
+  NArray to_narr() const;                        // (:123-126)
+  ProcView process(const Expr &body) const;      // (:119-121)
+  Coord apply(const Coord &coord) const;         // ComposedTransform#apply
+  Scalar get(const Coord &coord) const;
+};
+
+class MutableView : public View {
+ public:
+  MutableView mutable_view(const RegionLiteral &lit) const;
+  void set_chunk(const RegionLiteral &lit, const Operand &src);   // through the transform
+  void set_chunk(const RegionLiteral &lit, const Scalar &value);
+  void unsafe_set_chunk(const IndexRegion &region, const Operand &src);
+  void set_element(const Coord &coord, const Scalar &v);
+};
+
+// ProcView (src/proc_view.cr): a view plus one element block.
+class ProcView : public View {
+ public:
+  Expr body;  // over input 0 = the view's element
+  Operand operand() const;
+  NArray to_narr() const;
+  ProcView process(const Expr &next) const;  // composition new(proc(x))
+  Scalar get(const Coord &coord) const;
+};
+
+// --- elementwise (operator macros, src/multi_indexable.cr:931-985) ---------
+// Any MultiIndexable operand: NArray, View, ProcView (the block is fused into
+// the kernel) or a scalar.
+NArray elementwise(const std::string &op, const Operand &a, const Operand &b);
+NArray elementwise(const std::string &op, const Operand &a, const Scalar &b);
+NArray elementwise(const std::string &op, const Scalar &a, const Operand &b);  // patches/number.cr
+NArray elementwise_unary(const std::string &op, const Operand &a);
+NArray map(const Operand &a, const std::function<Expr(const Expr &)> &fn);
+// map_with over several operands of one shape (src/multi_indexable.cr:1026-1078)
+NArray map_with(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn);
+// eq (:899-913): elementwise equality, DimensionError on shape mismatch
+NArray eq(const Operand &a, const Operand &b);
+// equals? with == (:871-879)
+bool equals(const Operand &a, const Operand &b);
+
+// --- reductions -------------------------------------------------------------
+enum class Acc { F32C = PH_ACC_F32C, F64 = PH_ACC_F64 };
+// Enumerable#sum / #max over `each` (lex order); Int32 sums raise OverflowError
+Scalar sum(const Operand &a, Acc acc = Acc::F32C);
+Scalar max(const Operand &a);
+// slices(axis).map(&.sum) / (&.max)
+std::vector<double> slices_sum(const Operand &a, int axis, Acc acc = Acc::F32C);
+std::vector<double> slices_max(const Operand &a, int axis);
+struct Reduce2D {
+  double full_sum, full_max;
+  std::vector<double> row_sum, row_max, col_sum, col_max;
+};
+// One pass over a 2-D operand: sum and max in full, along slices(0) and slices(1).
+Reduce2D reduce2d_all(const Operand &a, Acc acc = Acc::F32C);
+
+// Device scratch owned by the caller (e.g. results of reductions).
+struct DeviceDoubles {
+  double *ptr = nullptr;
+  int64_t n = 0;
+  explicit DeviceDoubles(int64_t n);
+  ~DeviceDoubles();
+  std::vector<double> to_host() const;
+};
+
+// Explicit 2-D heat equation on the device (examples/heat_equation.cr
+// generalised to 2-D, Dirichlet zero): steps in place on `u`; returns the FP64
+// energies at 0, every, 2*every, ... (empty if every == 0).
+std::vector<double> heat2d(NArray &u, double r, int64_t steps, int64_t every);
+
+// Initialises the device (idempotent).
+void init(int device = 0);
+bool device_available();
+
+}  // namespace phase
