@@ -1,0 +1,290 @@
+"""Benchmark of the NArray data-parallel path on one MI355X.
+
+Headline workload (the metric's configuration): strided view gather and
+scatter, Int32 8192x8192, values U[0, 2^31) from seed 1.  One step is
+    g = x.view(1..3..-1, ..-2..).to_narr()          # strided gather
+    z = NArray.fill([8192, 8192], 0)                 # zeroed destination
+    z.mutable_view(1..3..-1, ..-2..)[..] = g         # scatter back
+(the fill and the scatter run as one fused kernel).  `value` is algorithmic
+HBM GB/s of the whole step with inputs resident in HBM:
+    gather   8 B/element x 11,186,176 elements            =  89.49 MB
+    fill+scatter: 4 B x 67,108,864 written + 4 B x 11,186,176 read = 313.18 MB
+The other workloads (broadcast, fused ProcView reductions, heat stencil) are
+reported under "extra".
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ph-core_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--heat-steps", type=int, default=2000)
+    return ap.parse_args()
+
+
+class Events:
+    """HIP events on the library stream (where every kernel is launched)."""
+
+    def __init__(self, phase):
+        self.lib = ctypes.CDLL(os.path.join(phase.LIB_DIR, "libphhip.so"))
+        self.lib.ph_event_elapsed.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.ph_event_record.argtypes = [ctypes.c_void_p]
+
+    def new(self):
+        e = ctypes.c_void_p()
+        assert self.lib.ph_event_create(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.lib.ph_event_record(e) == 0
+
+    def ms(self, a, b):
+        v = ctypes.c_float()
+        assert self.lib.ph_event_elapsed(ctypes.byref(v), a, b) == 0
+        return v.value
+
+    def sync(self):
+        assert self.lib.ph_sync() == 0
+
+
+def time_loop(ev, fn, reps):
+    a, b = ev.new(), ev.new()
+    ev.sync()
+    ev.record(a)
+    for _ in range(reps):
+        fn()
+    ev.record(b)
+    return ev.ms(a, b) / reps
+
+
+def load_traffic():
+    """Measured HBM bytes per launch of the dominant kernel (rocprofv3 PMC),
+    if a profile summary with them has been committed."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p))
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(xn, lits_parsed, n, alg_bytes):
+    """ph-core's own CPU path, restated: NArray#unsafe_fetch_chunk (buffered
+    lex odometer), NArray.fill, NArray#unsafe_set_chunk — single thread, on a
+    bounded sample (the full step repeated for about 10 s)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_api as O
+    r = O.region(lits_parsed, [n, n])
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        g = O.gather(xn, r)
+        z = np.zeros((n, n), dtype=np.int32)  # NArray.fill(shape, 0)
+        O.scatter(z, r, g)
+        reps += 1
+        if time.perf_counter() - t0 > 10.0 or reps >= 20:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": round(alg_bytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} full steps (gather + fill + scatter of the 8192x8192 Int32 workload) "
+                      f"with the oracle's restatement of ph-core's buffered odometer path; "
+                      f"{dt * 1e3:.1f} ms/step"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import phase
+    phase.init(0)
+    ev = Events(phase)
+    D = phase.DType
+
+    # ------------------------------------------------------------ headline C2
+    n = 8192
+    lits = ["1..3..-1", "..-2.."]
+    x = phase.NArray.random([n, n], D.I32, seed=1, dist="int31")
+    view = x.view(*lits)
+    ng = view.size  # 2731 * 4096
+    gather_bytes = 8 * ng
+    scatter_bytes = 4 * n * n + 4 * ng
+    step_bytes = gather_bytes + scatter_bytes
+
+    def step():
+        g = view.to_narr()
+        z = phase.NArray.fill([n, n], 0)
+        z.mutable_view(*lits)[[]] = g
+        return z
+
+    for _ in range(args.warmup):
+        step()
+    ev.sync()
+    t0 = time.perf_counter()
+    a, b = ev.new(), ev.new()
+    ev.record(a)
+    for _ in range(args.steps):
+        step()
+    ev.record(b)
+    ev.sync()
+    wall = time.perf_counter() - t0
+    ms_step = ev.ms(a, b) / args.steps
+    value = step_bytes / (ms_step * 1e-3) / 1e9
+
+    # per-kernel durations (HIP events around back-to-back launches)
+    g = view.to_narr()
+    ms_gather = time_loop(ev, lambda: view.to_narr(), 20)
+
+    zs = [phase.NArray.fill([n, n], 0) for _ in range(4)]
+    ms_fs = 0.0
+    for z in zs:  # each fill+scatter needs a fresh pending fill
+        aa, bb = ev.new(), ev.new()
+        ev.sync()
+        ev.record(aa)
+        z.mutable_view(*lits)[[]] = g
+        ev.record(bb)
+        ms_fs += ev.ms(aa, bb)
+    ms_fs /= len(zs)
+    achieved_fs = scatter_bytes / (ms_fs * 1e-3) / 1e9
+    achieved_g = gather_bytes / (ms_gather * 1e-3) / 1e9
+
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "kernel": "k_fill_scatter<uint32_t,8> (fused NArray.fill + MutableView scatter)",
+                "achieved": round(achieved_fs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved_fs / PEAK_HBM_GBS, 4),
+                "traffic": traffic.get("k_fill_scatter") if traffic else None,
+                "ms_per_launch": round(ms_fs, 4), "algorithmic_bytes": scatter_bytes}
+    extra = {"gather": {"kernel": "k_copy_rows<uint32_t,8>", "achieved_gbs": round(achieved_g, 1),
+                        "frac": round(achieved_g / PEAK_HBM_GBS, 4), "ms_per_launch": round(ms_gather, 4),
+                        "algorithmic_bytes": gather_bytes,
+                        "traffic": traffic.get("k_copy_rows") if traffic else None}}
+    del zs, g
+
+    if not args.quick:
+        extra.update(other_workloads(phase, ev, args))
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from crystal import parse_literal
+        cpu = cpu_baseline(x.to_numpy(), [parse_literal(s) for s in lits], n, step_bytes)
+
+    line = {
+        "metric": "HBM GB/s & % of peak; heat-stencil Gcell/s FP32 vs FP64 at 1e-5 rel err",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32",
+        "data": "synthetic (seeded splitmix64 on the device)",
+        "config": {"workload": "strided view gather + scatter, Int32 8192x8192, view(1..3..-1, ..-2..) "
+                               "-> [2731, 4096], written back into a zeroed array through a MutableView",
+                   "step_bytes": step_bytes, "frac_of_peak": round(value / PEAK_HBM_GBS, 4),
+                   "wall_s_timed": round(wall, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+def other_workloads(phase, ev, args):
+    import numpy as np
+    D = phase.DType
+    out = {}
+    # C1 broadcast at the roofline shape (16384^2 FP64; 1024^2 is the parity case)
+    n = 16384
+    a = phase.NArray.random([n, n], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
+    b = phase.NArray.random([1, n], D.F64, seed=0, offset=n * n, a=-1.0, b=1.0)
+    c = phase.NArray.random([n, n], D.F64, seed=0, offset=n * n + n, a=-1.0, b=1.0)
+    bt = b.tile(n, 1)
+    fn = lambda: (a + bt) * c  # noqa: E731
+    fn()
+    ms = time_loop(ev, fn, 10)
+    by = 24 * n * n + 8 * n
+    out["c1_broadcast_f64_16384"] = {"ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
+                                     "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by}
+    del a, b, c, bt
+
+    # C3 fused ProcView + six reductions (F32 and BF16 storage), one pass
+    n = 16384
+    for st, esz in [("F32", 4), ("BF16", 2)]:
+        x = phase.NArray.random([n, n], getattr(D, st), seed=2, dist="normal", a=0.0, b=1.0)
+        pv = x.process(lambda v: (v + 4) ** 2 * 0.5 - v)
+        pv.reduce2d_all()
+        ms = time_loop(ev, lambda: pv.reduce2d_all(), 5)
+        by = esz * n * n
+        out[f"c3_procview_reduce_{st.lower()}_16384"] = {
+            "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
+            "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
+            "note": "one pass: sum and max in full, along slices(0) and slices(1), FP32 compensated"}
+        del x, pv
+
+    # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64
+    n = 8192
+    steps = args.heat_steps
+    rng = np.random.default_rng(3)
+    cy, cx = rng.uniform(0.1, 0.9, 16) * n, rng.uniform(0.1, 0.9, 16) * n
+    sig = rng.uniform(0.01, 0.05, 16) * n
+    amp = rng.uniform(0.5, 2.0, 16)
+    yy = np.arange(n, dtype=np.float64)[:, None]
+    xx = np.arange(n, dtype=np.float64)[None, :]
+    u0 = np.zeros((n, n))
+    for k in range(16):
+        u0 += amp[k] * np.exp(-(((yy - cy[k]) ** 2) + ((xx - cx[k]) ** 2)) / (2 * sig[k] ** 2))
+    res = {}
+    fields = {}
+    for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
+        u = phase.NArray.from_numpy(u0.astype(npdt))
+        ev.sync()
+        aa, bb = ev.new(), ev.new()
+        ev.record(aa)
+        e = phase.heat2d(u, 0.2, steps, every=100)
+        ev.record(bb)
+        ms = ev.ms(aa, bb)
+        res[dt] = {"ms_total": round(ms, 2), "gcells": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
+                   "energy_first_last": [e[0], e[-1]]}
+        fields[dt] = u
+    lib = ev.lib
+    buf = phase.NArray.empty([2], D.F64)  # device scratch for the two sums
+    lib.ph_l2_diff.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                               ctypes.c_void_p]
+    # relative L2 of the FP32 run against the FP64 run (FP64 sums on the device)
+    f32 = fields["F32"].to_numpy().astype(np.float64)
+    f64 = fields["F64"].to_numpy()
+    rel = float(np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)))
+    del buf
+    out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
+                           "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
+                           "rel_l2_f32_vs_f64": rel,
+                           "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0}
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -989,8 +989,59 @@ ph_prog Expr::compile(int n_in) const {
 // ===========================================================================
 // Storage / descriptors
 // ===========================================================================
+// Device buffers are recycled through a size-keyed pool: NArrays created and
+// dropped in a loop (gather outputs, fresh fills) reuse memory instead of
+// paying hipMalloc/hipFree (which synchronises) every time.  All device work
+// runs on the library's single stream, so a recycled buffer is only touched
+// by work ordered after everything that used it before.
+namespace {
+struct Pool {
+  std::multimap<size_t, void *> free_list;
+  size_t cached = 0;
+  static constexpr size_t kCap = size_t(16) << 30;
+};
+Pool &pool() {
+  static Pool *p = new Pool();  // never destroyed: buffers die with the process
+  return *p;
+}
+size_t round_bytes(size_t b) {
+  if (b < 512) return 512;
+  size_t unit = b < (size_t(1) << 20) ? 512 : (size_t(1) << 20);
+  return (b + unit - 1) / unit * unit;
+}
+void *pool_alloc(size_t bytes) {
+  Pool &p = pool();
+  auto it = p.free_list.find(bytes);
+  if (it != p.free_list.end()) {
+    void *ptr = it->second;
+    p.free_list.erase(it);
+    p.cached -= bytes;
+    return ptr;
+  }
+  void *ptr = nullptr;
+  int rc = ph_alloc(&ptr, bytes);
+  if (rc == PH_E_OOM && !p.free_list.empty()) {  // give cached memory back and retry
+    for (auto &kv : p.free_list) ph_free(kv.second);
+    p.free_list.clear();
+    p.cached = 0;
+    rc = ph_alloc(&ptr, bytes);
+  }
+  check(rc, "NArray allocation");
+  return ptr;
+}
+void pool_release(void *ptr, size_t bytes) {
+  Pool &p = pool();
+  if (p.cached + bytes > Pool::kCap) {
+    ph_free(ptr);
+    return;
+  }
+  p.free_list.emplace(bytes, ptr);
+  p.cached += bytes;
+}
+}  // namespace
+
 Storage::~Storage() {
-  if (ptr) ph_free(ptr);
+  if (ptr) pool_release(ptr, round_bytes(bytes));
 }
 
 static ph_view_desc contiguous_desc(DType dt, const Shape &shape, int64_t extent) {
@@ -1025,7 +1076,7 @@ static std::shared_ptr<Storage> new_storage(DType dt, int64_t elements) {
   s->dtype = dt;
   s->elements = elements;
   s->bytes = (size_t)elements * dtype_size(dt);
-  check(ph_alloc(&s->ptr, s->bytes), "NArray allocation");
+  s->ptr = pool_alloc(round_bytes(s->bytes));
   return s;
 }
 
@@ -1938,12 +1989,10 @@ NArray eq(const Operand &a, const Operand &b) {
 // ===========================================================================
 DeviceDoubles::DeviceDoubles(int64_t n_) : n(n_) {
   init();
-  void *p = nullptr;
-  check(ph_alloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(double)), "result buffer");
-  ptr = (double *)p;
+  ptr = (double *)pool_alloc(round_bytes((size_t)std::max<int64_t>(n, 1) * sizeof(double)));
 }
 DeviceDoubles::~DeviceDoubles() {
-  if (ptr) ph_free(ptr);
+  if (ptr) pool_release(ptr, round_bytes((size_t)std::max<int64_t>(n, 1) * sizeof(double)));
 }
 std::vector<double> DeviceDoubles::to_host() const {
   std::vector<double> h((size_t)n);
