@@ -46,45 +46,61 @@ __device__ __forceinline__ void decode_row(const RowArgs &a, int64_t row, int64_
   *s_base = s;
 }
 
-template <typename E, int EPT>
+// Index arithmetic runs in I (int32_t when every address of both operands fits
+// in 31 bits, the common case: ph-core caps element counts at Int32::MAX), and
+// a unit innermost stride is a compile-time constant (DCONT / SCONT), so the
+// per-element work is one add and one address computation.
+template <typename E, int EPT, typename I, bool DCONT, bool SCONT>
 __global__ __launch_bounds__(kBlock) void k_copy_rows(E *__restrict__ dst,
                                                       const E *__restrict__ src, RowArgs a) {
   constexpr int64_t kTile = (int64_t)kBlock * EPT;
+  const I cols = (I)a.cols;
+  const I di = (I)a.d_inner, si = (I)a.s_inner;
   for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
     const int64_t row = b / a.ntiles;
     const int64_t tile = b - row * a.ntiles;
-    int64_t db, sb;
-    decode_row(a, row, &db, &sb);
-    const int64_t c0 = tile * kTile + threadIdx.x;
+    int64_t db64, sb64;
+    decode_row(a, row, &db64, &sb64);
+    const I db = (I)db64, sb = (I)sb64;
+    const I c0 = (I)(tile * kTile) + (I)threadIdx.x;
     E v[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int64_t c = c0 + (int64_t)e * kBlock;
-      if (c < a.cols) v[e] = src[sb + c * a.s_inner];
+      const I c = c0 + (I)(e * kBlock);
+      if (c < cols) v[e] = src[sb + (SCONT ? c : c * si)];
     }
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int64_t c = c0 + (int64_t)e * kBlock;
-      if (c < a.cols) dst[db + c * a.d_inner] = v[e];
+      const I c = c0 + (I)(e * kBlock);
+      if (c < cols) dst[db + (DCONT ? c : c * di)] = v[e];
     }
   }
 }
 
-template <typename E, int EPT>
+template <typename E, int EPT, typename I, bool DCONT>
 __global__ __launch_bounds__(kBlock) void k_fill_rows(E *__restrict__ dst, E value, RowArgs a) {
   constexpr int64_t kTile = (int64_t)kBlock * EPT;
+  const I cols = (I)a.cols;
+  const I di = (I)a.d_inner;
   for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
     const int64_t row = b / a.ntiles;
     const int64_t tile = b - row * a.ntiles;
-    int64_t db, sb;
-    decode_row(a, row, &db, &sb);
-    const int64_t c0 = tile * kTile + threadIdx.x;
+    int64_t db64, sb64;
+    decode_row(a, row, &db64, &sb64);
+    const I db = (I)db64;
+    const I c0 = (I)(tile * kTile) + (I)threadIdx.x;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int64_t c = c0 + (int64_t)e * kBlock;
-      if (c < a.cols) dst[db + c * a.d_inner] = value;
+      const I c = c0 + (I)(e * kBlock);
+      if (c < cols) dst[db + (DCONT ? c : c * di)] = value;
     }
   }
+}
+
+// Both operands' addresses fit in 31 bits: [min, max] of every reachable
+// index (the descriptors were validated against [0, extent)).
+bool fits_i32(const ph_view_desc *a, const ph_view_desc *b) {
+  return a->extent < (int64_t(1) << 31) && (!b || b->extent < (int64_t(1) << 31));
 }
 
 // Narrow rows (cols small): one element per thread, grid-stride, full decode.
@@ -231,13 +247,29 @@ int launch_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_vie
                        value);
     return ph::check_launch("k_copy_flat");
   }
+  const dim3 grid((unsigned)grid_for(a.nblocks)), block(kBlock);
+  const bool small = fits_i32(dd, src ? sd : nullptr);
+  const bool dc = a.d_inner == 1, sc = a.s_inner == 1;
   if (src) {
-    hipLaunchKernelGGL((k_copy_rows<E, EPT>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock), 0,
-                       st, (E *)dst, (const E *)src, a);
+#define PH_COPY_LAUNCH(I, D, S) \
+  hipLaunchKernelGGL((k_copy_rows<E, EPT, I, D, S>), grid, block, 0, st, (E *)dst, (const E *)src, a)
+    if (small) {
+      if (dc && sc) PH_COPY_LAUNCH(int32_t, true, true);
+      else if (dc) PH_COPY_LAUNCH(int32_t, true, false);
+      else if (sc) PH_COPY_LAUNCH(int32_t, false, true);
+      else PH_COPY_LAUNCH(int32_t, false, false);
+    } else {
+      PH_COPY_LAUNCH(int64_t, false, false);
+    }
+#undef PH_COPY_LAUNCH
     return ph::check_launch("k_copy_rows");
   }
-  hipLaunchKernelGGL((k_fill_rows<E, EPT>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock), 0,
-                     st, (E *)dst, value, a);
+  if (small && dc)
+    hipLaunchKernelGGL((k_fill_rows<E, EPT, int32_t, true>), grid, block, 0, st, (E *)dst, value, a);
+  else if (small)
+    hipLaunchKernelGGL((k_fill_rows<E, EPT, int32_t, false>), grid, block, 0, st, (E *)dst, value, a);
+  else
+    hipLaunchKernelGGL((k_fill_rows<E, EPT, int64_t, false>), grid, block, 0, st, (E *)dst, value, a);
   return ph::check_launch("k_fill_rows");
 }
 
@@ -270,7 +302,108 @@ struct FillScatterArgs {
   int64_t src_off;
   // innermost axis: member iff (c - first) = q*step, 0 <= q < count
   int64_t in_first, in_step, in_count, in_src_stride;
+  // |in_step| as a 32-bit magic divisor: q = (umulhi(u, m) + u) >> l
+  uint32_t in_d, in_m, in_l, pad2_;
 };
+
+// Granlund-Montgomery round-up divisor for 32-bit unsigned n < 2^32:
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1.
+void magic_u32(uint32_t d, uint32_t *m, uint32_t *l) {
+  uint32_t s = 0;
+  while ((uint64_t(1) << s) < d) ++s;
+  *l = s;
+  *m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t l) {
+  return (uint32_t)(((uint64_t)__umulhi(n, m) + n) >> l);
+}
+
+// Row membership (uniform per workgroup): decode the outer coordinate of
+// destination row `row`; member rows also give the source offset of the row.
+__device__ __forceinline__ bool fs_row(const FillScatterArgs &a, int64_t row, int64_t *soff_out) {
+  bool member = true;
+  int64_t soff = a.src_off;
+  int64_t r = row;
+  for (int k = a.rank - 2; k >= 0; --k) {
+    const int64_t n = a.shape[k];
+    const int64_t q = r / n;
+    const int64_t x = r - q * n;
+    r = q;
+    const int64_t rel = x - a.first[k];
+    const int64_t st = a.step[k];
+    if (a.axis[k] < 0) {
+      member = member && (rel == 0);
+    } else {
+      const int64_t j = rel / st;
+      member = member && (j * st == rel) && j >= 0 && j < a.count[k];
+      soff += j * a.src_stride[a.axis[k]];
+    }
+  }
+  *soff_out = soff;
+  return member;
+}
+
+// Vector path: a lane owns 16 consecutive bytes of a destination row (VEC
+// elements) per group, G groups per thread; every store is a 16-byte
+// non-temporal store (the destination is written once and not re-read here).
+// Innermost-axis membership uses 32-bit arithmetic with the magic divisor.
+template <typename E, int G>
+__global__ __launch_bounds__(kBlock) void k_fill_scatter_vec(E *__restrict__ dst,
+                                                             const E *__restrict__ src,
+                                                             FillScatterArgs a, E fill) {
+  constexpr int VEC = 16 / (int)sizeof(E);
+  typedef E V __attribute__((ext_vector_type(VEC)));
+  constexpr int64_t kTile = (int64_t)kBlock * VEC * G;
+  const int32_t cols = (int32_t)a.cols;
+  const int32_t in_first = (int32_t)a.in_first;
+  const int32_t in_step = (int32_t)a.in_step;
+  const uint32_t in_count = (uint32_t)a.in_count;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    const int64_t row = b / a.ntiles;
+    const int64_t tile = b - row * a.ntiles;
+    int64_t soff;
+    const bool member = fs_row(a, row, &soff);
+    E *drow = dst + row * a.cols;
+    if (!member) {
+      V f;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) f[v] = fill;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int32_t c = (int32_t)(tile * kTile) + (g * kBlock + (int32_t)threadIdx.x) * VEC;
+        if (c < cols) __builtin_nontemporal_store(f, (V *)(drow + c));
+      }
+      continue;
+    }
+    V out[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int32_t c0 = (int32_t)(tile * kTile) + (g * kBlock + (int32_t)threadIdx.x) * VEC;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int32_t rel = c0 + v - in_first;
+        bool ok;
+        uint32_t q;
+        if (a.last_axis < 0) {
+          ok = rel == 0;
+          q = 0;
+        } else {
+          const uint32_t u = (uint32_t)(in_step > 0 ? rel : -rel);
+          ok = in_step > 0 ? rel >= 0 : rel <= 0;
+          q = fast_div(u, a.in_m, a.in_l);
+          ok = ok && q * a.in_d == u && q < in_count && c0 + v < cols;
+        }
+        out[g][v] = ok ? src[soff + (int64_t)q * a.in_src_stride] : fill;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int32_t c = (int32_t)(tile * kTile) + (g * kBlock + (int32_t)threadIdx.x) * VEC;
+      if (c < cols) __builtin_nontemporal_store(out[g], (V *)(drow + c));
+    }
+  }
+}
 
 template <typename E, int EPT>
 __global__ __launch_bounds__(kBlock) void k_fill_scatter(E *__restrict__ dst,
@@ -336,6 +469,24 @@ template <typename E>
 int launch_fill_scatter(void *dst, FillScatterArgs a, const void *src, const void *fill_bits) {
   E fill{};
   std::memcpy(&fill, fill_bits, sizeof(E));
+  constexpr int VEC = 16 / (int)sizeof(E);
+  constexpr int G = 4;
+  magic_u32((uint32_t)(a.in_step < 0 ? -a.in_step : a.in_step), &a.in_m, &a.in_l);
+  a.in_d = (uint32_t)(a.in_step < 0 ? -a.in_step : a.in_step);
+  const bool vec = a.cols % VEC == 0 && ((uintptr_t)dst % 16) == 0 && a.cols < (int64_t(1) << 30) &&
+                   a.in_first < (int64_t(1) << 30) && (a.in_step < (int64_t(1) << 30)) &&
+                   (a.in_step > -(int64_t(1) << 30));
+  if (vec) {
+    const int64_t tile = (int64_t)kBlock * VEC * G;
+    a.ntiles = (a.cols + tile - 1) / tile;
+    a.nblocks = a.rows * a.ntiles;
+    if (a.nblocks == 0) return PH_OK;
+    int rc = ph::launch_gate("fill scatter");
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_fill_scatter_vec<E, G>), dim3((unsigned)grid_for(a.nblocks)), dim3(kBlock),
+                       0, ph::stream(), (E *)dst, (const E *)src, a, fill);
+    return ph::check_launch("k_fill_scatter_vec");
+  }
   constexpr int EPT = sizeof(E) >= 8 ? 4 : 8;
   const int64_t tile = (int64_t)kBlock * EPT;
   a.ntiles = (a.cols + tile - 1) / tile;

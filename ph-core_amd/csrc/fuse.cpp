@@ -454,8 +454,33 @@ __device__ __forceinline__ void st_init(SD &a) { a.s = 0.0; a.c = 0.0; a.m = -__
 __device__ __forceinline__ void st_init(SI &a) { a.s = 0; a.hi = -(1LL << 62); a.lo = (1LL << 62); a.m = 0; a.n = 0; }
 template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, F v) {
   const F t = a.s + v;
-  const F d = (__builtin_fabs((double)a.s) >= __builtin_fabs((double)v)) ? ((a.s - t) + v) : ((v - t) + a.s);
+  const F d = (ph_fabs(a.s) >= ph_fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
   a.c += d; a.s = t; a.m = (v > a.m) ? v : a.m;
+}
+// Transposing butterfly over a wavefront: each lane holds 16 partial values
+// (one per row of a 16-row group); after the fold, lane l holds the total over
+// all 64 lanes for row (l >> 2) & 15.  17 cross-lane exchanges for 16 rows.
+template <typename F, bool MAX> __device__ __forceinline__ F ph_comb2(F a, F b) {
+  if (MAX) return a > b ? a : b;
+  return a + b;
+}
+template <typename F, bool MAX> __device__ __forceinline__ F tfold16(F (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int half = 8, m = 32; half >= 1; half >>= 1, m >>= 1) {
+    const bool b = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const F keep = b ? v[i + half] : v[i];
+      const F send = b ? v[i] : v[i + half];
+      const F recv = __shfl_xor(send, m, 64);
+      v[i] = ph_comb2<F, MAX>(keep, recv);
+    }
+  }
+  F x = v[0];
+  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
+  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
+  return x;
 }
 __device__ __forceinline__ void st_add(SF &a, float v) { st_addf<SF, float>(a, v); }
 __device__ __forceinline__ void st_add(SD &a, double v) { st_addf<SD, double>(a, v); }
@@ -524,12 +549,105 @@ struct RedPlan {
   bool vec_ok[PH_MAX_INPUTS];  // whole-vector loads of the lane's VEC columns
 };
 
+// Tile kernel for floating-point accumulation (states SF / SD).  Column
+// states: Neumaier-compensated, one per thread column, accumulated row after
+// row.  Row partials: the lane's VEC values summed, then 16 rows at a time
+// folded across the wavefront by a transposing butterfly (pairwise summation
+// within a 256-column chunk), then the four wavefronts of the workgroup and
+// the column tiles are combined with error-free TwoSum.  Loads for four rows
+// are issued before any of them is used.
+std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
+                           const int64_t *cs, const RedPlan &P) {
+  const int V = P.vec;
+  const bool sd = std::string(P.state) == "SD";
+  const char *F = sd ? "double" : "float";
+  std::string s;
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
+  s += fmt("  typedef %s S; typedef %s F;\n", P.state, F);
+  s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
+           lit_i64(P.ntc).c_str());
+  s += "  const int lane = threadIdx.x & 63; const int wave = threadIdx.x >> 6;\n";
+  s += fmt("  const i64 col0 = ct * %d + wave * %d + lane * %d;\n", 256 * V, 64 * V, V);
+  s += fmt("  const i64 r0 = rt * %d;\n", P.tr);
+  s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
+           lit_i64(P.rows).c_str());
+  s += "  int fl = 0;\n";
+  s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
+  if (P.do_rows) s += fmt("  __shared__ F rsum_sh[%d][4]; __shared__ F rmax_sh[%d][4];\n", P.tr, P.tr);
+  s += "  for (i64 g0 = r0; g0 < r1; g0 += 16) {\n";
+  if (P.do_rows) s += "    F rsv[16], rmv[16];\n";
+  s += "#pragma unroll\n    for (int q = 0; q < 16; q += 4) {\n";
+  for (int j = 0; j < n_in; ++j) s += fmt("      %s x%d[4][%d];\n", ctype(compute_dt(in_dt[j])), j, V);
+  // loads of 4 rows
+  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n        if (r < r1) {\n";
+  for (int j = 0; j < n_in; ++j) {
+    s += fmt("          const i64 rb%d = A.in_off[%d] + r * %s;\n", j, j, lit_i64(rs[j]).c_str());
+    if (P.vec_ok[j]) {
+      const char *vt = in_dt[j] == PH_F64 ? "double4" : in_dt[j] == PH_BF16 ? "ushort4"
+                       : in_dt[j] == PH_I32 ? "int4" : in_dt[j] == PH_BOOL ? "uchar4" : "float4";
+      s += fmt("          if (col0 + %d <= %s) {\n", V, lit_i64(P.cols).c_str());
+      s += fmt("            const %s t = *(const %s *)((const %s *)A.in[%d] + rb%d + col0);\n", vt, vt,
+               stype(in_dt[j]), j, j);
+      s += fmt("            const %s tt[4] = {t.x, t.y, t.z, t.w};\n", stype(in_dt[j]));
+      s += fmt("#pragma unroll\n            for (int v = 0; v < 4; ++v) x%d[k][v] = ph_ld(tt, v);\n", j);
+      s += "          } else {\n";
+    } else {
+      s += "          {\n";
+    }
+    s += fmt("#pragma unroll\n            for (int v = 0; v < %d; ++v) {\n              const i64 c = col0 + v;\n", V);
+    s += fmt("              if (c < %s) x%d[k][v] = ph_ld((const %s *)A.in[%d], rb%d + c * %s);\n",
+             lit_i64(P.cols).c_str(), j, stype(in_dt[j]), j, j, lit_i64(cs[j]).c_str());
+    s += "            }\n          }\n";
+  }
+  s += "        }\n      }\n";
+  // compute 4 rows
+  std::vector<std::string> inexpr;
+  for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[k][v]", j));
+  std::string body;
+  std::string res = emit_prog(p, inexpr, &body, "");
+  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
+  s += "        F ls = (F)0; F lm = -__builtin_inf();\n";
+  s += "        if (r < r1) {\n";
+  s += fmt("#pragma unroll\n          for (int v = 0; v < %d; ++v) {\n            const i64 c = col0 + v;\n", V);
+  s += fmt("            if (c < %s) {\n", lit_i64(P.cols).c_str());
+  s += body;
+  s += fmt("              const F val = (F)(%s);\n", res.c_str());
+  if (P.do_cols) s += "              st_add(cst[v], val);\n";
+  if (P.do_rows) s += "              ls += val; lm = (val > lm) ? val : lm;\n";
+  s += "            }\n          }\n        }\n";
+  if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
+  s += "      }\n    }\n";
+  if (P.do_rows) {
+    s += "    const F fs = tfold16<F, false>(rsv);\n    const F fm = tfold16<F, true>(rmv);\n";
+    s += "    const int rr = (lane >> 2) & 15;\n";
+    s += "    if ((lane & 3) == 0 && g0 + rr < r1) { rsum_sh[g0 - r0 + rr][wave] = fs; rmax_sh[g0 - r0 + rr][wave] = fm; }\n";
+  }
+  s += "  }\n";
+  if (P.do_cols) {
+    s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = col0 + v;\n", V);
+    s += fmt("    if (c < %s) ((S *)A.colpart)[rt * %s + c] = cst[v];\n  }\n", lit_i64(P.cols).c_str(),
+             lit_i64(P.cols).c_str());
+  }
+  if (P.do_rows) {
+    s += "  __syncthreads();\n";
+    s += "  for (i64 i = threadIdx.x; i < r1 - r0; i += 256) {\n";
+    s += "    S a; a.s = rsum_sh[i][0]; a.c = (F)0; a.m = rmax_sh[i][0];\n";
+    s += "    for (int w = 1; w < 4; ++w) { S b; b.s = rsum_sh[i][w]; b.c = (F)0; b.m = rmax_sh[i][w]; a = st_comb(a, b); }\n";
+    s += fmt("    ((S *)A.rowpart)[ct * %s + r0 + i] = a;\n  }\n", lit_i64(P.rows).c_str());
+  }
+  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  return s;
+}
+
 std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
                        const int64_t *cs, const RedPlan &P) {
   std::string s = kPrelude;
   s += kRedPrelude;
   const int V = P.vec;
-  // --- tile kernel
+  if (std::string(P.state) != "SI") {
+    s += red_tile_float(p, n_in, in_dt, rs, cs, P);
+  } else {
+  // --- tile kernel (integer states: every fold in lexicographic order)
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
   s += fmt("  typedef %s S;\n", P.state);
   s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
@@ -594,6 +712,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     s += fmt("    ((S *)A.rowpart)[ct * %s + r0 + i] = a;\n  }\n", lit_i64(P.rows).c_str());
   }
   s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  }
   // --- finish: per-row and per-column results
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_finish(RedArgs A) {\n";
   s += fmt("  typedef %s S;\n", P.state);
@@ -661,10 +780,12 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   }
   P.ntc = (P.cols + 1023) / 1024;
   // rows per tile: aim for >= ~2048 workgroups, between 8 and 256 rows
+  // (a multiple of 16: the float path folds rows 16 at a time)
   int64_t tr = P.rows * P.ntc / 2048;
-  if (tr < 8) tr = 8;
   if (tr > 256) tr = 256;
-  if (tr > P.rows) tr = P.rows;
+  tr = (tr + 15) / 16 * 16;
+  if (tr < 16) tr = 16;
+  if (std::string(P.state) == "SI" && tr > P.rows) tr = P.rows;
   if (tr < 1) tr = 1;
   P.tr = (int)tr;
   P.ntr = (P.rows + P.tr - 1) / P.tr;

@@ -98,43 +98,53 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step(const T *__restrict__
   const int64_t x0 = strip * 64 * V;
   const int64_t y0 = band * kTY;
   const int64_t y1 = (y0 + kTY < a.ny) ? y0 + kTY : a.ny;
-  T n[V], c[V], s[V], cw, ce, dummy_w, dummy_e;
+  T n[V], c[V], cw, ce, dummy_w, dummy_e;
   load_row<T>(u, a, y0 - 1, x0, lane, vec_ok, n, &dummy_w, &dummy_e);
   load_row<T>(u, a, y0, x0, lane, vec_ok, c, &cw, &ce);
   const T four = T(4);
   const int64_t x = x0 + (int64_t)lane * V;
-  for (int64_t y = y0; y < y1; ++y) {
-    T sw, se;
-    load_row<T>(u, a, y + 1, x0, lane, vec_ok, s, &sw, &se);
-    // west neighbour of c[0] is the previous lane's c[V-1]; east of c[V-1] is
-    // the next lane's c[0]
-    T wl = __shfl_up(c[V - 1], 1, 64);
-    T er = __shfl_down(c[0], 1, 64);
-    if (lane == 0) wl = cw;
-    if (lane == 63) er = ce;
-    T o[V];
+  // kAhead south rows are loaded together before any of them is used, so a
+  // wavefront keeps kAhead row loads in flight instead of one.
+  constexpr int kAhead = 4;
+  for (int64_t y = y0; y < y1; y += kAhead) {
+    T s[kAhead][V], sw[kAhead], se[kAhead];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const T w = i == 0 ? wl : c[i - 1];
-      const T e = i == V - 1 ? er : c[i + 1];
-      const T d = (((n[i] - four * c[i]) + s[i]) + w) + e;
-      o[i] = c[i] + d * r;
-    }
-    T *orow = out + y * a.nx;
-    if (vec_ok && x + V <= a.nx) {
-      *(typename Vec<T>::type *)(orow + x) = pack<T>(o);
-    } else {
-#pragma unroll
-      for (int i = 0; i < V; ++i)
-        if (x + i < a.nx) orow[x + i] = o[i];
+    for (int k = 0; k < kAhead; ++k) {
+      if (y + 1 + k <= y1) load_row<T>(u, a, y + 1 + k, x0, lane, vec_ok, s[k], &sw[k], &se[k]);
     }
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      n[i] = c[i];
-      c[i] = s[i];
+    for (int k = 0; k < kAhead; ++k) {
+      if (y + k >= y1) break;
+      // west neighbour of c[0] is the previous lane's c[V-1]; east of c[V-1]
+      // is the next lane's c[0]
+      T wl = __shfl_up(c[V - 1], 1, 64);
+      T er = __shfl_down(c[0], 1, 64);
+      if (lane == 0) wl = cw;
+      if (lane == 63) er = ce;
+      T o[V];
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        const T w = i == 0 ? wl : c[i - 1];
+        const T e = i == V - 1 ? er : c[i + 1];
+        const T d = (((n[i] - four * c[i]) + s[k][i]) + w) + e;
+        o[i] = c[i] + d * r;
+      }
+      T *orow = out + (y + k) * a.nx;
+      if (vec_ok && x + V <= a.nx) {
+        *(typename Vec<T>::type *)(orow + x) = pack<T>(o);
+      } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+          if (x + i < a.nx) orow[x + i] = o[i];
+      }
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        n[i] = c[i];
+        c[i] = s[k][i];
+      }
+      cw = sw[k];
+      ce = se[k];
     }
-    cw = sw;
-    ce = se;
   }
 }
 
