@@ -56,6 +56,10 @@ void collapse(const ph_view_desc *const *descs, int nops, Layout *out);
 int launch_gate(const char *what);
 int check_launch(const char *what);
 
+// Zeroes n int32 words on the library stream with a kernel (a memset node
+// would misbehave inside a replayed HIP graph).
+int clear_i32(int *p, int n);
+
 }  // namespace ph
 
 #define PH_GUARD() std::lock_guard<std::recursive_mutex> ph_guard_(ph::abi_mutex())

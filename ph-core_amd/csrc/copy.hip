@@ -554,7 +554,18 @@ __global__ __launch_bounds__(kBlock) void k_random(void *buf, int dtype, int64_t
   }
 }
 
+__global__ void k_clear_i32(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 }  // namespace
+
+int ph::clear_i32(int *p, int n) {
+  if (n <= 0) return PH_OK;
+  hipLaunchKernelGGL(k_clear_i32, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ph::stream(), p, n);
+  return ph::check_launch("k_clear_i32");
+}
 
 extern "C" {
 

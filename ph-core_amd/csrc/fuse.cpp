@@ -24,6 +24,7 @@
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -402,21 +403,67 @@ int64_t grid_for(int64_t blocks) {
   return blocks < 1 ? 1 : (blocks > cap ? cap : blocks);
 }
 
-int get_flags_buffer(int **dflags) {
+// The flag word is cleared by a one-thread kernel (not a memset node, so the
+// sequence stays valid inside a captured HIP graph) and read back only when
+// the program contains an operation that can raise a flag.
+int get_flags_buffer(int **dflags, bool clear) {
   void *p;
   int rc = ph::scratch(64, &p, 3);
   if (rc) return rc;
   *dflags = (int *)p;
+  if (clear) return ph::clear_i32(*dflags, 1);
   return PH_OK;
 }
 
-int read_flags(int *dflags, int32_t *flags_out) {
+int read_flags(int *dflags, bool may_flag, int32_t *flags_out) {
   if (!flags_out) return PH_OK;
+  if (!may_flag) {
+    *flags_out = 0;
+    return PH_OK;
+  }
   int v = 0;
   PH_HIP(hipMemcpyAsync(&v, dflags, sizeof(int), hipMemcpyDeviceToHost, ph::stream()));
   PH_HIP(hipStreamSynchronize(ph::stream()));
   *flags_out = v;
   return PH_OK;
+}
+
+// Whether any element of `p` can raise a PH_FLAG_* condition.
+bool prog_may_flag(const ph_prog *p) {
+  for (int k = 0; k < p->n_insn; ++k) {
+    const ph_insn &in = p->insn[k];
+    if (in.op == PH_OP_IPOW || in.op == PH_OP_WIPOW) return true;
+    if (in.dt == PH_I32 && (in.op == PH_OP_ADD || in.op == PH_OP_SUB || in.op == PH_OP_MUL ||
+                            in.op == PH_OP_FLOORDIV || in.op == PH_OP_MOD || in.op == PH_OP_NEG ||
+                            in.op == PH_OP_ABS))
+      return true;
+  }
+  return false;
+}
+
+// Compiled fused kernels, keyed by everything their source depends on (the
+// program's instructions, operand dtypes and baked layout), so repeated calls
+// skip source generation.
+struct FusedKernels {
+  hipFunction_t k[3];
+  int64_t nblocks;
+};
+std::map<std::string, FusedKernels> &fused_cache() {
+  static std::map<std::string, FusedKernels> c;
+  return c;
+}
+
+void key_add(std::string *k, const void *p, size_t n) { k->append((const char *)p, n); }
+
+std::string prog_key(const ph_prog *p) {
+  std::string k;
+  key_add(&k, &p->n_insn, sizeof(int32_t) * 4);
+  for (int i = 0; i < p->n_insn; ++i) {
+    const ph_insn &in = p->insn[i];
+    key_add(&k, &in.op, 6);
+    key_add(&k, &in.imm, 8);
+  }
+  return k;
 }
 
 // Materialises staged (reshape / tile) inputs into contiguous scratch so the
@@ -481,6 +528,11 @@ template <typename F, bool MAX> __device__ __forceinline__ F tfold16(F (&v)[16])
   x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
   x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
   return x;
+}
+template <typename S, typename F> __device__ __forceinline__ void st_addsum(S &a, F v) {
+  const F t = a.s + v;
+  const F d = (ph_fabs(a.s) >= ph_fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
+  a.c += d; a.s = t;
 }
 __device__ __forceinline__ void st_add(SF &a, float v) { st_addf<SF, float>(a, v); }
 __device__ __forceinline__ void st_add(SD &a, double v) { st_addf<SD, double>(a, v); }
@@ -605,6 +657,10 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[k][v]", j));
   std::string body;
   std::string res = emit_prog(p, inexpr, &body, "");
+  // column chunk of 4 rows: plain sum (pairwise level) then one compensated
+  // add into the column state; maxima elementwise
+  s += fmt("      F cp[%d], cm[%d];\n#pragma unroll\n      for (int v = 0; v < %d; ++v) { cp[v] = (F)0; cm[v] = -__builtin_inf(); }\n",
+           V, V, V);
   s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
   s += "        F ls = (F)0; F lm = -__builtin_inf();\n";
   s += "        if (r < r1) {\n";
@@ -612,11 +668,14 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += fmt("            if (c < %s) {\n", lit_i64(P.cols).c_str());
   s += body;
   s += fmt("              const F val = (F)(%s);\n", res.c_str());
-  if (P.do_cols) s += "              st_add(cst[v], val);\n";
+  if (P.do_cols) s += "              cp[v] += val; cm[v] = (val > cm[v]) ? val : cm[v];\n";
   if (P.do_rows) s += "              ls += val; lm = (val > lm) ? val : lm;\n";
   s += "            }\n          }\n        }\n";
   if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
-  s += "      }\n    }\n";
+  s += "      }\n";
+  if (P.do_cols)
+    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = (cm[v] > cst[v].m) ? cm[v] : cst[v].m; }\n", V);
+  s += "    }\n";
   if (P.do_rows) {
     s += "    const F fs = tfold16<F, false>(rsv);\n    const F fm = tfold16<F, true>(rmv);\n";
     s += "    const int rr = (lane >> 2) & 15;\n";
@@ -789,14 +848,32 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   if (tr < 1) tr = 1;
   P.tr = (int)tr;
   P.ntr = (P.rows + P.tr - 1) / P.tr;
-  std::string src = red_source(prog, n_in, in_dt, V.rs, V.cs, P);
+  std::string key = "R" + prog_key(prog);
+  key_add(&key, in_dt, sizeof(int) * (size_t)n_in);
+  key_add(&key, V.rs, sizeof(int64_t) * (size_t)n_in);
+  key_add(&key, V.cs, sizeof(int64_t) * (size_t)n_in);
+  key_add(&key, P.vec_ok, sizeof(bool) * (size_t)n_in);
+  const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.do_rows, P.do_cols, P.do_full};
+  key_add(&key, plan, sizeof(plan));
+  key += P.state;
   hipFunction_t ktile, kfin, kfull;
-  int rc = ph::jit_kernel(src, "ph_red_tile", &ktile);
-  if (rc) return rc;
-  rc = ph::jit_kernel(src, "ph_red_finish", &kfin);
-  if (rc) return rc;
-  rc = ph::jit_kernel(src, "ph_red_full", &kfull);
-  if (rc) return rc;
+  int rc;
+  auto hit = fused_cache().find(key);
+  if (hit != fused_cache().end()) {
+    ktile = hit->second.k[0];
+    kfin = hit->second.k[1];
+    kfull = hit->second.k[2];
+  } else {
+    std::string src = red_source(prog, n_in, in_dt, V.rs, V.cs, P);
+    rc = ph::jit_kernel(src, "ph_red_tile", &ktile);
+    if (rc) return rc;
+    rc = ph::jit_kernel(src, "ph_red_finish", &kfin);
+    if (rc) return rc;
+    rc = ph::jit_kernel(src, "ph_red_full", &kfull);
+    if (rc) return rc;
+    fused_cache()[key] = FusedKernels{{ktile, kfin, kfull}, 0};
+  }
+  const bool may_flag = prog_may_flag(prog) || (std::string(P.state) == "SI" && (full_sum || row_sum || col_sum));
   // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [rows]
   const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
   const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
@@ -806,9 +883,8 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + 256, &base, 2);
   if (rc) return rc;
   int *dflags;
-  rc = get_flags_buffer(&dflags);
+  rc = get_flags_buffer(&dflags, may_flag);
   if (rc) return rc;
-  PH_HIP(hipMemsetAsync(dflags, 0, sizeof(int), ph::stream()));
   RedArgs A;
   std::memset(&A, 0, sizeof(A));
   for (int j = 0; j < n_in; ++j) {
@@ -838,7 +914,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
                                nullptr, cfg));
   if (P.do_full)
     PH_HIP(hipModuleLaunchKernel(kfull, 1, 1, 1, 256, 1, 1, 0, ph::stream(), nullptr, cfg));
-  return read_flags(dflags, flags_out);
+  return read_flags(dflags, may_flag, flags_out);
 }
 
 int check_inputs(const ph_prog *prog, int32_t n_in, const void *const *ins,
@@ -966,14 +1042,28 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   const int64_t cols = L.rank > 0 ? L.shape[L.rank - 1] : 1;
   const bool flat = L.rank > 1 && cols < 256;
   int64_t nblocks = 0;
-  std::string src = map_source(prog, n_in, in_dt, L, out_desc->dtype, ept, flat, &nblocks);
+  std::string key = "M" + prog_key(prog);
+  key_add(&key, in_dt, sizeof(int) * (size_t)n_in);
+  key_add(&key, &L.rank, sizeof(L.rank));
+  key_add(&key, L.shape, sizeof(int64_t) * (size_t)L.rank);
+  for (int o = 0; o <= n_in; ++o) key_add(&key, L.stride[o], sizeof(int64_t) * (size_t)L.rank);
+  const int32_t mk[3] = {out_desc->dtype, ept, flat ? 1 : 0};
+  key_add(&key, mk, sizeof(mk));
   hipFunction_t fn;
-  rc = ph::jit_kernel(src, "ph_map_kernel", &fn);
-  if (rc) return rc;
+  auto hit = fused_cache().find(key);
+  if (hit != fused_cache().end()) {
+    fn = hit->second.k[0];
+    nblocks = hit->second.nblocks;
+  } else {
+    std::string src = map_source(prog, n_in, in_dt, L, out_desc->dtype, ept, flat, &nblocks);
+    rc = ph::jit_kernel(src, "ph_map_kernel", &fn);
+    if (rc) return rc;
+    fused_cache()[key] = FusedKernels{{fn, nullptr, nullptr}, nblocks};
+  }
+  const bool may_flag = prog_may_flag(prog);
   int *dflags;
-  rc = get_flags_buffer(&dflags);
+  rc = get_flags_buffer(&dflags, may_flag);
   if (rc) return rc;
-  PH_HIP(hipMemsetAsync(dflags, 0, sizeof(int), ph::stream()));
   MapArgs A;
   std::memset(&A, 0, sizeof(A));
   for (int j = 0; j < n_in; ++j) {
@@ -990,7 +1080,7 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   if (rc) return rc;
   PH_HIP(hipModuleLaunchKernel(fn, (unsigned)grid_for(nblocks), 1, 1, kBlock, 1, 1, 0, ph::stream(),
                                nullptr, cfg));
-  return read_flags(dflags, flags_out);
+  return read_flags(dflags, may_flag, flags_out);
 }
 
 int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,

@@ -25,7 +25,7 @@ constexpr int kWaves = 4;    // wavefronts (strips) per workgroup
 
 template <typename T> struct Vec;
 template <> struct Vec<float> { typedef float4 type; static constexpr int n = 4; };
-template <> struct Vec<double> { typedef double2 type; static constexpr int n = 2; };
+template <> struct Vec<double> { typedef double4 type; static constexpr int n = 4; };
 
 template <typename T>
 __device__ __forceinline__ void unpack(const typename Vec<T>::type &v, T *o) {
@@ -265,7 +265,8 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   a.nblocks = groups_per_band * a.nbands;
   a.per_xcd = (a.nblocks + 7) / 8;
   const int64_t grid = a.per_xcd * 8;
-  const int vec_ok = (nx % V == 0) && ((uintptr_t)u % 16 == 0) && ((uintptr_t)tmp % 16 == 0);
+  constexpr uintptr_t kAlign = sizeof(typename Vec<T>::type);
+  const int vec_ok = (nx % V == 0) && ((uintptr_t)u % kAlign == 0) && ((uintptr_t)tmp % kAlign == 0);
   const T rr = (T)r;
   T *cur = u, *nxt = tmp;
   int64_t ei = 0;
