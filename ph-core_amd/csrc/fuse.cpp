@@ -507,9 +507,31 @@ template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, 
 // Transposing butterfly over a wavefront: each lane holds 16 partial values
 // (one per row of a 16-row group); after the fold, lane l holds the total over
 // all 64 lanes for row (l >> 2) & 15.  17 cross-lane exchanges for 16 rows.
+__device__ __forceinline__ float ph_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double ph_fmax(double a, double b) { return __builtin_fmax(a, b); }
 template <typename F, bool MAX> __device__ __forceinline__ F ph_comb2(F a, F b) {
-  if (MAX) return a > b ? a : b;
+  if (MAX) return ph_fmax(a, b);
   return a + b;
+}
+// As tfold16 for 8 rows: lane l ends with the total for row (l >> 3) & 7.
+template <typename F, bool MAX> __device__ __forceinline__ F tfold8(F (&v)[8]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int half = 4, m = 32; half >= 1; half >>= 1, m >>= 1) {
+    const bool b = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const F keep = b ? v[i + half] : v[i];
+      const F send = b ? v[i] : v[i + half];
+      const F recv = __shfl_xor(send, m, 64);
+      v[i] = ph_comb2<F, MAX>(keep, recv);
+    }
+  }
+  F x = v[0];
+  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 4, 64));
+  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
+  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
+  return x;
 }
 template <typename F, bool MAX> __device__ __forceinline__ F tfold16(F (&v)[16]) {
   const int lane = threadIdx.x & 63;
@@ -602,17 +624,25 @@ struct RedPlan {
 };
 
 // Tile kernel for floating-point accumulation (states SF / SD).  Column
-// states: Neumaier-compensated, one per thread column, accumulated row after
-// row.  Row partials: the lane's VEC values summed, then 16 rows at a time
-// folded across the wavefront by a transposing butterfly (pairwise summation
-// within a 256-column chunk), then the four wavefronts of the workgroup and
-// the column tiles are combined with error-free TwoSum.  Loads for four rows
-// are issued before any of them is used.
+// states: Neumaier-compensated, one per thread column; each chunk of four
+// rows is summed plainly first (a pairwise level) and then added with
+// compensation.  Row partials: the lane's VEC values summed, then 8 rows at a
+// time folded across the wavefront by a transposing butterfly (pairwise
+// summation within a 256-column chunk); the four wavefronts of the workgroup
+// and the column tiles are then combined with error-free TwoSum.  Loads for
+// four rows are issued before any of them is used.  Bounds checks are emitted
+// only when the shape is not a whole number of tiles, so the interior code is
+// free of predication.
 std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
                            const int64_t *cs, const RedPlan &P) {
   const int V = P.vec;
   const bool sd = std::string(P.state) == "SD";
   const char *F = sd ? "double" : "float";
+  const bool full_cols = P.cols % (256 * V) == 0;
+  const bool full_rows = P.rows % P.tr == 0;
+  const std::string C = lit_i64(P.cols);
+  auto ccheck = [&](const std::string &expr) { return full_cols ? std::string("true") : "(" + expr + " < " + C + ")"; };
+  const std::string rcheck = full_rows ? "true" : "(r < r1)";
   std::string s;
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
   s += fmt("  typedef %s S; typedef %s F;\n", P.state, F);
@@ -626,29 +656,33 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += "  int fl = 0;\n";
   s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
   if (P.do_rows) s += fmt("  __shared__ F rsum_sh[%d][4]; __shared__ F rmax_sh[%d][4];\n", P.tr, P.tr);
-  s += "  for (i64 g0 = r0; g0 < r1; g0 += 16) {\n";
-  if (P.do_rows) s += "    F rsv[16], rmv[16];\n";
-  s += "#pragma unroll\n    for (int q = 0; q < 16; q += 4) {\n";
+  s += "  for (i64 g0 = r0; g0 < r1; g0 += 8) {\n";
+  if (P.do_rows) s += "    F rsv[8], rmv[8];\n";
+  s += "#pragma unroll\n    for (int q = 0; q < 8; q += 4) {\n";
   for (int j = 0; j < n_in; ++j) s += fmt("      %s x%d[4][%d];\n", ctype(compute_dt(in_dt[j])), j, V);
   // loads of 4 rows
-  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n        if (r < r1) {\n";
+  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
+  s += "        if (" + rcheck + ") {\n";
   for (int j = 0; j < n_in; ++j) {
-    s += fmt("          const i64 rb%d = A.in_off[%d] + r * %s;\n", j, j, lit_i64(rs[j]).c_str());
+    s += fmt("          const %s *rp%d = (const %s *)A.in[%d] + (A.in_off[%d] + r * %s);\n", stype(in_dt[j]), j,
+             stype(in_dt[j]), j, j, lit_i64(rs[j]).c_str());
     if (P.vec_ok[j]) {
       const char *vt = in_dt[j] == PH_F64 ? "double4" : in_dt[j] == PH_BF16 ? "ushort4"
                        : in_dt[j] == PH_I32 ? "int4" : in_dt[j] == PH_BOOL ? "uchar4" : "float4";
-      s += fmt("          if (col0 + %d <= %s) {\n", V, lit_i64(P.cols).c_str());
-      s += fmt("            const %s t = *(const %s *)((const %s *)A.in[%d] + rb%d + col0);\n", vt, vt,
-               stype(in_dt[j]), j, j);
+      s += "          if (" + ccheck(fmt("col0 + %d - 1", V)) + ") {\n";
+      s += fmt("            const %s t = *(const %s *)(rp%d + col0);\n", vt, vt, j);
       s += fmt("            const %s tt[4] = {t.x, t.y, t.z, t.w};\n", stype(in_dt[j]));
       s += fmt("#pragma unroll\n            for (int v = 0; v < 4; ++v) x%d[k][v] = ph_ld(tt, v);\n", j);
+      if (full_cols) {
+        s += "          }\n";
+        continue;
+      }
       s += "          } else {\n";
     } else {
       s += "          {\n";
     }
     s += fmt("#pragma unroll\n            for (int v = 0; v < %d; ++v) {\n              const i64 c = col0 + v;\n", V);
-    s += fmt("              if (c < %s) x%d[k][v] = ph_ld((const %s *)A.in[%d], rb%d + c * %s);\n",
-             lit_i64(P.cols).c_str(), j, stype(in_dt[j]), j, j, lit_i64(cs[j]).c_str());
+    s += "              if (" + ccheck("c") + fmt(") x%d[k][v] = ph_ld(rp%d, c * %s);\n", j, j, lit_i64(cs[j]).c_str());
     s += "            }\n          }\n";
   }
   s += "        }\n      }\n";
@@ -657,35 +691,32 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[k][v]", j));
   std::string body;
   std::string res = emit_prog(p, inexpr, &body, "");
-  // column chunk of 4 rows: plain sum (pairwise level) then one compensated
-  // add into the column state; maxima elementwise
   s += fmt("      F cp[%d], cm[%d];\n#pragma unroll\n      for (int v = 0; v < %d; ++v) { cp[v] = (F)0; cm[v] = -__builtin_inf(); }\n",
            V, V, V);
   s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
   s += "        F ls = (F)0; F lm = -__builtin_inf();\n";
-  s += "        if (r < r1) {\n";
+  s += "        if (" + rcheck + ") {\n";
   s += fmt("#pragma unroll\n          for (int v = 0; v < %d; ++v) {\n            const i64 c = col0 + v;\n", V);
-  s += fmt("            if (c < %s) {\n", lit_i64(P.cols).c_str());
+  s += "            if (" + ccheck("c") + ") {\n";
   s += body;
   s += fmt("              const F val = (F)(%s);\n", res.c_str());
-  if (P.do_cols) s += "              cp[v] += val; cm[v] = (val > cm[v]) ? val : cm[v];\n";
-  if (P.do_rows) s += "              ls += val; lm = (val > lm) ? val : lm;\n";
+  if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], val);\n";
+  if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, val);\n";
   s += "            }\n          }\n        }\n";
   if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
   s += "      }\n";
   if (P.do_cols)
-    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = (cm[v] > cst[v].m) ? cm[v] : cst[v].m; }\n", V);
+    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = ph_fmax(cst[v].m, cm[v]); }\n", V);
   s += "    }\n";
   if (P.do_rows) {
-    s += "    const F fs = tfold16<F, false>(rsv);\n    const F fm = tfold16<F, true>(rmv);\n";
-    s += "    const int rr = (lane >> 2) & 15;\n";
-    s += "    if ((lane & 3) == 0 && g0 + rr < r1) { rsum_sh[g0 - r0 + rr][wave] = fs; rmax_sh[g0 - r0 + rr][wave] = fm; }\n";
+    s += "    const F fs = tfold8<F, false>(rsv);\n    const F fm = tfold8<F, true>(rmv);\n";
+    s += "    const int rr = (lane >> 3) & 7;\n";
+    s += "    if ((lane & 7) == 0 && g0 + rr < r1) { rsum_sh[g0 - r0 + rr][wave] = fs; rmax_sh[g0 - r0 + rr][wave] = fm; }\n";
   }
   s += "  }\n";
   if (P.do_cols) {
     s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = col0 + v;\n", V);
-    s += fmt("    if (c < %s) ((S *)A.colpart)[rt * %s + c] = cst[v];\n  }\n", lit_i64(P.cols).c_str(),
-             lit_i64(P.cols).c_str());
+    s += "    if (" + ccheck("c") + fmt(") ((S *)A.colpart)[rt * %s + c] = cst[v];\n  }\n", C.c_str());
   }
   if (P.do_rows) {
     s += "  __syncthreads();\n";

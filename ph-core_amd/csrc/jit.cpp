@@ -79,13 +79,23 @@ void write_file_atomic(const std::string &path, const std::vector<char> &data) {
 }
 
 int compile(const std::string &source, const std::string &arch, std::vector<char> *code) {
+  // PH_JIT_DUMP=<dir>: keep a copy of every generated source (for inspection)
+  if (const char *dump = std::getenv("PH_JIT_DUMP")) {
+    char name[64];
+    std::snprintf(name, sizeof(name), "/%016llx.hip", (unsigned long long)fnv1a(source));
+    mkdirs(dump);
+    std::ofstream f(std::string(dump) + name);
+    f << source;
+  }
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, source.c_str(), "ph_fused.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS)
     return set_error(PH_E_HIP, "hiprtcCreateProgram failed");
   std::string arch_opt = "--offload-arch=" + arch;
-  const char *opts[] = {arch_opt.c_str(), "-O3", "-ffp-contract=off", "-std=c++17",
-                        "-fno-fast-math"};
+  // no packed FP32 instructions (v_pk_add/mul/fma_f32) in generated code,
+  // exactly as for the ahead-of-time kernels
+  const char *opts[] = {arch_opt.c_str(), "-O3", "-ffp-contract=off", "-std=c++17", "-fno-fast-math",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
   hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
   if (r != HIPRTC_SUCCESS) {
     size_t n = 0;
@@ -102,6 +112,12 @@ int compile(const std::string &source, const std::string &arch, std::vector<char
   hiprtcGetCode(prog, code->data());
   hiprtcDestroyProgram(&prog);
   ++g_compiles;
+  if (const char *dump = std::getenv("PH_JIT_DUMP")) {
+    char name[64];
+    std::snprintf(name, sizeof(name), "/%016llx.co", (unsigned long long)fnv1a(source));
+    std::ofstream f(std::string(dump) + name, std::ios::binary);
+    f.write(code->data(), (std::streamsize)code->size());
+  }
   return PH_OK;
 }
 
