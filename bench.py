@@ -246,8 +246,8 @@ def other_workloads(phase, ev, args):
     for st, esz in [("F32", 4), ("BF16", 2)]:
         x = phase.NArray.random([n, n], getattr(D, st), seed=2, dist="normal", a=0.0, b=1.0)
         pv = x.process(lambda v: (v + 4) ** 2 * 0.5 - v)
-        pv.reduce2d_all()
-        ms = time_loop(ev, lambda: pv.reduce2d_all(), 5)
+        pv.reduce2d_all_device()
+        ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10)
         by = esz * n * n
         out[f"c3_procview_reduce_{st.lower()}_16384"] = {
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),

@@ -809,7 +809,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   s += "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n  int fl = 0;\n";
   if (P.do_rows) {
     s += fmt("  if (i < %s) {\n    S a = ((const S *)A.rowpart)[i];\n", lit_i64(P.rows).c_str());
-    s += fmt("    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
+    s += fmt("#pragma unroll 8\n    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
              lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
     s += "    if (A.row_sum) A.row_sum[i] = st_sum(a);\n    if (A.row_max) A.row_max[i] = st_max(a);\n";
     s += "    if (A.row_sum) fl |= st_ovf(a);\n";
@@ -817,7 +817,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   }
   if (P.do_cols) {
     s += fmt("  if (i < %s) {\n    S a = ((const S *)A.colpart)[i];\n", lit_i64(P.cols).c_str());
-    s += fmt("    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + i]);\n",
+    s += fmt("#pragma unroll 8\n    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + i]);\n",
              lit_i64(P.ntr).c_str(), lit_i64(P.cols).c_str());
     s += "    if (A.col_sum) { A.col_sum[i] = st_sum(a); fl |= st_ovf(a); }\n    if (A.col_max) A.col_max[i] = st_max(a);\n  }\n";
   }

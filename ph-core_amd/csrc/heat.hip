@@ -14,6 +14,7 @@
 // The cell update keeps the operation order of examples/heat_equation.cr:47
 // and :50 extended to two axes, with no contraction into FMAs:
 //   d = (((uN - 4u) + uS) + uW) + uE;  u' = u + d * r
+#include <algorithm>
 #include <cstring>
 
 #include "common.hpp"
@@ -149,6 +150,161 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step(const T *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Two time steps per pass (temporal blocking).  The wavefront sweeps its strip
+// keeping three rows of level t0 (read from memory) and three rows of level
+// t1 (computed) in registers; level t2 is written.  Level-t1 values are also
+// needed one column beyond each strip edge, so lanes 0 and 63 carry two halo
+// columns of t0 and one of t1.  Every t1 and t2 value is computed with the
+// single-step formula, so the result is bit-identical to two single steps;
+// cells outside the grid stay 0 at every level.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Row0 {  // a t0 row of the strip plus two halo columns on each side
+  T v[Vec<T>::n];
+  T w1, w2, e1, e2;  // x0-1, x0-2, x0+W, x0+W+1
+};
+template <typename T>
+struct Row1 {  // a t1 row plus one halo column on each side
+  T v[Vec<T>::n];
+  T w, e;  // x0-1, x0+W
+};
+
+template <typename T>
+__device__ __forceinline__ void load_row0(const T *__restrict__ u, const HeatArgs &a, int64_t y, int64_t x0,
+                                          int lane, bool vec_ok, Row0<T> &r) {
+  constexpr int V = Vec<T>::n;
+  const int64_t x = x0 + (int64_t)lane * V;
+  r.w1 = r.w2 = r.e1 = r.e2 = T(0);
+  if (y < 0 || y >= a.ny) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) r.v[i] = T(0);
+    return;
+  }
+  const T *row = u + y * a.nx;
+  if (vec_ok && x + V <= a.nx) {
+    unpack<T>(*(const typename Vec<T>::type *)(row + x), r.v);
+  } else {
+#pragma unroll
+    for (int i = 0; i < V; ++i) r.v[i] = (x + i < a.nx) ? row[x + i] : T(0);
+  }
+  const int64_t xe = x0 + 64 * V;
+  if (lane == 0) {
+    r.w1 = x0 - 1 >= 0 ? row[x0 - 1] : T(0);
+    r.w2 = x0 - 2 >= 0 ? row[x0 - 2] : T(0);
+  }
+  if (lane == 63) {
+    r.e1 = xe < a.nx ? row[xe] : T(0);
+    r.e2 = xe + 1 < a.nx ? row[xe + 1] : T(0);
+  }
+}
+
+// t1 row from t0 rows n (above), c, s (below); row index y1row.
+template <typename T>
+__device__ __forceinline__ void step_row(const Row0<T> &n, const Row0<T> &c, const Row0<T> &s, T r,
+                                         const HeatArgs &a, int64_t yrow, int64_t x0, int lane, Row1<T> &o) {
+  constexpr int V = Vec<T>::n;
+  const T four = T(4);
+  if (yrow < 0 || yrow >= a.ny) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) o.v[i] = T(0);
+    o.w = o.e = T(0);
+    return;
+  }
+  T wl = __shfl_up(c.v[V - 1], 1, 64);
+  T er = __shfl_down(c.v[0], 1, 64);
+  if (lane == 0) wl = c.w1;
+  if (lane == 63) er = c.e1;
+  const int64_t x = x0 + (int64_t)lane * V;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const T w = i == 0 ? wl : c.v[i - 1];
+    const T e = i == V - 1 ? er : c.v[i + 1];
+    const T d = (((n.v[i] - four * c.v[i]) + s.v[i]) + w) + e;
+    o.v[i] = (x + i < a.nx) ? c.v[i] + d * r : T(0);
+  }
+  o.w = T(0);
+  o.e = T(0);
+  if (lane == 0 && x0 - 1 >= 0) {  // column x0-1: west x0-2, east x0 (own v[0])
+    const T d = (((n.w1 - four * c.w1) + s.w1) + c.w2) + c.v[0];
+    o.w = c.w1 + d * r;
+  }
+  const int64_t xe = x0 + 64 * V;
+  if (lane == 63 && xe < a.nx) {  // column x0+W: west x0+W-1 (own v[V-1]), east x0+W+1
+    const T d = (((n.e1 - four * c.e1) + s.e1) + c.v[V - 1]) + c.e2;
+    o.e = c.e1 + d * r;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict__ u,
+                                                            T *__restrict__ out, HeatArgs a, T r,
+                                                            int vec_ok) {
+  constexpr int V = Vec<T>::n;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t b = blockIdx.x;
+  const int64_t lb = (b % 8) * a.per_xcd + b / 8;
+  if (lb >= a.nblocks) return;
+  const int64_t gpb = (a.nstrips + kWaves - 1) / kWaves;
+  const int64_t strip = (lb % gpb) * kWaves + wave;
+  const int64_t band = lb / gpb;
+  if (strip >= a.nstrips) return;
+  const int64_t x0 = strip * 64 * V;
+  const int64_t y0 = band * kTY;
+  const int64_t y1 = (y0 + kTY < a.ny) ? y0 + kTY : a.ny;
+  const int64_t x = x0 + (int64_t)lane * V;
+  const T four = T(4);
+  // start-up: t0 rows y0-2 .. y0+1, t1 rows y0-1, y0
+  Row0<T> t0a, t0b, t0c, t0d;
+  load_row0<T>(u, a, y0 - 2, x0, lane, vec_ok, t0a);
+  load_row0<T>(u, a, y0 - 1, x0, lane, vec_ok, t0b);
+  load_row0<T>(u, a, y0, x0, lane, vec_ok, t0c);
+  load_row0<T>(u, a, y0 + 1, x0, lane, vec_ok, t0d);
+  Row1<T> P, Q;
+  step_row<T>(t0a, t0b, t0c, r, a, y0 - 1, x0, lane, P);
+  step_row<T>(t0b, t0c, t0d, r, a, y0, x0, lane, Q);
+  Row0<T> A = t0c, B = t0d;
+  constexpr int kAhead = 2;
+  for (int64_t y = y0; y < y1; y += kAhead) {
+    Row0<T> C[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k)
+      if (y + k < y1) load_row0<T>(u, a, y + 2 + k, x0, lane, vec_ok, C[k]);
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      if (y + k >= y1) break;
+      Row1<T> R;
+      step_row<T>(A, B, C[k], r, a, y + k + 1, x0, lane, R);
+      // level t2 row y+k from t1 rows P (above), Q, R (below)
+      T wl = __shfl_up(Q.v[V - 1], 1, 64);
+      T er = __shfl_down(Q.v[0], 1, 64);
+      if (lane == 0) wl = Q.w;
+      if (lane == 63) er = Q.e;
+      T o[V];
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        const T w = i == 0 ? wl : Q.v[i - 1];
+        const T e = i == V - 1 ? er : Q.v[i + 1];
+        const T d = (((P.v[i] - four * Q.v[i]) + R.v[i]) + w) + e;
+        o[i] = Q.v[i] + d * r;
+      }
+      T *orow = out + (y + k) * a.nx;
+      if (vec_ok && x + V <= a.nx) {
+        *(typename Vec<T>::type *)(orow + x) = pack<T>(o);
+      } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+          if (x + i < a.nx) orow[x + i] = o[i];
+      }
+      A = B;
+      B = C[k];
+      P = Q;
+      Q = R;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic FP64 sums: pass 1 gives each workgroup a fixed contiguous
 // chunk (Neumaier-compensated per thread, ordered tree per group); pass 2
 // folds the per-group partials in order in one workgroup.
@@ -275,17 +431,30 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
     if (rc) return rc;
   }
-  for (int64_t t = 0; t < steps; ++t) {
+  // Steps run two at a time (k_heat_step2) up to each energy checkpoint; an
+  // odd remainder runs as a single step.
+  int64_t t = 0;
+  while (t < steps) {
+    int64_t until = steps;
+    if (energy && every > 0) until = std::min<int64_t>(steps, (t / every + 1) * every);
+    const bool pair = until - t >= 2;
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_heat_step<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                       (const T *)cur, nxt, a, rr, vec_ok);
-    rc = ph::check_launch("k_heat_step");
+    if (pair) {
+      hipLaunchKernelGGL(k_heat_step2<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
+                         (const T *)cur, nxt, a, rr, vec_ok);
+      rc = ph::check_launch("k_heat_step2");
+    } else {
+      hipLaunchKernelGGL(k_heat_step<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
+                         (const T *)cur, nxt, a, rr, vec_ok);
+      rc = ph::check_launch("k_heat_step");
+    }
     if (rc) return rc;
     T *sw = cur;
     cur = nxt;
     nxt = sw;
-    if (energy && every > 0 && (t + 1) % every == 0) {
+    t += pair ? 2 : 1;
+    if (energy && every > 0 && t % every == 0) {
       rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
       if (rc) return rc;
     }

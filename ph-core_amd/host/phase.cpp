@@ -2102,6 +2102,21 @@ Reduce2D reduce2d_all(const Operand &a, Acc acc) {
   return r;
 }
 
+NArray reduce2d_all_device(const Operand &a, Acc acc) {
+  if (a.shape.size() != 2) throw DimensionError("reduce2d_all needs a 2-dimensional MultiIndexable");
+  if (count_of(a) == 0) throw EmptyError("Empty enumerable");
+  Prepared p = prepare(a);
+  const int64_t R = a.shape[0], C = a.shape[1];
+  NArray out = NArray::empty({2 + 2 * R + 2 * C}, DType::F64);
+  double *b = (double *)out.storage->ptr;
+  int32_t flags = 0;
+  check(ph_reduce2d_all(&p.prog, 1, p.ptrs.data(), p.descs.data(), (int32_t)acc, b, b + 1, b + 2, b + 2 + R,
+                        b + 2 + 2 * R, b + 2 + 2 * R + C, &flags),
+        "reduce2d_all");
+  check_flags(flags, "reduce2d_all");
+  return out;
+}
+
 bool equals(const Operand &a, const Operand &b) {
   if (a.shape != b.shape) return false;
   if (count_of(a) == 0) return true;

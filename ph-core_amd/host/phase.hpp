@@ -375,6 +375,9 @@ struct Reduce2D {
 };
 // One pass over a 2-D operand: sum and max in full, along slices(0) and slices(1).
 Reduce2D reduce2d_all(const Operand &a, Acc acc = Acc::F32C);
+// The same, leaving the results in device memory: a Float64 NArray laid out
+// [full_sum, full_max, row_sum[R], row_max[R], col_sum[C], col_max[C]].
+NArray reduce2d_all_device(const Operand &a, Acc acc = Acc::F32C);
 
 // Device scratch owned by the caller (e.g. results of reductions).
 struct DeviceDoubles {

@@ -212,6 +212,9 @@ void def_operators(C &cls) {
         d["col_max"] = r.col_max;
         return d;
       }, py::arg("acc") = "f32c");
+  cls.def("reduce2d_all_device", [](py::object self, const std::string &acc) {
+        return reduce2d_all_device(to_operand(self), acc == "f64" ? Acc::F64 : Acc::F32C);
+      }, py::arg("acc") = "f32c");
   cls.def_property_readonly("value_dtype", [](py::object self) { return value_dtype_of(self); });
 }
 
