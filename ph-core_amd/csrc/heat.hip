@@ -15,13 +15,14 @@
 // and :50 extended to two axes, with no contraction into FMAs:
 //   d = (((uN - 4u) + uS) + uW) + uE;  u' = u + d * r
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
 
 namespace {
 
-constexpr int kTY = 64;      // rows per band
+constexpr int kTY = 64;      // default rows per band
 constexpr int kWaves = 4;    // wavefronts (strips) per workgroup
 
 template <typename T> struct Vec;
@@ -49,6 +50,7 @@ __device__ __forceinline__ typename Vec<T>::type pack(const T *o) {
 
 struct HeatArgs {
   int64_t ny, nx;
+  int64_t ty;        // rows per band
   int64_t nstrips;   // strips per row band
   int64_t nbands;
   int64_t nblocks;   // workgroups
@@ -97,8 +99,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step(const T *__restrict__
   const int64_t band = lb / ((a.nstrips + kWaves - 1) / kWaves);
   if (strip >= a.nstrips) return;
   const int64_t x0 = strip * 64 * V;
-  const int64_t y0 = band * kTY;
-  const int64_t y1 = (y0 + kTY < a.ny) ? y0 + kTY : a.ny;
+  const int64_t y0 = band * a.ty;
+  const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
   T n[V], c[V], cw, ce, dummy_w, dummy_e;
   load_row<T>(u, a, y0 - 1, x0, lane, vec_ok, n, &dummy_w, &dummy_e);
   load_row<T>(u, a, y0, x0, lane, vec_ok, c, &cw, &ce);
@@ -235,7 +237,7 @@ __device__ __forceinline__ void step_row(const Row0<T> &n, const Row0<T> &c, con
   }
 }
 
-template <typename T>
+template <typename T, int kAhead>
 __global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict__ u,
                                                             T *__restrict__ out, HeatArgs a, T r,
                                                             int vec_ok) {
@@ -250,8 +252,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict_
   const int64_t band = lb / gpb;
   if (strip >= a.nstrips) return;
   const int64_t x0 = strip * 64 * V;
-  const int64_t y0 = band * kTY;
-  const int64_t y1 = (y0 + kTY < a.ny) ? y0 + kTY : a.ny;
+  const int64_t y0 = band * a.ty;
+  const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
   const int64_t x = x0 + (int64_t)lane * V;
   const T four = T(4);
   // start-up: t0 rows y0-2 .. y0+1, t1 rows y0-1, y0
@@ -264,7 +266,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict_
   step_row<T>(t0a, t0b, t0c, r, a, y0 - 1, x0, lane, P);
   step_row<T>(t0b, t0c, t0d, r, a, y0, x0, lane, Q);
   Row0<T> A = t0c, B = t0d;
-  constexpr int kAhead = 2;
   for (int64_t y = y0; y < y1; y += kAhead) {
     Row0<T> C[kAhead];
 #pragma unroll
@@ -416,7 +417,15 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   a.ny = ny;
   a.nx = nx;
   a.nstrips = (nx + 64 * V - 1) / (64 * V);
-  a.nbands = (ny + kTY - 1) / kTY;
+  // Band height and row prefetch depth of the two-step kernel, per precision
+  // (measured on MI355X at 8192^2: FP32 32 rows / no prefetch, FP64 16 rows /
+  // 2 rows ahead); PH_HEAT_TY and PH_HEAT_AHEAD override them for tuning.
+  int64_t ty_env = sizeof(T) == 4 ? 32 : 16;
+  int64_t ahead_env = sizeof(T) == 4 ? 1 : 2;
+  if (const char *e = std::getenv("PH_HEAT_TY")) ty_env = std::max<int64_t>(8, std::atol(e));
+  if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead_env = std::atol(f);
+  a.ty = ty_env;
+  a.nbands = (ny + a.ty - 1) / a.ty;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
   a.nblocks = groups_per_band * a.nbands;
   a.per_xcd = (a.nblocks + 7) / 8;
@@ -441,8 +450,15 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
     if (pair) {
-      hipLaunchKernelGGL(k_heat_step2<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                         (const T *)cur, nxt, a, rr, vec_ok);
+      if (ahead_env == 4)
+        hipLaunchKernelGGL((k_heat_step2<T, 4>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
+                           (const T *)cur, nxt, a, rr, vec_ok);
+      else if (ahead_env == 1)
+        hipLaunchKernelGGL((k_heat_step2<T, 1>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
+                           (const T *)cur, nxt, a, rr, vec_ok);
+      else
+        hipLaunchKernelGGL((k_heat_step2<T, 2>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
+                           (const T *)cur, nxt, a, rr, vec_ok);
       rc = ph::check_launch("k_heat_step2");
     } else {
       hipLaunchKernelGGL(k_heat_step<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
