@@ -9,8 +9,8 @@ scatter, Int32 8192x8192, values U[0, 2^31) from seed 1.  One step is
 HBM GB/s of the whole step with inputs resident in HBM:
     gather   8 B/element x 11,186,176 elements            =  89.49 MB
     fill+scatter: 4 B x 67,108,864 written + 4 B x 11,186,176 read = 313.18 MB
-The other workloads (broadcast, fused ProcView reductions, heat stencil) are
-reported under "extra".
+The other workloads (broadcast, fused ProcView reductions, heat stencil,
+batched small arrays) are reported under "extra".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -280,20 +280,73 @@ def other_workloads(phase, ev, args):
         res[dt] = {"ms_total": round(ms, 2), "gcells": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
                    "energy_first_last": [e[0], e[-1]]}
         fields[dt] = u
-    lib = ev.lib
-    buf = phase.NArray.empty([2], D.F64)  # device scratch for the two sums
-    lib.ph_l2_diff.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
-                               ctypes.c_void_p]
-    # relative L2 of the FP32 run against the FP64 run (FP64 sums on the device)
+    # relative L2 of the FP32 run against the FP64 run
     f32 = fields["F32"].to_numpy().astype(np.float64)
     f64 = fields["F64"].to_numpy()
     rel = float(np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)))
-    del buf
     out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
                            "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
                            "rel_l2_f32_vs_f64": rel,
                            "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0}
+    del fields, u
+    out["c5_batched_20000"] = batched_workload(phase, ev, args)
     return out
+
+
+def batched_workload(phase, ev, args):
+    """20000 Float32 arrays (rank 1-3, sides log-uniform [8, 512], seed 4,
+    arrays over 2^18 elements drawn again): per array y = x + s_k, x.sum and
+    the step-2 slice copy, as one graph replay of the persistent kernel, as
+    three launches per array, and on the CPU (the oracle's per-array loop)."""
+    import numpy as np
+    from phase.workloads import batch_stats, batched_shapes
+    shapes, scalars, redraws = batched_shapes()
+    st = batch_stats(shapes)
+    ne, nz = st["elements"], st["slice_elements"]
+    op_bytes = 12 * ne + 8 * nz        # add 8 B, sum 4 B per element; copy 8 B per slice element
+    pass_bytes = 8 * ne + 4 * nz       # one fused pass: read x, write y and z
+    b = phase.Batch(shapes, scalars.tolist())
+    b.fill_uniform(4)
+    chunk, grid, dyn = b.tune()
+    b.run(0)
+    ms_graph = time_loop(ev, lambda: b.run(0), 20)
+    ms_direct = time_loop(ev, lambda: b.run(1), 20)
+    b.run(2)
+    ev.sync()
+    t0 = time.perf_counter()
+    ms_unbatched = time_loop(ev, lambda: b.run(2), 2)
+    wall_unbatched = (time.perf_counter() - t0) / 2
+    res = {
+        "arrays": st["arrays"], "elements": ne, "slice_elements": nz, "redraws_over_cap": redraws,
+        "cap_elements": 1 << 18, "log2_histogram": st["log2_histogram"],
+        "schedule": {"chunk": chunk, "grid": grid, "queue": bool(dyn)},
+        "graph_ms": round(ms_graph, 4), "direct_ms": round(ms_direct, 4),
+        "unbatched_ms": round(ms_unbatched, 3), "unbatched_wall_ms": round(wall_unbatched * 1e3, 3),
+        "speedup_vs_unbatched": round(ms_unbatched / ms_graph, 2),
+        "op_bytes": op_bytes, "op_gbs": round(op_bytes / ms_graph / 1e6, 1),
+        "pass_bytes": pass_bytes, "pass_gbs": round(pass_bytes / ms_graph / 1e6, 1),
+        "pass_frac": round(pass_bytes / ms_graph / 1e6 / PEAK_HBM_GBS, 4),
+        "note": "op_bytes counts each op's own traffic (12 B/element + 8 B/slice element); the fused pass "
+                "moves pass_bytes (8 B/element + 4 B/slice element), whose rate is the roofline figure",
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_api as O
+        x = b.x.to_numpy()
+        xo, zo = np.array(b.offsets_x), np.array(b.offsets_z)
+        k = 4000  # bounded sample: the first 4000 arrays
+        sub = shapes[:k]
+        end = int(xo[k])
+        t0 = time.perf_counter()
+        O.batch_loop(sub, scalars[:k], x[:end])
+        dt = time.perf_counter() - t0
+        sb = 12 * sum(int(np.prod(s)) for s in sub) + 8 * sum(int(np.prod([(d + 1) // 2 for d in s])) for s in sub)
+        res["cpu"] = {"value": round(sb / dt / 1e9, 3), "unit": "GB/s (op bytes)", "cores": 1, "kind": "port",
+                      "sample": f"first {k} arrays, oracle per-array loop (map +, Float32 sum, step-2 view "
+                                f"gather), {dt * 1e3:.1f} ms",
+                      "est_full_batch_ms": round(dt * op_bytes / sb * 1e3, 1)}
+    del b
+    return res
 
 
 if __name__ == "__main__":

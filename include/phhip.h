@@ -328,6 +328,55 @@ int ph_l2_diff(int32_t dtype_a, const void *a, int32_t dtype_b, const void *b,
                int64_t n, double *out);
 
 /* ---------------------------------------------------------------------------
+ * Batched small arrays.  No reference counterpart: ph-core loops over its
+ * arrays on the host.  For each Float32 array x_k (rank 1..3) of a batch it
+ * computes, in one persistent-kernel pass over the packed buffer,
+ *   y_k = x_k + s_k                      per element, like `x_k.map &.+(s_k)`
+ *                                        (n_array.cr:589-595) in Float32;
+ *   sums[k] = x_k.sum                    Enumerable#sum over `each` (FP32
+ *                                        Neumaier accumulation, returned as
+ *                                        FP64);
+ *   z_k = x_k.view(...).to_narr          every axis from 0 with step 2
+ *                                        (multi_indexable.cr:881-887,
+ *                                        view.cr:123).
+ * The caller packs x_k row-major at the offsets ph_batch_sizes reports
+ * (16-byte aligned), and receives y at the same offsets and z_k packed
+ * contiguously.  Sums are deterministic (each array's partials are folded in
+ * a fixed order whatever the schedule).
+ * ------------------------------------------------------------------------- */
+/* ranks[n], shapes[3*n] (row k: its first ranks[k] entries, each in
+ * [1, 2^30], at most 2^31-1 elements per array), scalars[n].  An extent of 0
+ * is PH_E_INDEX: the step-2 view of an empty axis raises IndexError in the
+ * reference (range_syntax.cr:120-122).  x, y, z, sums
+ * are device pointers (may be NULL here and bound later with ph_batch_bind).
+ * The batch owns its descriptor table and work queue on the device. */
+int ph_batch_create(void **handle, int32_t n, const int32_t *ranks,
+                    const int64_t *shapes, const float *x, const float *scalars,
+                    float *y, float *z, double *sums);
+/* Element counts of the packed x/y and z buffers, and per-array offsets
+ * (offsets_x / offsets_z may be NULL, else n entries each). */
+int ph_batch_sizes(void *handle, int64_t *xy_elements, int64_t *z_elements,
+                   int64_t *offsets_x, int64_t *offsets_z);
+int ph_batch_bind(void *handle, const float *x, float *y, float *z,
+                  double *sums);
+/* mode 0: one replay of the HIP graph holding the whole batch (captured on
+ * first use); 1: the same three kernels launched directly; 2: the unbatched
+ * form, three kernels per array (add, sum, slice copy). */
+int ph_batch_run(void *handle, int32_t mode);
+/* Times candidate schedules on the bound buffers: work-item size (an array
+ * larger than that is split into several items, each a wavefront's work),
+ * workgroups per CU, and item assignment (a device queue, or a static
+ * largest-first round robin); keeps the fastest, cached per histogram of
+ * array sizes, and reports it (any output may be NULL).  Overwrites y, z and
+ * sums. */
+int ph_batch_tune(void *handle, int64_t *chunk_out, int32_t *grid_out,
+                  int32_t *dynamic_out);
+/* Sets a schedule explicitly (chunk >= 64 elements, grid workgroups). */
+int ph_batch_schedule(void *handle, int64_t chunk, int32_t grid,
+                      int32_t dynamic);
+int ph_batch_destroy(void *handle);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no reference counterpart).
  * ph_compile_check generates and compiles, without launching, the fused
  * kernels ph_map (kind 0) and ph_reduce2d_all (kind 1) would use for `prog`

@@ -1183,6 +1183,14 @@ double or_sum_seq_f64(const double *x, int64_t n) {
   return s;
 }
 
+// Enumerable#sum for Float32 elements: sequential lex-order sum in Float32
+// from Float32 zero (the additive identity of the element type).
+float or_sum_seq_f32(const float *x, int64_t n) {
+  float s = 0.0f;
+  for (int64_t i = 0; i < n; ++i) s += x[i];
+  return s;
+}
+
 // Enumerable#max: keeps the first element, replaces on `>`.
 double or_max_seq_f64(const double *x, int64_t n) {
   if (n == 0) return 0.0;
@@ -1267,6 +1275,46 @@ int or_slices_reduce(const double *x, const int64_t *shape, int rank, int axis,
     if (mode == 0) out[j] = or_sum_seq_f64(buf.data(), (int64_t)buf.size());
     else if (mode == 1) out[j] = or_max_seq_f64(buf.data(), (int64_t)buf.size());
     else out[j] = or_sum_exact(buf.data(), (int64_t)buf.size());
+  }
+  return OK;
+}
+
+// ===========================================================================
+// Batched small arrays: the host loop ph-core would run over a list of
+// Float32 NArrays (no batched API exists in the reference):
+//   y_k = x_k.map { |e| e + s_k }            n_array.cr:589-595 (Float32 +)
+//   sums[k] = x_k.sum                        Enumerable#sum, Float32 sequence
+//   z_k = x_k.view(0..2.., ...).to_narr      multi_indexable.cr:881-887,
+//                                            view.cr:123 (region 0..2.. per axis)
+// Arrays are packed: array k at element xoffs[k] of x / y, its slice at
+// zoffs[k] of z.  shapes is n x 3, row k holding ranks[k] extents.
+// ===========================================================================
+int or_batch_loop(int32_t n, const int32_t *ranks, const int64_t *shapes, const int64_t *xoffs,
+                  const int64_t *zoffs, const float *scalars, const float *x, float *y, float *z,
+                  double *sums, char *msg, int msglen) {
+  for (int32_t k = 0; k < n; ++k) {
+    const int r = ranks[k];
+    const int64_t *shp = shapes + 3 * (int64_t)k;
+    int64_t cnt = 1;
+    for (int i = 0; i < r; ++i) cnt *= shp[i];
+    const float *xk = x + xoffs[k];
+    float *yk = y + xoffs[k];
+    const float s = scalars[k];
+    for (int64_t i = 0; i < cnt; ++i) yk[i] = xk[i] + s;
+    sums[k] = (double)or_sum_seq_f32(xk, cnt);
+    or_lit lits[OR_MAXR];
+    std::memset(lits, 0, sizeof(lits));
+    for (int i = 0; i < r; ++i) {
+      lits[i].kind = 0;
+      lits[i].has_first = 1;
+      lits[i].first = 0;
+      lits[i].has_step = 1;
+      lits[i].step = 2;
+    }
+    or_region reg;
+    int rc = or_region_new(lits, r, shp, r, 1, &reg, msg, msglen);
+    if (rc != OK) return rc;
+    or_gather(xk, shp, r, 4, &reg, z + zoffs[k]);
   }
   return OK;
 }

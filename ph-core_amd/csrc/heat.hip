@@ -22,7 +22,6 @@
 
 namespace {
 
-constexpr int kTY = 64;      // default rows per band
 constexpr int kWaves = 4;    // wavefronts (strips) per workgroup
 
 template <typename T> struct Vec;

@@ -2146,4 +2146,91 @@ std::vector<double> heat2d(NArray &u, double r, int64_t steps, int64_t every) {
   return h;
 }
 
+// ===========================================================================
+// Batched small arrays
+// ===========================================================================
+Batch::Batch(const std::vector<Shape> &shapes_, const std::vector<float> &scalars_)
+    : shapes(shapes_), scalars(scalars_) {
+  init();
+  if (shapes.size() != scalars.size()) throw ArgumentError("one scalar per array");
+  if (shapes.size() > (size_t)INT32_MAX) throw ArgumentError("too many arrays");
+  const int32_t n = (int32_t)shapes.size();
+  std::vector<int32_t> ranks((size_t)n);
+  std::vector<int64_t> flat((size_t)n * 3, 0);
+  for (int32_t k = 0; k < n; ++k) {
+    const Shape &s = shapes[(size_t)k];
+    if (s.empty() || s.size() > 3) throw DimensionError("batched arrays have rank 1, 2 or 3");
+    for (int64_t d : s)
+      if (d < 0) throw DimensionError("Cannot create NArray: One or more of the provided dimensions was negative.");
+    ranks[(size_t)k] = (int32_t)s.size();
+    for (size_t i = 0; i < s.size(); ++i) flat[(size_t)k * 3 + i] = s[i];
+  }
+  check(ph_batch_create(&handle_, n, ranks.data(), flat.data(), nullptr, scalars.data(), nullptr, nullptr, nullptr),
+        "batch");
+  int64_t nxy = 0, nz = 0;
+  offsets_x.resize((size_t)n);
+  offsets_z.resize((size_t)n);
+  check(ph_batch_sizes(handle_, &nxy, &nz, offsets_x.data(), offsets_z.data()), "batch sizes");
+  x = NArray::empty({nxy}, DType::F32);
+  y = NArray::empty({nxy}, DType::F32);
+  z = NArray::empty({nz}, DType::F32);
+  sums_dev = std::make_unique<DeviceDoubles>(std::max<int64_t>(n, 1));
+  check(ph_batch_bind(handle_, (const float *)x.device_ptr(), (float *)y.device_ptr(), (float *)z.device_ptr(),
+                      sums_dev->ptr),
+        "batch bind");
+}
+
+Batch::~Batch() {
+  if (handle_) ph_batch_destroy(handle_);
+}
+
+NArray Batch::piece(const NArray &flat, int64_t off, const Shape &shape) const {
+  NArray r = NArray::empty(shape, DType::F32);
+  const int64_t n = r.size();
+  if (n) check(ph_d2d(r.device_ptr(), (const float *)flat.device_ptr() + off, (size_t)n * 4), "batch element copy");
+  return r;
+}
+
+static void check_k(int64_t k, int64_t n) {
+  if (k < 0 || k >= n) throw IndexError("batch index out of range");
+}
+
+NArray Batch::x_at(int64_t k) const {
+  check_k(k, count());
+  return piece(x, offsets_x[(size_t)k], shapes[(size_t)k]);
+}
+NArray Batch::y_at(int64_t k) const {
+  check_k(k, count());
+  return piece(y, offsets_x[(size_t)k], shapes[(size_t)k]);
+}
+NArray Batch::z_at(int64_t k) const {
+  check_k(k, count());
+  Shape zs = shapes[(size_t)k];
+  for (int64_t &d : zs) d = (d + 1) / 2;
+  return piece(z, offsets_z[(size_t)k], zs);
+}
+
+void Batch::fill_uniform(uint64_t seed, double a, double b) {
+  check(ph_fill_random(x.device_ptr(), PH_F32, x.size(), seed, 0, 0, a, b), "batch random fill");
+}
+
+void Batch::run(int mode) { check(ph_batch_run(handle_, mode), "batch run"); }
+
+std::tuple<int64_t, int, int> Batch::tune() {
+  int64_t chunk = 0;
+  int32_t grid = 0, dynamic = 0;
+  check(ph_batch_tune(handle_, &chunk, &grid, &dynamic), "batch tune");
+  return {chunk, (int)grid, (int)dynamic};
+}
+
+void Batch::schedule(int64_t chunk, int grid, int dynamic) {
+  check(ph_batch_schedule(handle_, chunk, grid, dynamic), "batch schedule");
+}
+
+std::vector<double> Batch::sums() const {
+  std::vector<double> h = sums_dev->to_host();
+  h.resize((size_t)count());
+  return h;
+}
+
 }  // namespace phase

@@ -20,6 +20,7 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/phhip.h"
@@ -392,6 +393,42 @@ struct DeviceDoubles {
 // generalised to 2-D, Dirichlet zero): steps in place on `u`; returns the FP64
 // energies at 0, every, 2*every, ... (empty if every == 0).
 std::vector<double> heat2d(NArray &u, double r, int64_t steps, int64_t every);
+
+// A batch of small Float32 arrays (rank 1..3) packed into one device buffer.
+// run() computes, for every array k, what the host loop
+//     y_k = x_k.map &.+(s_k); sums[k] = x_k.sum; z_k = x_k.view(0..2.., ...).to_narr
+// computes, in one persistent launch (mode 0 replays a HIP graph; 1 launches
+// directly; 2 is the unbatched three-launches-per-array form).
+class Batch {
+ public:
+  Batch(const std::vector<Shape> &shapes, const std::vector<float> &scalars);
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  int64_t count() const { return (int64_t)shapes.size(); }
+  // x as array k (a copy), y_k, z_k (copies)
+  NArray x_at(int64_t k) const;
+  NArray y_at(int64_t k) const;
+  NArray z_at(int64_t k) const;
+  // x (whole packed buffer, pads included) <- U[a, b) draws of stream `seed`
+  void fill_uniform(uint64_t seed, double a, double b);
+  void run(int mode = 0);
+  // (work-item elements, workgroups, queue 1 / static 0)
+  std::tuple<int64_t, int, int> tune();
+  void schedule(int64_t chunk, int grid, int dynamic);
+  std::vector<double> sums() const;
+
+  std::vector<Shape> shapes;
+  std::vector<float> scalars;
+  std::vector<int64_t> offsets_x, offsets_z;
+  NArray x, y, z;  // packed, flat Float32
+  std::unique_ptr<DeviceDoubles> sums_dev;
+
+ private:
+  void *handle_ = nullptr;
+  NArray piece(const NArray &flat, int64_t off, const Shape &shape) const;
+};
 
 // Initialises the device (idempotent).
 void init(int device = 0);

@@ -552,4 +552,25 @@ PYBIND11_MODULE(_phase, m) {
       .def_property_readonly("body", [](const ProcView &v) { return v.body; });
 
   m.def("heat2d", &phase::heat2d, py::arg("u"), py::arg("r"), py::arg("steps"), py::arg("every") = 0);
+
+  py::class_<Batch>(m, "Batch")
+      .def(py::init([](const std::vector<Shape> &shapes, const std::vector<float> &scalars) {
+             return std::make_unique<Batch>(shapes, scalars);
+           }),
+           py::arg("shapes"), py::arg("scalars"))
+      .def("__len__", &Batch::count)
+      .def("x_at", &Batch::x_at)
+      .def("y_at", &Batch::y_at)
+      .def("z_at", &Batch::z_at)
+      .def("fill_uniform", &Batch::fill_uniform, py::arg("seed"), py::arg("a") = -1.0, py::arg("b") = 1.0)
+      .def("run", &Batch::run, py::arg("mode") = 0)
+      .def("tune", &Batch::tune)
+      .def("schedule", &Batch::schedule, py::arg("chunk"), py::arg("grid"), py::arg("dynamic") = 1)
+      .def("sums", &Batch::sums)
+      .def_readonly("shapes", &Batch::shapes)
+      .def_readonly("offsets_x", &Batch::offsets_x)
+      .def_readonly("offsets_z", &Batch::offsets_z)
+      .def_property_readonly("x", [](const Batch &b) { return b.x; })
+      .def_property_readonly("y", [](const Batch &b) { return b.y; })
+      .def_property_readonly("z", [](const Batch &b) { return b.z; });
 }

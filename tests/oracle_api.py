@@ -100,6 +100,10 @@ def lib():
         L.or_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
         L.or_sum_seq_f64.argtypes = [C.c_void_p, C.c_int64]
         L.or_sum_seq_f64.restype = C.c_double
+        L.or_sum_seq_f32.argtypes = [C.c_void_p, C.c_int64]
+        L.or_sum_seq_f32.restype = C.c_float
+        L.or_batch_loop.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
         L.or_max_seq_f64.argtypes = [C.c_void_p, C.c_int64]
         L.or_max_seq_f64.restype = C.c_double
         L.or_sum_exact.argtypes = [C.c_void_p, C.c_int64]
@@ -359,6 +363,48 @@ def sum_seq(x):
 def max_seq(x):
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
     return lib().or_max_seq_f64(_ptr(x), x.size)
+
+
+def sum_seq_f32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    return lib().or_sum_seq_f32(_ptr(x), x.size)
+
+
+def batch_layout(shapes):
+    """Packed layout of a batch: x/y offsets (16-byte aligned), z offsets,
+    totals (the same layout the device batch reports)."""
+    xo, zo, ox, oz = [], [], 0, 0
+    for s in shapes:
+        n = int(np.prod(s))
+        xo.append(ox)
+        zo.append(oz)
+        ox += (n + 3) // 4 * 4
+        oz += int(np.prod([(d + 1) // 2 for d in s]))
+    return np.array(xo, np.int64), np.array(zo, np.int64), ox, oz
+
+
+def batch_loop(shapes, scalars, x, xoffs=None, zoffs=None):
+    """The per-array host loop (y = x + s, Float32 sequential sum, step-2
+    slice) over packed x; returns (y, z, sums)."""
+    n = len(shapes)
+    lay = batch_layout(shapes)
+    if xoffs is None:
+        xoffs, zoffs = lay[0], lay[1]
+    ranks = np.array([len(s) for s in shapes], np.int32)
+    shp = np.zeros((max(n, 1), 3), np.int64)
+    for k, s in enumerate(shapes):
+        shp[k, :len(s)] = s
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x)
+    z = np.zeros(max(lay[3], 1), np.float32)
+    sums = np.zeros(max(n, 1), np.float64)
+    sc = np.ascontiguousarray(scalars, dtype=np.float32)
+    xoffs = np.ascontiguousarray(xoffs, np.int64)
+    zoffs = np.ascontiguousarray(zoffs, np.int64)
+    buf = C.create_string_buffer(256)
+    _check(lib().or_batch_loop(n, _ptr(ranks), _ptr(shp), _ptr(xoffs), _ptr(zoffs), _ptr(sc), _ptr(x), _ptr(y),
+                               _ptr(z), _ptr(sums), buf, 256), buf)
+    return y, z[:lay[3]], sums[:n]
 
 
 def sum_exact(x):
