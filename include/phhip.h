@@ -354,7 +354,9 @@ int ph_batch_create(void **handle, int32_t n, const int32_t *ranks,
                     const int64_t *shapes, const float *x, const float *scalars,
                     float *y, float *z, double *sums);
 /* Element counts of the packed x/y and z buffers, and per-array offsets
- * (offsets_x / offsets_z may be NULL, else n entries each). */
+ * (offsets_x / offsets_z may be NULL, else n entries each).  x and y must
+ * hold xy_elements (every array padded to a multiple of 4 elements; the
+ * kernel reads and writes whole 16-byte vectors) and be 16-byte aligned. */
 int ph_batch_sizes(void *handle, int64_t *xy_elements, int64_t *z_elements,
                    int64_t *offsets_x, int64_t *offsets_z);
 int ph_batch_bind(void *handle, const float *x, float *y, float *z,
@@ -371,7 +373,8 @@ int ph_batch_run(void *handle, int32_t mode);
  * sums. */
 int ph_batch_tune(void *handle, int64_t *chunk_out, int32_t *grid_out,
                   int32_t *dynamic_out);
-/* Sets a schedule explicitly (chunk >= 64 elements, grid workgroups). */
+/* Sets a schedule explicitly (chunk a multiple of 64 elements, grid
+ * workgroups of 256 threads, dynamic 1 = queue, 0 = static). */
 int ph_batch_schedule(void *handle, int64_t chunk, int32_t grid,
                       int32_t dynamic);
 int ph_batch_destroy(void *handle);

@@ -3,19 +3,21 @@
 //     y_k = x_k + s_k                  (NArray#map / scalar operator)
 //     sum_k = x_k.sum                  (Enumerable#sum)
 //     z_k = x_k.view(0..2.., ...).to_narr   (step-2 slice on every axis)
-// in ONE pass over x: a persistent kernel whose wavefronts pull work items
-// (an array, or a chunk of a large array) from a device-side queue.  Each
-// element of x is read once; y is written; the element is also written to z
-// when all of its coordinates are even.  Per-item partial sums (FP32
-// Neumaier, folded across the wavefront with TwoSum) land in fixed slots, and
-// a second kernel folds each array's slots in item order, so the sums do not
-// depend on which wavefront ran which item.  The three kernels (queue reset,
-// main, finalize) are captured once in a HIP graph and replayed.
+// in one pass over x: a persistent kernel whose wavefronts run work items
+// (an array, or a chunk of a large one) from a device-side table.  A
+// wavefront streams its item's elements once (16-byte loads of x, 16-byte
+// non-temporal stores of y, FP32 Neumaier sums), then gathers the item's
+// slice elements from the lines it has just read (L2 hits) and writes them
+// to z contiguously.  Per-item partial sums land in fixed slots and a second
+// kernel folds each array's slots in item order, so the sums do not depend
+// on which wavefront ran which item.  The kernels are captured once in a HIP
+// graph and replayed.
 //
-// Schedule: items are chunks of at most `chunk` elements (arrays no larger
-// than a chunk are one wavefront each), dispatched largest first; the
-// autotuner times a few (chunk, workgroups-per-CU) pairs on the batch itself
-// and keeps the fastest, cached by a histogram of array sizes.
+// Schedule: items are chunks of at most `chunk` elements, ordered largest
+// first; wavefronts take them either round-robin (static) or from an atomic
+// queue (dynamic).  The autotuner times (chunk, workgroups per CU, static or
+// dynamic) on the batch itself and keeps the fastest, cached by a histogram
+// of array sizes.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,31 +28,34 @@
 
 namespace {
 
-struct BArray {
+struct BArray {  // host-side description of one array
   int64_t xoff, zoff;   // element offsets into the packed x/y and z buffers
   int64_t n;            // elements
   int32_t rank, pad_;
   uint32_t shp[3];      // shape (rank padded on the left with 1)
   uint32_t zshp[3];     // slice shape: ceil(shp/2)
-  uint32_t m_c, l_c, m_b, l_b;  // magic divisors for shp[2], shp[1]
   float scalar, pad2_;
 };
 
-struct BItem {
-  int32_t arr;
-  int32_t pad_;
-  int64_t lo, hi;  // element range [lo, hi) of the array
+// One work item, in dispatch order: everything its wavefront needs.
+struct BRec {
+  int64_t xoff, zoff;               // the array's offsets in x/y and z
+  uint32_t lo, hi;                  // element range [lo, hi) of the array
+  uint32_t zlo, zhi;                // slice elements whose source lies in [lo, hi)
+  uint32_t shp[3], zshp[3];
+  uint32_t mz_c, lz_c, mz_b, lz_b;  // magic divisors for zshp[2], zshp[1]
+  float scalar;
+  int32_t slot;                     // partial-sum slot (array-major item index)
 };
+static_assert(sizeof(BRec) == 80, "BRec layout");
 
 struct Batch {
   int n = 0;
   std::vector<BArray> arrays;
   int64_t total = 0, ztotal = 0;
   // device buffers owned by the batch
-  BArray *d_arrays = nullptr;
-  BItem *d_items = nullptr;       // canonical order (array-major)
-  int32_t *d_order = nullptr;     // dispatch order -> canonical item
-  int32_t *d_first = nullptr;     // first canonical item of each array (n+1)
+  BRec *d_recs = nullptr;         // work items in dispatch order
+  int32_t *d_first = nullptr;     // first slot of each array (n+1)
   float2 *d_partial = nullptr;
   int32_t *d_counter = nullptr;
   int64_t nitems = 0;
@@ -94,24 +99,7 @@ __device__ __forceinline__ void two_sum_fold(float &s, float &c, float s2, float
   c = (c + c2) + err;
 }
 
-// One element: y = x + s, compensated sum, slice write when all coordinates
-// are even.
-__device__ __forceinline__ void do_elem(const BArray &A, const float *__restrict__ x, float *__restrict__ y,
-                                        float *__restrict__ z, uint32_t e, float &s, float &c) {
-  const float v = x[A.xoff + e];
-  y[A.xoff + e] = v + A.scalar;
-  neu_add(s, c, v);
-  // coordinates (i, j, k) of e in shp[0] x shp[1] x shp[2]
-  const uint32_t t = fdiv(e, A.m_c, A.l_c);
-  const uint32_t k = e - t * A.shp[2];
-  const uint32_t i = fdiv(t, A.m_b, A.l_b);
-  const uint32_t j = t - i * A.shp[1];
-  if (((i | j | k) & 1u) == 0) {
-    const uint32_t zi = ((i >> 1) * A.zshp[1] + (j >> 1)) * A.zshp[2] + (k >> 1);
-    z[A.zoff + zi] = v;
-  }
-}
-
+// Butterfly fold of the 64 lanes' (sum, compensation) pairs with TwoSum.
 __device__ __forceinline__ void wave_fold(float &s, float &c) {
   for (int off = 32; off >= 1; off >>= 1) {
     const float s2 = __shfl_xor(s, off, 64);
@@ -120,14 +108,26 @@ __device__ __forceinline__ void wave_fold(float &s, float &c) {
   }
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void neu_add4(float &s, float &c, f4 v) {
+  neu_add(s, c, v.x);
+  neu_add(s, c, v.y);
+  neu_add(s, c, v.z);
+  neu_add(s, c, v.w);
+}
+
 // Persistent: each wavefront runs work items until none are left.  dynamic:
 // items come from a queue (one atomic per item, lane 0); static: wavefront w
 // of W takes dispatch positions w, w + W, w + 2W, ... of the largest-first
-// order, which needs no atomics.
-__global__ __launch_bounds__(256) void k_batch(const BArray *__restrict__ arrays, const BItem *__restrict__ items,
-                                               const int32_t *__restrict__ order, int32_t nitems,
-                                               int32_t *counter, int32_t dynamic, const float *__restrict__ x,
-                                               float *__restrict__ y, float *__restrict__ z, float2 *partial) {
+// order, which needs no atomics.  Lane l handles elements lo + 4l + 256m
+// (m = 0, 1, ...) as 16-byte vectors: every item starts at a multiple of 4
+// and every array is padded to 4 elements in the packed layout, so a vector
+// never leaves x; a partial last vector is stored element by element.
+__global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, int32_t nitems, int32_t *counter,
+                                               int32_t dynamic, const float *__restrict__ x,
+                                               float *__restrict__ y, float *__restrict__ z,
+                                               float2 *__restrict__ partial) {
   const int lane = threadIdx.x & 63;
   const int32_t nwaves = (int32_t)gridDim.x * 4;
   int32_t q = (int32_t)blockIdx.x * 4 + (int32_t)(threadIdx.x >> 6);
@@ -137,22 +137,50 @@ __global__ __launch_bounds__(256) void k_batch(const BArray *__restrict__ arrays
     q = __builtin_amdgcn_readfirstlane(t);
   }
   while (q < nitems) {  // every wavefront leaves once the items are exhausted
-    const int32_t it = order[q];
-    const BItem I = items[it];
-    const BArray A = arrays[I.arr];
+    const BRec R = recs[q];
+    const float *__restrict__ xa = x + R.xoff;
+    float *__restrict__ ya = y + R.xoff;
     float s = 0.f, c = 0.f;
-    uint32_t e = (uint32_t)I.lo + (uint32_t)lane;
-    const uint32_t hi = (uint32_t)I.hi;
-    // four elements in flight per lane
-    for (; e + 192 < hi; e += 256) {
-      do_elem(A, x, y, z, e, s, c);
-      do_elem(A, x, y, z, e + 64, s, c);
-      do_elem(A, x, y, z, e + 128, s, c);
-      do_elem(A, x, y, z, e + 192, s, c);
+    uint32_t e = R.lo + 4u * (uint32_t)lane;
+    for (; e + 260 <= R.hi; e += 512) {  // two full vectors per lane in flight
+      const f4 v0 = *(const f4 *)(xa + e);
+      const f4 v1 = *(const f4 *)(xa + e + 256);
+      __builtin_nontemporal_store(v0 + R.scalar, (f4 *)(ya + e));
+      __builtin_nontemporal_store(v1 + R.scalar, (f4 *)(ya + e + 256));
+      neu_add4(s, c, v0);
+      neu_add4(s, c, v1);
     }
-    for (; e < hi; e += 64) do_elem(A, x, y, z, e, s, c);
+    for (; e < R.hi; e += 256) {
+      const f4 v = *(const f4 *)(xa + e);
+      const f4 r = v + R.scalar;
+      if (e + 4 <= R.hi) {
+        __builtin_nontemporal_store(r, (f4 *)(ya + e));
+        neu_add4(s, c, v);
+      } else {
+        const uint32_t cnt = R.hi - e;  // 1..3
+        ya[e] = r.x;
+        neu_add(s, c, v.x);
+        if (cnt > 1) {
+          ya[e + 1] = r.y;
+          neu_add(s, c, v.y);
+        }
+        if (cnt > 2) {
+          ya[e + 2] = r.z;
+          neu_add(s, c, v.z);
+        }
+      }
+    }
     wave_fold(s, c);
-    if (lane == 0) partial[it] = make_float2(s, c);
+    if (lane == 0) partial[R.slot] = make_float2(s, c);
+    // the item's slice elements, in z order: source (2a, 2b, 2c) of z (a, b, c)
+    float *__restrict__ za = z + R.zoff;
+    for (uint32_t zi = R.zlo + (uint32_t)lane; zi < R.zhi; zi += 64) {
+      const uint32_t t = fdiv(zi, R.mz_c, R.lz_c);
+      const uint32_t cc = zi - t * R.zshp[2];
+      const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
+      const uint32_t bb = t - aa * R.zshp[1];
+      za[zi] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
+    }
     if (dynamic) {
       int32_t t = 0;
       if (lane == 0) t = atomicAdd(counter, 1);
@@ -211,47 +239,71 @@ __global__ __launch_bounds__(256) void k_one_slice(const float *__restrict__ x, 
 }
 
 // ---------------------------------------------------------------------------
+// Slice elements of an array of shape shp whose source index is < e:
+// sources are row-major indices of (2a, 2b, 2c), increasing with the slice
+// index, so the slice elements sourced from [lo, hi) are a contiguous range.
+int64_t slice_before(const BArray &A, int64_t e) {
+  const int64_t S2 = A.shp[2], S1 = A.shp[1];
+  const int64_t Z2 = A.zshp[2], Z1 = A.zshp[1], Z0 = A.zshp[0];
+  if (e >= A.n) return Z0 * Z1 * Z2;
+  const int64_t k = e % S2, t = e / S2, j = t % S1, i = t / S1;
+  int64_t cnt = ((i + 1) / 2) * Z1 * Z2;  // whole slice planes of even i' < i
+  if (i % 2 == 0) {
+    cnt += ((j + 1) / 2) * Z2;                // whole slice rows of even j' < j
+    if (j % 2 == 0) cnt += (k + 1) / 2;       // even k' < k
+  }
+  return cnt;
+}
+
 int build_items(Batch *B, int64_t chunk) {
-  std::vector<BItem> items;
+  std::vector<BRec> items;
   std::vector<int32_t> first((size_t)B->n + 1);
-  std::vector<int64_t> cost;
   for (int k = 0; k < B->n; ++k) {
     first[(size_t)k] = (int32_t)items.size();
-    const int64_t n = B->arrays[(size_t)k].n;
-    for (int64_t lo = 0; lo < n; lo += chunk) {
-      BItem it;
-      it.arr = k;
-      it.pad_ = 0;
-      it.lo = lo;
-      it.hi = std::min(n, lo + chunk);
-      items.push_back(it);
-      cost.push_back(it.hi - it.lo);
+    const BArray &A = B->arrays[(size_t)k];
+    BRec r;
+    std::memset(&r, 0, sizeof(r));
+    r.xoff = A.xoff;
+    r.zoff = A.zoff;
+    for (int i = 0; i < 3; ++i) {
+      r.shp[i] = A.shp[i];
+      r.zshp[i] = A.zshp[i];
+    }
+    magic_u32(A.zshp[2], &r.mz_c, &r.lz_c);
+    magic_u32(A.zshp[1], &r.mz_b, &r.lz_b);
+    r.scalar = A.scalar;
+    for (int64_t lo = 0; lo < A.n; lo += chunk) {
+      r.lo = (uint32_t)lo;
+      r.hi = (uint32_t)std::min(A.n, lo + chunk);
+      r.zlo = (uint32_t)slice_before(A, lo);
+      r.zhi = (uint32_t)slice_before(A, r.hi);
+      r.slot = (int32_t)items.size();
+      items.push_back(r);
     }
   }
   first[(size_t)B->n] = (int32_t)items.size();
-  std::vector<int32_t> order(items.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[(size_t)a] > cost[(size_t)b]; });
   const int64_t ni = (int64_t)items.size();
   if (ni >= INT32_MAX) return ph::set_error(PH_E_ARGUMENT, "batch: too many work items (%lld)", (long long)ni);
+  // dispatch order: largest first (ties in array order)
+  std::stable_sort(items.begin(), items.end(),
+                   [](const BRec &a, const BRec &b) { return a.hi - a.lo > b.hi - b.lo; });
   if (ni > B->items_cap) {
-    if (B->d_items) {
+    if (B->d_recs) {
       PH_HIP(hipStreamSynchronize(ph::stream()));
-      PH_HIP(hipFree(B->d_items));
-      PH_HIP(hipFree(B->d_order));
+      PH_HIP(hipFree(B->d_recs));
       PH_HIP(hipFree(B->d_partial));
+      B->d_recs = nullptr;
+      B->d_partial = nullptr;
+      B->items_cap = 0;
     }
-    PH_HIP(hipMalloc(&B->d_items, sizeof(BItem) * (size_t)ni));
-    PH_HIP(hipMalloc(&B->d_order, sizeof(int32_t) * (size_t)ni));
+    PH_HIP(hipMalloc(&B->d_recs, sizeof(BRec) * (size_t)ni));
     PH_HIP(hipMalloc(&B->d_partial, sizeof(float2) * (size_t)ni));
     B->items_cap = ni;
   }
   PH_HIP(hipStreamSynchronize(ph::stream()));
-  if (ni) {
-    PH_HIP(hipMemcpy(B->d_items, items.data(), sizeof(BItem) * (size_t)ni, hipMemcpyHostToDevice));
-    PH_HIP(hipMemcpy(B->d_order, order.data(), sizeof(int32_t) * (size_t)ni, hipMemcpyHostToDevice));
-  }
+  if (ni) PH_HIP(hipMemcpy(B->d_recs, items.data(), sizeof(BRec) * (size_t)ni, hipMemcpyHostToDevice));
   PH_HIP(hipMemcpy(B->d_first, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice));
+  PH_HIP(hipMemset(B->d_counter, 0, 64));
   B->nitems = ni;
   B->chunk = chunk;
   return PH_OK;
@@ -260,10 +312,9 @@ int build_items(Batch *B, int64_t chunk) {
 int launch_batch(Batch *B) {
   if (B->n == 0) return PH_OK;  // nothing to launch
   hipStream_t st = ph::stream();
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, B->d_counter);
-  hipLaunchKernelGGL(k_batch, dim3((unsigned)B->grid), dim3(256), 0, st, (const BArray *)B->d_arrays,
-                     (const BItem *)B->d_items, (const int32_t *)B->d_order, (int32_t)B->nitems, B->d_counter,
-                     (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
+  if (B->dynamic) hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, B->d_counter);
+  hipLaunchKernelGGL(k_batch, dim3((unsigned)B->grid), dim3(256), 0, st, (const BRec *)B->d_recs,
+                     (int32_t)B->nitems, B->d_counter, (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
   hipLaunchKernelGGL(k_batch_final, dim3((unsigned)((B->n + 255) / 256)), dim3(256), 0, st,
                      (const int32_t *)B->d_first, B->n, (const float2 *)B->d_partial, B->sums);
   return ph::check_launch("k_batch");
@@ -335,6 +386,8 @@ int ph_batch_create(void **handle, int32_t n, const int32_t *ranks, const int64_
   PH_REQUIRE_INIT();
   if (!handle || n < 0 || (n > 0 && (!ranks || !shapes || !scalars)))
     return ph::set_error(PH_E_ARGUMENT, "ph_batch_create arguments");
+  if (((uintptr_t)x | (uintptr_t)y) % 16 != 0)
+    return ph::set_error(PH_E_ARGUMENT, "batch: x and y must be 16-byte aligned");
   Batch *B = new Batch();
   B->n = n;
   B->x = x;
@@ -376,8 +429,6 @@ int ph_batch_create(void **handle, int32_t n, const int32_t *ranks, const int64_
       a.shp[i] = (uint32_t)s3[i];
       a.zshp[i] = (uint32_t)((s3[i] + 1) / 2);
     }
-    magic_u32(a.shp[2] ? a.shp[2] : 1, &a.m_c, &a.l_c);
-    magic_u32(a.shp[1] ? a.shp[1] : 1, &a.m_b, &a.l_b);
     a.xoff = off;
     a.zoff = zoff;
     a.scalar = scalars[k];
@@ -387,13 +438,11 @@ int ph_batch_create(void **handle, int32_t n, const int32_t *ranks, const int64_
   }
   B->total = off;
   B->ztotal = zoff;
-  if (hipMalloc(&B->d_arrays, sizeof(BArray) * (size_t)std::max(n, 1)) != hipSuccess ||
-      hipMalloc(&B->d_first, sizeof(int32_t) * (size_t)(n + 1)) != hipSuccess ||
+  if (hipMalloc(&B->d_first, sizeof(int32_t) * (size_t)(n + 1)) != hipSuccess ||
       hipMalloc(&B->d_counter, 64) != hipSuccess) {
-    delete B;
+    ph_batch_destroy(B);
     return ph::set_error(PH_E_OOM, "batch tables");
   }
-  if (n) PH_HIP(hipMemcpy(B->d_arrays, B->arrays.data(), sizeof(BArray) * (size_t)n, hipMemcpyHostToDevice));
   B->grid = cus() * 4;
   int rc = build_items(B, 16384);
   if (rc) {
@@ -424,6 +473,8 @@ int ph_batch_bind(void *handle, const float *x, float *y, float *z, double *sums
   PH_GUARD();
   Batch *B = (Batch *)handle;
   if (!B) return ph::set_error(PH_E_ARGUMENT, "null batch");
+  if (((uintptr_t)x | (uintptr_t)y) % 16 != 0)
+    return ph::set_error(PH_E_ARGUMENT, "batch: x and y must be 16-byte aligned");
   B->x = x;
   B->y = y;
   B->z = z;
@@ -497,7 +548,7 @@ int ph_batch_tune(void *handle, int64_t *chunk_out, int32_t *grid_out, int32_t *
     PH_HIP(hipEventCreate(&e0));
     PH_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
-    for (int64_t chunk : {2048, 8192, 32768, 131072}) {
+    for (int64_t chunk : {1024, 2048, 4096, 8192, 32768}) {
       int rc = build_items(B, chunk);
       if (rc) return rc;
       for (int dyn : {1, 0}) {
@@ -548,7 +599,7 @@ int ph_batch_schedule(void *handle, int64_t chunk, int32_t grid, int32_t dynamic
   PH_REQUIRE_INIT();
   Batch *B = (Batch *)handle;
   if (!B) return ph::set_error(PH_E_ARGUMENT, "null batch");
-  if (chunk < 64 || chunk > (int64_t(1) << 31) || grid < 1 || grid > (1 << 20))
+  if (chunk < 64 || chunk % 64 != 0 || chunk > (int64_t(1) << 30) || grid < 1 || grid > (1 << 20))
     return ph::set_error(PH_E_ARGUMENT, "batch schedule: chunk %lld, grid %d", (long long)chunk, grid);
   int rc = build_items(B, chunk);
   if (rc) return rc;
@@ -570,9 +621,7 @@ int ph_batch_destroy(void *handle) {
   (void)hipStreamSynchronize(ph::stream());
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
-  (void)hipFree(B->d_arrays);
-  (void)hipFree(B->d_items);
-  (void)hipFree(B->d_order);
+  (void)hipFree(B->d_recs);
   (void)hipFree(B->d_first);
   (void)hipFree(B->d_partial);
   (void)hipFree(B->d_counter);
