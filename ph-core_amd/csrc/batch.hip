@@ -6,9 +6,9 @@
 // in one pass over x: a persistent kernel whose wavefronts run work items
 // (an array, or a chunk of a large one) from a device-side table.  A
 // wavefront streams its item's elements once (16-byte loads of x, 16-byte
-// non-temporal stores of y, FP32 Neumaier sums), then gathers the item's
-// slice elements from the lines it has just read (L2 hits) and writes them
-// to z contiguously.  Per-item partial sums land in fixed slots and a second
+// non-temporal stores of y, FP32 Neumaier sums), 1024 elements at a time,
+// gathering after each step the slice elements sourced from it out of the
+// lines it has just read (L2 hits) and writing them to z contiguously.  Per-item partial sums land in fixed slots and a second
 // kernel folds each array's slots in item order, so the sums do not depend
 // on which wavefront ran which item.  The kernels are captured once in a HIP
 // graph and replayed.
@@ -43,11 +43,12 @@ struct BRec {
   uint32_t lo, hi;                  // element range [lo, hi) of the array
   uint32_t zlo, zhi;                // slice elements whose source lies in [lo, hi)
   uint32_t shp[3], zshp[3];
+  uint32_t m_c, l_c, m_b, l_b;      // magic divisors for shp[2], shp[1]
   uint32_t mz_c, lz_c, mz_b, lz_b;  // magic divisors for zshp[2], zshp[1]
   float scalar;
   int32_t slot;                     // partial-sum slot (array-major item index)
 };
-static_assert(sizeof(BRec) == 80, "BRec layout");
+static_assert(sizeof(BRec) == 96, "BRec layout");
 
 struct Batch {
   int n = 0;
@@ -117,13 +118,35 @@ __device__ __forceinline__ void neu_add4(float &s, float &c, f4 v) {
   neu_add(s, c, v.w);
 }
 
+// Slice elements whose source index is < e (e < the array's size): the
+// sources (2a, 2b, 2c) increase with the slice index, so the slice elements
+// sourced from a range of x form a contiguous range of z.
+__device__ __forceinline__ uint32_t slice_before(const BRec &R, uint32_t e) {
+  const uint32_t t = fdiv(e, R.m_c, R.l_c);
+  const uint32_t k = e - t * R.shp[2];
+  const uint32_t i = fdiv(t, R.m_b, R.l_b);
+  const uint32_t j = t - i * R.shp[1];
+  uint32_t cnt = ((i + 1) >> 1) * R.zshp[1] * R.zshp[2];  // slice planes of even i' < i
+  if ((i & 1u) == 0) {
+    cnt += ((j + 1) >> 1) * R.zshp[2];                   // slice rows of even j' < j
+    if ((j & 1u) == 0) cnt += (k + 1) >> 1;               // even k' < k
+  }
+  return cnt;
+}
+
+// Elements a wavefront streams before gathering their slice elements: the
+// lines just read are still in L2 when they are gathered.
+constexpr uint32_t kSub = 1024;
+
 // Persistent: each wavefront runs work items until none are left.  dynamic:
 // items come from a queue (one atomic per item, lane 0); static: wavefront w
 // of W takes dispatch positions w, w + W, w + 2W, ... of the largest-first
-// order, which needs no atomics.  Lane l handles elements lo + 4l + 256m
-// (m = 0, 1, ...) as 16-byte vectors: every item starts at a multiple of 4
-// and every array is padded to 4 elements in the packed layout, so a vector
-// never leaves x; a partial last vector is stored element by element.
+// order, which needs no atomics.  An item is processed kSub elements at a
+// time: lane l takes elements base + 4l + 256m (m = 0..3) as 16-byte
+// vectors (items start at multiples of 4 and every array is padded to 4
+// elements in the packed layout, so a vector never leaves x; a partial last
+// vector is stored element by element), then the wavefront gathers the
+// slice elements sourced from [base, base + kSub) and writes them to z.
 __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, int32_t nitems, int32_t *counter,
                                                int32_t dynamic, const float *__restrict__ x,
                                                float *__restrict__ y, float *__restrict__ z,
@@ -140,47 +163,51 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
     const BRec R = recs[q];
     const float *__restrict__ xa = x + R.xoff;
     float *__restrict__ ya = y + R.xoff;
+    float *__restrict__ za = z + R.zoff;
     float s = 0.f, c = 0.f;
-    uint32_t e = R.lo + 4u * (uint32_t)lane;
-    for (; e + 260 <= R.hi; e += 512) {  // two full vectors per lane in flight
-      const f4 v0 = *(const f4 *)(xa + e);
-      const f4 v1 = *(const f4 *)(xa + e + 256);
-      __builtin_nontemporal_store(v0 + R.scalar, (f4 *)(ya + e));
-      __builtin_nontemporal_store(v1 + R.scalar, (f4 *)(ya + e + 256));
-      neu_add4(s, c, v0);
-      neu_add4(s, c, v1);
-    }
-    for (; e < R.hi; e += 256) {
-      const f4 v = *(const f4 *)(xa + e);
-      const f4 r = v + R.scalar;
-      if (e + 4 <= R.hi) {
-        __builtin_nontemporal_store(r, (f4 *)(ya + e));
-        neu_add4(s, c, v);
-      } else {
-        const uint32_t cnt = R.hi - e;  // 1..3
-        ya[e] = r.x;
-        neu_add(s, c, v.x);
-        if (cnt > 1) {
-          ya[e + 1] = r.y;
-          neu_add(s, c, v.y);
-        }
-        if (cnt > 2) {
-          ya[e + 2] = r.z;
-          neu_add(s, c, v.z);
+    uint32_t zlo = R.zlo;
+    for (uint32_t base = R.lo; base < R.hi; base += kSub) {
+      const uint32_t end = min(R.hi, base + kSub);
+      f4 v[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
+        if (e < end) v[m] = *(const f4 *)(xa + e);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
+        const f4 r = v[m] + R.scalar;
+        if (e + 4 <= end) {
+          __builtin_nontemporal_store(r, (f4 *)(ya + e));
+          neu_add4(s, c, v[m]);
+        } else if (e < end) {
+          const uint32_t cnt = end - e;  // 1..3
+          ya[e] = r.x;
+          neu_add(s, c, v[m].x);
+          if (cnt > 1) {
+            ya[e + 1] = r.y;
+            neu_add(s, c, v[m].y);
+          }
+          if (cnt > 2) {
+            ya[e + 2] = r.z;
+            neu_add(s, c, v[m].z);
+          }
         }
       }
+      // slice elements in z order: source (2a, 2b, 2c) of z (a, b, c)
+      const uint32_t zhi = end == R.hi ? R.zhi : slice_before(R, end);
+      for (uint32_t zi = zlo + (uint32_t)lane; zi < zhi; zi += 64) {
+        const uint32_t t = fdiv(zi, R.mz_c, R.lz_c);
+        const uint32_t cc = zi - t * R.zshp[2];
+        const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
+        const uint32_t bb = t - aa * R.zshp[1];
+        za[zi] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
+      }
+      zlo = zhi;
     }
     wave_fold(s, c);
     if (lane == 0) partial[R.slot] = make_float2(s, c);
-    // the item's slice elements, in z order: source (2a, 2b, 2c) of z (a, b, c)
-    float *__restrict__ za = z + R.zoff;
-    for (uint32_t zi = R.zlo + (uint32_t)lane; zi < R.zhi; zi += 64) {
-      const uint32_t t = fdiv(zi, R.mz_c, R.lz_c);
-      const uint32_t cc = zi - t * R.zshp[2];
-      const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
-      const uint32_t bb = t - aa * R.zshp[1];
-      za[zi] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
-    }
     if (dynamic) {
       int32_t t = 0;
       if (lane == 0) t = atomicAdd(counter, 1);
@@ -269,6 +296,8 @@ int build_items(Batch *B, int64_t chunk) {
       r.shp[i] = A.shp[i];
       r.zshp[i] = A.zshp[i];
     }
+    magic_u32(A.shp[2], &r.m_c, &r.l_c);
+    magic_u32(A.shp[1], &r.m_b, &r.l_b);
     magic_u32(A.zshp[2], &r.mz_c, &r.lz_c);
     magic_u32(A.zshp[1], &r.mz_b, &r.lz_b);
     r.scalar = A.scalar;
