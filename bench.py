@@ -251,6 +251,8 @@ def other_workloads(phase, ev, args):
                   "note": "a.map_with(b.tile, c) { (x + y) * z }: one kernel, stride-0 broadcast"},
         "bytes": by}
     del a, b, c, bt
+    tr = load_traffic() or {}
+    out["c1_broadcast_f64_16384"]["fused"]["traffic"] = tr.get("ph_map_fused_c1")
 
     # C3 fused ProcView + six reductions (F32 and BF16 storage), one pass
     n = 16384
@@ -263,6 +265,7 @@ def other_workloads(phase, ev, args):
         out[f"c3_procview_reduce_{st.lower()}_16384"] = {
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
             "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
+            "traffic": tr.get(f"ph_red_tile_{st.lower()}"),
             "note": "one pass: sum and max in full, along slices(0) and slices(1), FP32 compensated"}
         del x, pv
 
@@ -298,7 +301,9 @@ def other_workloads(phase, ev, args):
     out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
                            "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
                            "rel_l2_f32_vs_f64": rel,
-                           "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0}
+                           "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0,
+                           "traffic_per_launch_of_two_steps": {"f64": tr.get("k_heat_step2_f64"),
+                                                               "f32": tr.get("k_heat_step2_f32")}}
     del fields, u
     out["c5_batched_20000"] = batched_workload(phase, ev, args)
     return out
@@ -340,6 +345,7 @@ def batched_workload(phase, ev, args):
         "op_bytes": op_bytes, "op_gbs": round(op_bytes / ms_graph / 1e6, 1),
         "pass_bytes": pass_bytes, "pass_gbs": round(pass_bytes / ms_graph / 1e6, 1),
         "pass_frac": round(pass_bytes / ms_graph / 1e6 / PEAK_HBM_GBS, 4),
+        "traffic": (load_traffic() or {}).get("k_batch"),
         "note": "op_bytes counts each op's own traffic (12 B/element + 8 B/slice element); the fused pass "
                 "moves pass_bytes (8 B/element + 4 B/slice element), whose rate is the roofline figure",
     }
