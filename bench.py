@@ -65,9 +65,30 @@ class Events:
         assert self.lib.ph_sync() == 0
 
 
-def time_loop(ev, fn, reps):
+_BUSY = {}
+
+
+def busy(phase):
+    """Queues a few milliseconds of read-only device work (one-pass
+    reductions of a 256 MiB array; they write no more than a few hundred KB)
+    so that the launches queued after it start back to back: the events
+    around them then time the kernels, not the host's launch gaps."""
+    D = phase.DType
+    if "pv" not in _BUSY:
+        _BUSY["buf"] = phase.NArray.random([8192, 8192], D.F32, seed=100)
+        _BUSY["pv"] = _BUSY["buf"].view()
+    for _ in range(24):
+        _BUSY["pv"].reduce2d_all_device()
+
+
+def time_loop(ev, fn, reps, phase=None):
+    """Average device time of `fn` over `reps` back-to-back calls (HIP events
+    on the library stream).  With `phase`, device work is queued first so the
+    host's launch overhead does not open gaps between the calls."""
     a, b = ev.new(), ev.new()
     ev.sync()
+    if phase is not None:
+        busy(phase)
     ev.record(a)
     for _ in range(reps):
         fn()
@@ -155,18 +176,19 @@ def main():
 
     # per-kernel durations (HIP events around back-to-back launches)
     g = view.to_narr()
-    ms_gather = time_loop(ev, lambda: view.to_narr(), 20)
+    ms_gather = time_loop(ev, lambda: view.to_narr(), 20, phase)
 
-    zs = [phase.NArray.fill([n, n], 0) for _ in range(4)]
-    ms_fs = 0.0
-    for z in zs:  # each fill+scatter needs a fresh pending fill
-        aa, bb = ev.new(), ev.new()
-        ev.sync()
-        ev.record(aa)
-        z.mutable_view(*lits)[[]] = g
-        ev.record(bb)
-        ms_fs += ev.ms(aa, bb)
-    ms_fs /= len(zs)
+    # fill + scatter: 8 back-to-back launches, each into a fresh pending fill
+    zs = [phase.NArray.fill([n, n], 0) for _ in range(8)]
+    mvs = [z.mutable_view(*lits) for z in zs]
+    aa, bb = ev.new(), ev.new()
+    ev.sync()
+    busy(phase)
+    ev.record(aa)
+    for mv in mvs:
+        mv[[]] = g
+    ev.record(bb)
+    ms_fs = ev.ms(aa, bb) / len(zs)
     achieved_fs = scatter_bytes / (ms_fs * 1e-3) / 1e9
     achieved_g = gather_bytes / (ms_gather * 1e-3) / 1e9
 
@@ -180,13 +202,13 @@ def main():
                         "frac": round(achieved_g / PEAK_HBM_GBS, 4), "ms_per_launch": round(ms_gather, 4),
                         "algorithmic_bytes": gather_bytes,
                         "traffic": traffic.get("k_copy_rows") if traffic else None}}
-    del zs, g
+    del zs, mvs, g
 
     # plain contiguous copy of 256 MiB (Int32): the practical copy ceiling of
     # the device, and the known-size kernel that scales the PMC byte counters
     cx = phase.NArray.random([1 << 26], D.I32, seed=9, dist="int31")
     cx.clone()
-    ms_copy = time_loop(ev, lambda: cx.clone(), 10)
+    ms_copy = time_loop(ev, lambda: cx.clone(), 10, phase)
     extra["copy_256mib"] = {"kernel": "k_copy_rows (contiguous)", "bytes": 2 << 28, "ms_per_launch": round(ms_copy, 4),
                             "achieved_gbs": round((2 << 28) / ms_copy / 1e6, 1)}
     del cx
@@ -238,11 +260,11 @@ def other_workloads(phase, ev, args):
     # eager, as ph-core evaluates it: two operator kernels and a temporary
     fn = lambda: (a + bt) * c  # noqa: E731
     fn()
-    ms = time_loop(ev, fn, 10)
+    ms = time_loop(ev, fn, 10, phase)
     # fused: one map_with kernel, same per-element operations and roundings
     fz = lambda: a.map_with(bt, c, block=lambda x, y, z: (x + y) * z)  # noqa: E731
     fz()
-    msf = time_loop(ev, fz, 10)
+    msf = time_loop(ev, fz, 10, phase)
     out["c1_broadcast_f64_16384"] = {
         "eager": {"ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
                   "note": "(a + b.tile) * c as two elementwise kernels (the reference's eager operators)"},
@@ -260,7 +282,7 @@ def other_workloads(phase, ev, args):
         x = phase.NArray.random([n, n], getattr(D, st), seed=2, dist="normal", a=0.0, b=1.0)
         pv = x.process(lambda v: (v + 4) ** 2 * 0.5 - v)
         pv.reduce2d_all_device()
-        ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10)
+        ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10, phase)
         by = esz * n * n
         out[f"c3_procview_reduce_{st.lower()}_16384"] = {
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
@@ -272,15 +294,8 @@ def other_workloads(phase, ev, args):
     # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64
     n = 8192
     steps = args.heat_steps
-    rng = np.random.default_rng(3)
-    cy, cx = rng.uniform(0.1, 0.9, 16) * n, rng.uniform(0.1, 0.9, 16) * n
-    sig = rng.uniform(0.01, 0.05, 16) * n
-    amp = rng.uniform(0.5, 2.0, 16)
-    yy = np.arange(n, dtype=np.float64)[:, None]
-    xx = np.arange(n, dtype=np.float64)[None, :]
-    u0 = np.zeros((n, n))
-    for k in range(16):
-        u0 += amp[k] * np.exp(-(((yy - cy[k]) ** 2) + ((xx - cx[k]) ** 2)) / (2 * sig[k] ** 2))
+    from phase.workloads import heat_initial_field
+    u0 = heat_initial_field(n, seed=3)
     res = {}
     fields = {}
     for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
@@ -325,8 +340,8 @@ def batched_workload(phase, ev, args):
     b.fill_uniform(4)
     chunk, grid, dyn = b.tune()
     b.run(0)
-    ms_graph = time_loop(ev, lambda: b.run(0), 20)
-    ms_direct = time_loop(ev, lambda: b.run(1), 20)
+    ms_graph = time_loop(ev, lambda: b.run(0), 20, phase)
+    ms_direct = time_loop(ev, lambda: b.run(1), 20, phase)
     ms_unbatched = wall_unbatched = None
     if not args.no_unbatched:
         b.run(2)

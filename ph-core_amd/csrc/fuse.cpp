@@ -493,10 +493,10 @@ int materialize(const ph_view_desc *d, const void *src, int slot, ph_view_desc *
 // Reductions
 // ---------------------------------------------------------------------------
 const char *kRedPrelude = R"PH(
-struct SF { float s, c, m; };
+struct SF { float s, c; double m; };  // the max is kept exactly, whatever the sum's precision
 struct SD { double s, c, m; };
 struct SI { i64 s, hi, lo; int m; int n; };
-__device__ __forceinline__ void st_init(SF &a) { a.s = 0.f; a.c = 0.f; a.m = -__builtin_inff(); }
+__device__ __forceinline__ void st_init(SF &a) { a.s = 0.f; a.c = 0.f; a.m = -__builtin_inf(); }
 __device__ __forceinline__ void st_init(SD &a) { a.s = 0.0; a.c = 0.0; a.m = -__builtin_inf(); }
 __device__ __forceinline__ void st_init(SI &a) { a.s = 0; a.hi = -(1LL << 62); a.lo = (1LL << 62); a.m = 0; a.n = 0; }
 template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, F v) {
@@ -603,6 +603,13 @@ struct RedArgs {
 };
 )PH";
 
+// Host mirrors of the kernels' reduction states (kRedPrelude), for sizing
+// the scratch that holds them.
+struct HostSF { float s, c; double m; };
+struct HostSD { double s, c, m; };
+struct HostSI { int64_t s, hi, lo; int32_t m, n; };
+static_assert(sizeof(HostSF) == 16 && sizeof(HostSD) == 24 && sizeof(HostSI) == 32, "state layouts");
+
 struct RedArgs {
   const void *in[PH_MAX_INPUTS];
   int64_t in_off[PH_MAX_INPUTS];
@@ -638,6 +645,8 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   const int V = P.vec;
   const bool sd = std::string(P.state) == "SD";
   const char *F = sd ? "double" : "float";
+  // maxima are taken in the value's own type (exact), sums in F
+  const char *M = (sd || p->out_dtype == PH_F64) ? "double" : "float";
   const bool full_cols = P.cols % (256 * V) == 0;
   const bool full_rows = P.rows % P.tr == 0;
   const std::string C = lit_i64(P.cols);
@@ -645,7 +654,7 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   const std::string rcheck = full_rows ? "true" : "(r < r1)";
   std::string s;
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
-  s += fmt("  typedef %s S; typedef %s F;\n", P.state, F);
+  s += fmt("  typedef %s S; typedef %s F; typedef %s M;\n", P.state, F, M);
   s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
            lit_i64(P.ntc).c_str());
   s += "  const int lane = threadIdx.x & 63; const int wave = threadIdx.x >> 6;\n";
@@ -655,9 +664,9 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
            lit_i64(P.rows).c_str());
   s += "  int fl = 0;\n";
   s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
-  if (P.do_rows) s += fmt("  __shared__ F rsum_sh[%d][4]; __shared__ F rmax_sh[%d][4];\n", P.tr, P.tr);
+  if (P.do_rows) s += fmt("  __shared__ F rsum_sh[%d][4]; __shared__ M rmax_sh[%d][4];\n", P.tr, P.tr);
   s += "  for (i64 g0 = r0; g0 < r1; g0 += 8) {\n";
-  if (P.do_rows) s += "    F rsv[8], rmv[8];\n";
+  if (P.do_rows) s += "    F rsv[8]; M rmv[8];\n";
   s += "#pragma unroll\n    for (int q = 0; q < 8; q += 4) {\n";
   for (int j = 0; j < n_in; ++j) s += fmt("      %s x%d[4][%d];\n", ctype(compute_dt(in_dt[j])), j, V);
   // loads of 4 rows
@@ -691,25 +700,25 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[k][v]", j));
   std::string body;
   std::string res = emit_prog(p, inexpr, &body, "");
-  s += fmt("      F cp[%d], cm[%d];\n#pragma unroll\n      for (int v = 0; v < %d; ++v) { cp[v] = (F)0; cm[v] = -__builtin_inf(); }\n",
+  s += fmt("      F cp[%d]; M cm[%d];\n#pragma unroll\n      for (int v = 0; v < %d; ++v) { cp[v] = (F)0; cm[v] = -__builtin_inf(); }\n",
            V, V, V);
   s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
-  s += "        F ls = (F)0; F lm = -__builtin_inf();\n";
+  s += "        F ls = (F)0; M lm = -__builtin_inf();\n";
   s += "        if (" + rcheck + ") {\n";
   s += fmt("#pragma unroll\n          for (int v = 0; v < %d; ++v) {\n            const i64 c = col0 + v;\n", V);
   s += "            if (" + ccheck("c") + ") {\n";
   s += body;
-  s += fmt("              const F val = (F)(%s);\n", res.c_str());
-  if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], val);\n";
-  if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, val);\n";
+  s += fmt("              const F val = (F)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
+  if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], mval);\n";
+  if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, mval);\n";
   s += "            }\n          }\n        }\n";
   if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
   s += "      }\n";
   if (P.do_cols)
-    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = ph_fmax(cst[v].m, cm[v]); }\n", V);
+    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = ph_fmax(cst[v].m, (double)cm[v]); }\n", V);
   s += "    }\n";
   if (P.do_rows) {
-    s += "    const F fs = tfold8<F, false>(rsv);\n    const F fm = tfold8<F, true>(rmv);\n";
+    s += "    const F fs = tfold8<F, false>(rsv);\n    const M fm = tfold8<M, true>(rmv);\n";
     s += "    const int rr = (lane >> 3) & 7;\n";
     s += "    if ((lane & 7) == 0 && g0 + rr < r1) { rsum_sh[g0 - r0 + rr][wave] = fs; rmax_sh[g0 - r0 + rr][wave] = fm; }\n";
   }
@@ -856,11 +865,11 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   P.do_rows = row_sum || row_max || P.do_full;
   const int odt = prog->out_dtype;
   if (odt == PH_I32 || odt == PH_BOOL) {
-    P.state = "SI"; P.state_bytes = 40; P.acc_t = "i64";
+    P.state = "SI"; P.state_bytes = sizeof(HostSI); P.acc_t = "i64";
   } else if (acc == PH_ACC_F64) {
-    P.state = "SD"; P.state_bytes = 24; P.acc_t = "double";
+    P.state = "SD"; P.state_bytes = sizeof(HostSD); P.acc_t = "double";
   } else {
-    P.state = "SF"; P.state_bytes = 12; P.acc_t = "float";
+    P.state = "SF"; P.state_bytes = sizeof(HostSF); P.acc_t = "float";
   }
   for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = false;
   for (int j = 0; j < n_in; ++j) {
@@ -1018,9 +1027,9 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   P.do_cols = P.do_rows = P.do_full = true;
   const int odt = prog->out_dtype;
   if (odt == PH_I32 || odt == PH_BOOL) {
-    P.state = "SI"; P.state_bytes = 40; P.acc_t = "i64";
+    P.state = "SI"; P.state_bytes = sizeof(HostSI); P.acc_t = "i64";
   } else {
-    P.state = "SF"; P.state_bytes = 12; P.acc_t = "float";
+    P.state = "SF"; P.state_bytes = sizeof(HostSF); P.acc_t = "float";
   }
   for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = j < n_in && cols % 4 == 0;
   P.ntc = (cols + 1023) / 1024;

@@ -411,6 +411,15 @@ PYBIND11_MODULE(_phase, m) {
                   py::arg("shape"), py::arg("dtype"), py::arg("seed"), py::arg("offset") = 0,
                   py::arg("dist") = "uniform", py::arg("a") = 0.0, py::arg("b") = 1.0)
       .def_static("empty", &NArray::empty)
+      .def("fill_random",
+           [](NArray &self, uint64_t seed, const std::string &dist, double a, double b) {
+             int k = dist == "uniform" ? 0 : dist == "normal" ? 1 : dist == "int31" ? 2 : -1;
+             if (k < 0) throw ArgumentError("dist must be uniform, normal or int31");
+             check(ph_fill_random(self.device_ptr(), (int32_t)self.dtype(), self.size(), seed, 0, k, a, b),
+                   "random fill");
+           },
+           py::arg("seed"), py::arg("dist") = "uniform", py::arg("a") = 0.0, py::arg("b") = 1.0,
+           "overwrites every element with draws of stream `seed` (as NArray.random)")
       .def_property_readonly("shape", &NArray::shape)
       .def_property_readonly("size", &NArray::size)
       .def_property_readonly("dimensions", &NArray::dimensions)

@@ -1,4 +1,9 @@
-"""Synthetic workload shapes shared by the benchmark and the tests.
+"""Synthetic workload inputs shared by the benchmark and the tests.
+
+Heat-equation initial field: 16 Gaussian bumps on an n x n grid, centres
+U[0.1, 0.9) * n per axis, widths sigma U[0.01, 0.05) * n, amplitudes
+U[0.5, 2.0), drawn in that order from numpy's PCG64 seeded with `seed`;
+evaluated in Float64.
 
 Batched uneven arrays: `n` Float32 arrays, rank uniform in {1, 2, 3}, each
 side log-uniform in [8, 512] (floor of exp(U[ln 8, ln 513))), from numpy's
@@ -49,3 +54,17 @@ def batch_stats(shapes):
         "max_elements": int(max(sizes)) if sizes else 0,
         "log2_histogram": {f"<=2^{b}": hist[b] for b in sorted(hist)},
     }
+
+
+def heat_initial_field(n=8192, seed=3, bumps=16):
+    """u0[i, j] = sum_k A_k exp(-((i - cy_k)^2 + (j - cx_k)^2) / (2 sigma_k^2)), Float64."""
+    rng = np.random.default_rng(seed)
+    cy, cx = rng.uniform(0.1, 0.9, bumps) * n, rng.uniform(0.1, 0.9, bumps) * n
+    sig = rng.uniform(0.01, 0.05, bumps) * n
+    amp = rng.uniform(0.5, 2.0, bumps)
+    yy = np.arange(n, dtype=np.float64)[:, None]
+    xx = np.arange(n, dtype=np.float64)[None, :]
+    u0 = np.zeros((n, n))
+    for k in range(bumps):
+        u0 += amp[k] * np.exp(-(((yy - cy[k]) ** 2) + ((xx - cx[k]) ** 2)) / (2 * sig[k] ** 2))
+    return u0
