@@ -24,6 +24,10 @@
 //     float64 arithmetic computes the same closed form);
 //   sums                         — exact sums of integer-valued data, Python
 //     math.fsum on small inputs;
+//   Float32 sequential sum       — numpy's sequential float32 cumsum and
+//     hand-computed absorption cases;
+//   batched loop                 — numpy float32 add, basic slicing
+//     x[::2, ::2, ::2] and float32 cumsum, array by array;
 //   heat 2-D                     — separable eigenmode decay (closed form);
 //   heat 1-D example             — steady-state / symmetry invariants.
 // Crystal-stdlib semantics that no reference test covers (S1-S4 below) are
