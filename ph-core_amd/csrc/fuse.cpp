@@ -812,37 +812,56 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   }
   s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
   }
-  // --- finish: per-row and per-column results
+  // --- finish.  Column blocks: 64 columns x 4 parts; part p folds row tiles
+  // [p*ntr/4, (p+1)*ntr/4) in order, then the 4 parts are folded in order.
+  // Row blocks: one row per thread (its column tiles in order), then the
+  // block's 256 rows are folded in order (a pairwise tree of adjacent
+  // ranges) into one state per block for the full result.
+  const int64_t ncb = P.do_cols ? (P.cols + 63) / 64 : 0;
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_finish(RedArgs A) {\n";
   s += fmt("  typedef %s S;\n", P.state);
-  s += "  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;\n  int fl = 0;\n";
-  if (P.do_rows) {
-    s += fmt("  if (i < %s) {\n    S a = ((const S *)A.rowpart)[i];\n", lit_i64(P.rows).c_str());
-    s += fmt("#pragma unroll 8\n    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
-             lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
-    s += "    if (A.row_sum) A.row_sum[i] = st_sum(a);\n    if (A.row_max) A.row_max[i] = st_max(a);\n";
-    s += "    if (A.row_sum) fl |= st_ovf(a);\n";
-    s += "    ((S *)A.rowfinal)[i] = a;\n  }\n";
-  }
+  s += "  __shared__ S sh[256];\n  int fl = 0;\n";
   if (P.do_cols) {
-    s += fmt("  if (i < %s) {\n    S a = ((const S *)A.colpart)[i];\n", lit_i64(P.cols).c_str());
-    s += fmt("#pragma unroll 8\n    for (i64 t = 1; t < %s; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + i]);\n",
-             lit_i64(P.ntr).c_str(), lit_i64(P.cols).c_str());
-    s += "    if (A.col_sum) { A.col_sum[i] = st_sum(a); fl |= st_ovf(a); }\n    if (A.col_max) A.col_max[i] = st_max(a);\n  }\n";
+    s += fmt("  if ((i64)blockIdx.x < %s) {\n", lit_i64(ncb).c_str());
+    s += "    const int lane = threadIdx.x & 63; const int part = threadIdx.x >> 6;\n";
+    s += "    const i64 c = (i64)blockIdx.x * 64 + lane;\n";
+    s += fmt("    const i64 t0 = part * %s / 4, t1 = (part + 1) * %s / 4;\n", lit_i64(P.ntr).c_str(),
+             lit_i64(P.ntr).c_str());
+    s += "    S a; st_init(a);\n";
+    s += fmt("    if (c < %s) {\n#pragma unroll 4\n      for (i64 t = t0; t < t1; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + c]);\n    }\n",
+             lit_i64(P.cols).c_str(), lit_i64(P.cols).c_str());
+    s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
+    s += fmt("    if (part == 0 && c < %s) {\n", lit_i64(P.cols).c_str());
+    s += "      S r = sh[lane];\n      for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
+    s += "      if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n      if (A.col_max) A.col_max[c] = st_max(r);\n";
+    s += "    }\n  } else {\n";
+  } else {
+    s += "  {\n";
   }
-  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
-  // --- finish: full result over the per-row states, in row order
-  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_full(RedArgs A) {\n";
+  if (P.do_rows) {
+    s += fmt("    const i64 rb = (i64)blockIdx.x - %s;\n", lit_i64(ncb).c_str());
+    s += "    const i64 i = rb * 256 + threadIdx.x;\n    S a; st_init(a);\n";
+    s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
+             lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
+    s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
+    if (P.do_full) {
+      s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
+      s += "    for (int off = 1; off < 256; off <<= 1) {\n      S o = sh[threadIdx.x];\n";
+      s += "      if ((threadIdx.x & (2 * off - 1)) == 0) o = st_comb(o, sh[threadIdx.x + off]);\n";
+      s += "      __syncthreads();\n      sh[threadIdx.x] = o;\n      __syncthreads();\n    }\n";
+      s += "    if (threadIdx.x == 0) ((S *)A.rowfinal)[rb] = sh[0];\n";
+    }
+  }
+  s += "  }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
+  // --- full: one wavefront folds the row blocks' states in order
+  const int64_t nrb = (P.rows + 255) / 256;
+  s += "extern \"C\" __global__ void __launch_bounds__(64) ph_red_full(RedArgs A) {\n";
   s += fmt("  typedef %s S;\n", P.state);
-  s += "  __shared__ S sh[256];\n";
-  s += fmt("  const i64 n = %s; const i64 chunk = (n + 255) / 256;\n", lit_i64(P.rows).c_str());
-  s += "  const i64 b = threadIdx.x * chunk; const i64 e = (b + chunk < n) ? b + chunk : n;\n";
-  s += "  S a; st_init(a);\n  for (i64 i = b; i < e; ++i) a = (i == b) ? ((const S *)A.rowfinal)[i] : st_comb(a, ((const S *)A.rowfinal)[i]);\n";
-  s += "  sh[threadIdx.x] = a;\n  __syncthreads();\n";
-  s += "  for (int off = 1; off < 256; off <<= 1) {\n    S o = sh[threadIdx.x];\n";
-  s += "    if ((threadIdx.x & (2 * off - 1)) == 0 && threadIdx.x + off < 256) o = st_comb(o, sh[threadIdx.x + off]);\n";
-  s += "    __syncthreads();\n    sh[threadIdx.x] = o;\n    __syncthreads();\n  }\n";
-  s += "  if (threadIdx.x == 0) { const S r = sh[0]; if (A.full_sum) { A.full_sum[0] = st_sum(r); if (st_ovf(r)) atomicOr(A.flags, 1); } if (A.full_max) A.full_max[0] = st_max(r); }\n}\n";
+  s += fmt("  const i64 n = %s; const i64 per = (n + 63) / 64;\n", lit_i64(nrb).c_str());
+  s += "  const i64 b = threadIdx.x * per; const i64 e = (b + per < n) ? b + per : n;\n";
+  s += "  S a; st_init(a);\n  for (i64 j = b; j < e; ++j) a = st_comb(a, ((const S *)A.rowfinal)[j]);\n";
+  s += "  a = wave_fold(a);\n";
+  s += "  if (threadIdx.x == 0) { if (A.full_sum) { A.full_sum[0] = st_sum(a); if (st_ovf(a)) atomicOr(A.flags, 1); } if (A.full_max) A.full_max[0] = st_max(a); }\n}\n";
   return s;
 }
 
@@ -914,7 +933,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
     fused_cache()[key] = FusedKernels{{ktile, kfin, kfull}, 0};
   }
   const bool may_flag = prog_may_flag(prog) || (std::string(P.state) == "SI" && (full_sum || row_sum || col_sum));
-  // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [rows]
+  // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [row blocks of 256] (sized [rows])
   const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
   const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
   const size_t rowfinal_b = P.do_rows ? (size_t)P.rows * P.state_bytes : 0;
@@ -949,11 +968,12 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   if (rc) return rc;
   PH_HIP(hipModuleLaunchKernel(ktile, (unsigned)(P.ntr * P.ntc), 1, 1, 256, 1, 1, 0, ph::stream(),
                                nullptr, cfg));
-  const int64_t nfin = (P.rows > P.cols ? P.rows : P.cols);
-  PH_HIP(hipModuleLaunchKernel(kfin, (unsigned)((nfin + 255) / 256), 1, 1, 256, 1, 1, 0, ph::stream(),
-                               nullptr, cfg));
+  // finish: column blocks of 64 columns, then row blocks of 256 rows
+  const int64_t nfin = (P.do_cols ? (P.cols + 63) / 64 : 0) + (P.do_rows ? (P.rows + 255) / 256 : 0);
+  if (nfin > 0)
+    PH_HIP(hipModuleLaunchKernel(kfin, (unsigned)nfin, 1, 1, 256, 1, 1, 0, ph::stream(), nullptr, cfg));
   if (P.do_full)
-    PH_HIP(hipModuleLaunchKernel(kfull, 1, 1, 1, 256, 1, 1, 0, ph::stream(), nullptr, cfg));
+    PH_HIP(hipModuleLaunchKernel(kfull, 1, 1, 1, 64, 1, 1, 0, ph::stream(), nullptr, cfg));
   return read_flags(dflags, may_flag, flags_out);
 }
 
