@@ -72,7 +72,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_rows(E *__restrict__ dst,
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const I c = c0 + (I)(e * kBlock);
-      if (c < cols) dst[db + (DCONT ? c : c * di)] = v[e];
+      if (c < cols) {
+        if (DCONT) __builtin_nontemporal_store(v[e], dst + db + c);  // contiguous rows: streamed
+        else dst[db + c * di] = v[e];
+      }
     }
   }
 }

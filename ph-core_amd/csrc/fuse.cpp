@@ -262,9 +262,10 @@ __device__ __forceinline__ int ph_ld(const int *p, i64 i) { return p[i]; }
 __device__ __forceinline__ float ph_ld(const float *p, i64 i) { return p[i]; }
 __device__ __forceinline__ double ph_ld(const double *p, i64 i) { return p[i]; }
 __device__ __forceinline__ void ph_st(unsigned char *p, i64 i, bool v) { p[i] = v ? 1 : 0; }
-__device__ __forceinline__ void ph_st(int *p, i64 i, int v) { p[i] = v; }
-__device__ __forceinline__ void ph_st(float *p, i64 i, float v) { p[i] = v; }
-__device__ __forceinline__ void ph_st(double *p, i64 i, double v) { p[i] = v; }
+// results are written once and not read again by the kernel: non-temporal
+__device__ __forceinline__ void ph_st(int *p, i64 i, int v) { __builtin_nontemporal_store(v, p + i); }
+__device__ __forceinline__ void ph_st(float *p, i64 i, float v) { __builtin_nontemporal_store(v, p + i); }
+__device__ __forceinline__ void ph_st(double *p, i64 i, double v) { __builtin_nontemporal_store(v, p + i); }
 __device__ __forceinline__ float ph_floor(float x) { return __builtin_floorf(x); }
 __device__ __forceinline__ double ph_floor(double x) { return __builtin_floor(x); }
 __device__ __forceinline__ float ph_sqrt(float x) { return __builtin_sqrtf(x); }
