@@ -193,12 +193,12 @@ def main():
     achieved_g = gather_bytes / (ms_gather * 1e-3) / 1e9
 
     traffic = load_traffic()
-    roofline = {"bound": "hbm", "kernel": "k_fill_scatter<uint32_t,8> (fused NArray.fill + MutableView scatter)",
+    roofline = {"bound": "hbm", "kernel": "k_fill_scatter_vec<uint32_t,4> (fused NArray.fill + MutableView scatter)",
                 "achieved": round(achieved_fs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved_fs / PEAK_HBM_GBS, 4),
                 "traffic": traffic.get("k_fill_scatter") if traffic else None,
                 "ms_per_launch": round(ms_fs, 4), "algorithmic_bytes": scatter_bytes}
-    extra = {"gather": {"kernel": "k_copy_rows<uint32_t,8>", "achieved_gbs": round(achieved_g, 1),
+    extra = {"gather": {"kernel": "k_copy_rows<uint32_t,8,int,DCONT> (strided gather)", "achieved_gbs": round(achieved_g, 1),
                         "frac": round(achieved_g / PEAK_HBM_GBS, 4), "ms_per_launch": round(ms_gather, 4),
                         "algorithmic_bytes": gather_bytes,
                         "traffic": traffic.get("k_copy_rows") if traffic else None}}

@@ -223,10 +223,9 @@ int ph_fill(void *buf, const ph_view_desc *dst, const void *value);
  *   src) (src/n_array.cr:484-492), MultiWritable#unsafe_set_chunk
  *   (src/multi_writable.cr:29-36) through MutableView#unsafe_set_element
  *   (src/mutable_view.cr:16-18).
- * flags & PH_COPY_ZERO_REST: additionally write 0 to every element of dst's
- *   buffer (dd->extent elements) that dd does not address — NArray.fill(
- *   shape, 0) followed by the scatter, in one pass.  dd must be affine (no
- *   stages) and its strides must address distinct elements.
+ * flags: must be 0.  PH_COPY_ZERO_REST (a fill of the rest of dst fused
+ *   into the scatter) is reserved and returns PH_E_UNSUPPORTED: the fused
+ *   fill + scatter takes the region form, ph_fill_scatter below.
  * ------------------------------------------------------------------------- */
 #define PH_COPY_ZERO_REST 1
 int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src,

@@ -617,6 +617,7 @@ int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src, const ph
   if (flags & PH_COPY_ZERO_REST)
     return ph::set_error(PH_E_UNSUPPORTED,
                          "PH_COPY_ZERO_REST needs the region form: use ph_fill_scatter");
+  if (flags != 0) return ph::set_error(PH_E_ARGUMENT, "ph_strided_copy: unknown flags %d", flags);
   return dispatch_copy(dst, dd, src, sd, nullptr);
 }
 
