@@ -321,6 +321,28 @@ def other_workloads(phase, ev, args):
                                                                "f32": tr.get("k_heat_step2_f32")}}
     del fields, u
     out["c5_batched_20000"] = batched_workload(phase, ev, args)
+
+    # NArray.concatenate of 4 x [4096, 4096] Float32 along axis 1 (one
+    # strided copy per input into its slab of the [4096, 16384] result)
+    parts = [phase.NArray.random([4096, 4096], D.F32, seed=40 + k) for k in range(4)]
+    cat = lambda: phase.NArray.concatenate_all(*parts, axis=1)  # noqa: E731
+    cat()
+    ms = time_loop(ev, cat, 10, phase)
+    by = 2 * 4 * 4096 * 4096 * 4
+    out["concatenate_f32_4x4096x4096_axis1"] = {"ms": round(ms, 4), "bytes": by,
+                                                "achieved_gbs": round(by / ms / 1e6, 1),
+                                                "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
+    del parts
+    # JSON / YAML text of a 1024 x 1024 Float64 array (device -> host -> text
+    # and back; host-bound)
+    a = phase.NArray.random([1024, 1024], D.F64, seed=41, a=-1.0, b=1.0)
+    t0 = time.perf_counter()
+    j = a.to_json()
+    t1 = time.perf_counter()
+    phase.NArray.from_json(j, D.F64)
+    t2 = time.perf_counter()
+    out["json_f64_1024x1024"] = {"to_json_ms": round((t1 - t0) * 1e3, 1), "from_json_ms": round((t2 - t1) * 1e3, 1),
+                                 "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing, 1 thread"}
     return out
 
 

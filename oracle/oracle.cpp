@@ -1284,6 +1284,34 @@ int or_slices_reduce(const double *x, const int64_t *shape, int rank, int axis,
 }
 
 // ===========================================================================
+// NArray.concatenate_to_slice (src/n_array.cr:727-751): the output has the
+// inputs' common shape with `axis` summed; for each of the
+// prod(shape[:axis]) chunks, each input in turn contributes its next
+// shape_i[axis] * prod(shape[axis+1:]) elements in lexicographic order.
+// bufs[i] is input i (row-major, esize-byte elements), shapes is n x rank.
+// Returns the number of elements written.
+// ===========================================================================
+int64_t or_concatenate(int n, const void *const *bufs, const int64_t *shapes, int rank, int esize, int axis,
+                       void *out) {
+  int64_t inner = 1, chunks = 1;
+  for (int d = axis + 1; d < rank; ++d) inner *= shapes[d];
+  for (int d = 0; d < axis; ++d) chunks *= shapes[d];
+  std::vector<const char *> it((size_t)n);
+  for (int i = 0; i < n; ++i) it[(size_t)i] = (const char *)bufs[i];
+  char *o = (char *)out;
+  int64_t written = 0;
+  for (int64_t c = 0; c < chunks; ++c)
+    for (int i = 0; i < n; ++i) {
+      const int64_t len = shapes[(int64_t)i * rank + axis] * inner;
+      std::memcpy(o, it[(size_t)i], (size_t)(len * esize));
+      o += len * esize;
+      it[(size_t)i] += len * esize;
+      written += len;
+    }
+  return written;
+}
+
+// ===========================================================================
 // Batched small arrays: the host loop ph-core would run over a list of
 // Float32 NArrays (no batched API exists in the reference):
 //   y_k = x_k.map { |e| e + s_k }            n_array.cr:589-595 (Float32 +)

@@ -1393,6 +1393,77 @@ NArray NArray::permute(const std::vector<int64_t> *order) const { return view().
 NArray NArray::reverse() const { return view().reverse().to_narr(); }
 NArray NArray::clone() const { return view().to_narr(); }
 
+static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc &dd, const Shape &buf_shape,
+                         const Operand &src_in);
+
+// compatible?(*others, axis) (n_array.cr:666-673): every dimension matches,
+// except at index `axis` (as given: a negative axis exempts no dimension).
+// Inputs of another rank are rejected with the same error.
+static void check_concat(const std::vector<Operand> &narrs, int64_t axis) {
+  const Shape &s0 = narrs[0].shape;
+  const std::string msg = "Cannot concatenate these arrays along axis " + std::to_string(axis) +
+                          ": shapes do not match";
+  for (const Operand &n : narrs) {
+    if (n.shape.size() != s0.size()) throw DimensionError(msg);
+    for (size_t idx = 0; idx < s0.size(); ++idx)
+      if (n.shape[idx] != s0[idx] && (int64_t)idx != axis) throw DimensionError(msg);
+  }
+}
+
+// NArray.concatenate_to_slice (n_array.cr:727-751): the output takes the
+// common shape with `axis` summed; input i fills the slab of `axis`
+// coordinates [sum of the earlier inputs' extents, + its extent).  Each slab
+// is one strided scatter, so a chunk of consecutive output elements is
+// written from consecutive input elements, in the reference's order.
+NArray NArray::concatenate(const std::vector<Operand> &narrs, int64_t axis) {
+  if (narrs.empty()) throw ArgumentError("concatenate needs at least one MultiIndexable");
+  check_concat(narrs, axis);
+  const DType vt = narrs[0].value_dtype();
+  for (const Operand &n : narrs)
+    if (n.value_dtype() != vt) throw ArgumentError("concatenate: every MultiIndexable must have the same element type");
+  const int64_t rank = (int64_t)narrs[0].shape.size();
+  const int64_t ax = axis < 0 ? axis + rank : axis;  // Crystal's negative indexing of the shape
+  if (ax < 0 || ax >= rank) throw IndexError("Index out of bounds");
+  Shape out_shape = narrs[0].shape;
+  out_shape[(size_t)ax] = 0;
+  for (const Operand &n : narrs) out_shape[(size_t)ax] += n.shape[(size_t)ax];
+  DType st = vt;
+  if (vt == compute_dtype((DType)narrs[0].desc.dtype) && !narrs[0].body) st = (DType)narrs[0].desc.dtype;
+  NArray out = NArray::empty(out_shape, st);
+  int64_t off = 0;
+  for (const Operand &n : narrs) {
+    const int64_t len = n.shape[(size_t)ax];
+    if (len > 0 && out.size() > 0) {
+      View v = out.view();
+      IndexRegion r = IndexRegion::cover(out_shape);
+      r.first[(size_t)ax] = off;
+      r.last[(size_t)ax] = off + len - 1;
+      r.proper_shape[(size_t)ax] = len;
+      r.reduced_shape = r.proper_shape;
+      v.restrict_to(r);
+      scatter_into(out.storage, v.desc_, out_shape, n);
+    }
+    off += len;
+  }
+  return out;
+}
+
+NArray NArray::concatenate_with(const std::vector<Operand> &others, int64_t axis) const {
+  std::vector<Operand> all{operand()};
+  all.insert(all.end(), others.begin(), others.end());
+  return NArray::concatenate(all, axis);
+}
+
+NArray &NArray::concatenate_inplace(const std::vector<Operand> &others, int64_t axis) {
+  NArray r = concatenate_with(others, axis);
+  storage = r.storage;
+  shape_ = r.shape_;
+  return *this;
+}
+
+// push(*others, axis = 0) (n_array.cr:688-710): in-place along axis 0
+NArray &NArray::push(const std::vector<Operand> &others) { return concatenate_inplace(others, 0); }
+
 std::vector<NArray> NArray::slices(int axis) const {
   if (axis < 0 || axis >= dimensions()) throw IndexError("axis out of range");
   Shape chunk = shape_;

@@ -60,6 +60,9 @@ struct EmptyError : PhaseError {  // Enumerable::EmptyError
 struct DeviceError : PhaseError {  // a failure inside the HIP library
   using PhaseError::PhaseError;
 };
+struct ParseError : PhaseError {  // JSON::Error / YAML::Error
+  using PhaseError::PhaseError;
+};
 
 // Raises the exception matching a C-ABI status code.
 [[noreturn]] void raise_status(int code, const std::string &context);
@@ -282,6 +285,12 @@ class NArray {
   NArray reverse() const;
   NArray clone() const;
   std::vector<NArray> slices(int axis = 0) const;                      // (:784-786)
+  // NArray.concatenate(*narrs, axis) (n_array.cr:723-725, 727-751) and the
+  // instance forms concatenate / concatenate! / push (:712-721, :688-710)
+  static NArray concatenate(const std::vector<Operand> &narrs, int64_t axis = 0);
+  NArray concatenate_with(const std::vector<Operand> &others, int64_t axis = 0) const;
+  NArray &concatenate_inplace(const std::vector<Operand> &others, int64_t axis = 0);
+  NArray &push(const std::vector<Operand> &others);
 
   // MultiWritable
   void set_chunk(const RegionLiteral &lit, const Operand &src);        // (multi_writable.cr:55-63)
@@ -429,6 +438,15 @@ class Batch {
   void *handle_ = nullptr;
   NArray piece(const NArray &flat, int64_t off, const Shape &shape) const;
 };
+
+// Serialisation (n_array.cr:807-911): {"shape": [...], "elements": [...]}
+// as JSON and as YAML, elements in buffer order, each as its type's to_s.
+std::string to_json(const NArray &a);
+NArray narray_from_json(const std::string &text, DType dtype);
+std::string to_yaml(const NArray &a);
+NArray narray_from_yaml(const std::string &text, DType dtype);
+// Float#to_s: shortest round-trip digits, plain or d.ddde+XX notation.
+std::string float_to_s(double v, bool single);
 
 // Initialises the device (idempotent).
 void init(int device = 0);

@@ -232,6 +232,7 @@ PYBIND11_MODULE(_phase, m) {
   py::register_exception<DivisionByZeroError>(m, "DivisionByZeroError", PyExc_ZeroDivisionError);
   py::register_exception<EmptyError>(m, "EmptyError", base.ptr());
   py::register_exception<DeviceError>(m, "DeviceError", base.ptr());
+  py::register_exception<ParseError>(m, "ParseError", PyExc_ValueError);
 
   py::enum_<DType>(m, "DType")
       .value("I32", DType::I32)
@@ -467,6 +468,38 @@ PYBIND11_MODULE(_phase, m) {
       })
       .def("reverse", &NArray::reverse)
       .def("clone", &NArray::clone)
+      .def("to_json", [](const NArray &a) { return to_json(a); })
+      .def("to_yaml", [](const NArray &a) { return to_yaml(a); })
+      .def_static("from_json", &narray_from_json, py::arg("text"), py::arg("dtype"))
+      .def_static("from_yaml", &narray_from_yaml, py::arg("text"), py::arg("dtype"))
+      .def_static("concatenate_all",
+                  [](py::args narrs, int64_t axis) {
+                    std::vector<Operand> v;
+                    for (auto o : narrs) v.push_back(to_operand(o));
+                    return NArray::concatenate(v, axis);
+                  },
+                  "NArray.concatenate(*narrs, axis: 0)", py::kw_only(), py::arg("axis") = 0)
+      .def("concatenate",
+           [](const NArray &a, py::args others, int64_t axis) {
+             std::vector<Operand> v;
+             for (auto o : others) v.push_back(to_operand(o));
+             return a.concatenate_with(v, axis);
+           },
+           py::kw_only(), py::arg("axis") = 0)
+      .def("concatenate_",
+           [](NArray &a, py::args others, int64_t axis) -> NArray & {
+             std::vector<Operand> v;
+             for (auto o : others) v.push_back(to_operand(o));
+             return a.concatenate_inplace(v, axis);
+           },
+           py::kw_only(), py::arg("axis") = 0, py::return_value_policy::reference_internal)
+      .def("push",
+           [](NArray &a, py::args others) -> NArray & {
+             std::vector<Operand> v;
+             for (auto o : others) v.push_back(to_operand(o));
+             return a.push(v);
+           },
+           py::return_value_policy::reference_internal)
       .def("to_narr", &NArray::clone)
       .def("slices", &NArray::slices, py::arg("axis") = 0)
       .def("map_", [](NArray &a, py::function fn) -> NArray & {
@@ -560,6 +593,7 @@ PYBIND11_MODULE(_phase, m) {
       .def("get", [](const ProcView &v, py::args c) { return from_scalar(v.get(args_ints(c))); })
       .def_property_readonly("body", [](const ProcView &v) { return v.body; });
 
+  m.def("float_to_s", &phase::float_to_s, py::arg("value"), py::arg("single") = false);
   m.def("heat2d", &phase::heat2d, py::arg("u"), py::arg("r"), py::arg("steps"), py::arg("every") = 0);
 
   py::class_<Batch>(m, "Batch")

@@ -100,6 +100,8 @@ def lib():
         L.or_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
         L.or_sum_seq_f64.argtypes = [C.c_void_p, C.c_int64]
         L.or_sum_seq_f64.restype = C.c_double
+        L.or_concatenate.argtypes = [C.c_int, C.POINTER(C.c_void_p), i64p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.or_concatenate.restype = C.c_int64
         L.or_sum_seq_f32.argtypes = [C.c_void_p, C.c_int64]
         L.or_sum_seq_f32.restype = C.c_float
         L.or_batch_loop.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -363,6 +365,20 @@ def sum_seq(x):
 def max_seq(x):
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
     return lib().or_max_seq_f64(_ptr(x), x.size)
+
+
+def concatenate(arrays, axis=0):
+    """NArray.concatenate_to_slice over numpy arrays of one dtype and rank
+    (axis >= 0)."""
+    arrays = [np.ascontiguousarray(a) for a in arrays]
+    rank = arrays[0].ndim
+    shape = list(arrays[0].shape)
+    shape[axis] = sum(a.shape[axis] for a in arrays)
+    out = np.zeros(max(int(np.prod(shape)), 1), dtype=arrays[0].dtype)
+    ptrs = (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+    shp = _i64([d for a in arrays for d in a.shape] or [0])
+    n = lib().or_concatenate(len(arrays), ptrs, shp, rank, arrays[0].itemsize, axis, _ptr(out))
+    return out[:n].reshape(shape)
 
 
 def sum_seq_f32(x):

@@ -233,3 +233,22 @@ def test_element_programs_compile_for_gfx950(host):
             host.compile_check(e, [d], 64, 1024, 1)
     two = host.input(0, host.DType.F64) + host.input(1, host.DType.F64)
     host.compile_check(two * host.input(2, host.DType.F64), [host.DType.F64] * 3, 16, 16, 0)
+
+
+def test_float_text_layout(host):
+    """Float element text for JSON/YAML: shortest round-trip digits in
+    Crystal's layout (parity unpinned: no reference spec prints a float; the
+    cases follow the layout rule: plain notation while the decimal point is
+    within 15 digits left / 3 zeros right of the first digit)."""
+    cases = [(1.0, "1.0"), (0.1, "0.1"), (-2.5, "-2.5"), (100.0, "100.0"), (123.456, "123.456"),
+             (1e14, "100000000000000.0"), (1e15, "1.0e+15"), (1.5e20, "1.5e+20"), (0.0001, "0.0001"),
+             (1e-5, "1.0e-5"), (-0.0, "-0.0"), (0.0, "0.0"), (float("inf"), "Infinity"),
+             (float("-inf"), "-Infinity"), (123456789012345.0, "123456789012345.0"),
+             (0.30000000000000004, "0.30000000000000004")]
+    for v, want in cases:
+        assert host.float_to_s(v) == want, (v, host.float_to_s(v))
+    assert host.float_to_s(float("nan")) == "NaN"
+    assert host.float_to_s(float(np.float32(0.1)), True) == "0.1"
+    rng = np.random.default_rng(3)
+    for v in rng.standard_normal(2000) * 10.0 ** rng.integers(-30, 30, 2000):
+        assert float(host.float_to_s(float(v)).replace("e+", "e")) == v  # reads back exactly

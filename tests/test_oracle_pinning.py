@@ -468,3 +468,20 @@ def test_generator_is_splitmix64():
         r = math.sqrt(-2.0 * math.log(1.0 - u1))
         t = 2 * math.pi * u2
         assert nrm[k] == pytest.approx(r * (math.sin(t) if k & 1 else math.cos(t)), rel=1e-14)
+
+
+def test_concatenate_vs_numpy():
+    """or_concatenate (concatenate_to_slice's chunk loop) against
+    numpy.concatenate on random shapes, axes and element sizes."""
+    rng = np.random.default_rng(12)
+    for _ in range(200):
+        rank = int(rng.integers(1, 5))
+        base = [int(v) for v in rng.integers(0, 5, rank)]
+        axis = int(rng.integers(0, rank))
+        dt = [np.int32, np.float64, np.uint16, np.uint8][int(rng.integers(0, 4))]
+        arrs = []
+        for _ in range(int(rng.integers(1, 5))):
+            s = list(base)
+            s[axis] = int(rng.integers(0, 5))
+            arrs.append(rng.integers(0, 200, s).astype(dt))
+        assert np.array_equal(O.concatenate(arrs, axis), np.concatenate(arrs, axis=axis))
