@@ -24,6 +24,7 @@
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <vector>
@@ -898,10 +899,14 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
                   ((uintptr_t)ins[j] % vbytes) == 0;
   }
   P.ntc = (P.cols + 1023) / 1024;
-  // rows per tile: aim for >= ~2048 workgroups, between 8 and 256 rows
-  // (a multiple of 16: the float path folds rows 16 at a time)
-  int64_t tr = P.rows * P.ntc / 2048;
-  if (tr > 256) tr = 256;
+  // rows per tile: aim for ~512 workgroups (fewer, taller tiles write fewer
+  // column partial states; measured best on 16384^2), a multiple of 16 (rows
+  // are folded 8 and 16 at a time), at most 1024 rows (256 for integer
+  // states, whose per-row LDS states are larger)
+  int64_t tr = P.rows * P.ntc / 512;
+  if (const char *e = std::getenv("PH_RED_TR")) tr = std::atol(e);  // tuning override
+  const int64_t tr_cap = std::string(P.state) == "SI" ? 256 : 1024;
+  if (tr > tr_cap) tr = tr_cap;
   tr = (tr + 15) / 16 * 16;
   if (tr < 16) tr = 16;
   if (std::string(P.state) == "SI" && tr > P.rows) tr = P.rows;
