@@ -1,15 +1,14 @@
 // Explicit 2-D heat equation (5-point FTCS, Dirichlet zero outside the grid)
 // and the FP64 reductions around it (total energy, relative L2 distance).
 //
-// Step kernel: a wavefront owns a strip of 64*VEC consecutive columns and
-// sweeps TY rows top to bottom, keeping the north, centre and south rows of
-// its strip in registers (each cell of the grid is read from memory once per
-// step, plus one row above and below each TY-row band).  West/east neighbours
-// come from the adjacent lane by a cross-lane shuffle; the two cells beyond
-// the strip's edges are loaded by lanes 0 and 63.  Loads are VEC-wide
-// (16 bytes per lane) and fully coalesced.  Workgroups are numbered so that
-// strips adjacent in the grid run on the same XCD (blocks are dispatched to
-// the 8 XCDs round-robin), which keeps the strip-edge loads in that XCD's L2.
+// Step kernel (k_heat_multi): K time steps per pass over the grid.  A
+// wavefront owns a strip of columns and sweeps a band of rows top to bottom,
+// keeping two rows per time level in registers, so each pass reads the grid
+// once (plus 2K halo rows per band and one overlapping lane per strip edge)
+// and writes it once.  Loads and stores are VEC-wide (16 or 32 bytes per
+// lane) and coalesced.  Workgroups are numbered so that strips adjacent in
+// the grid run on the same XCD (workgroups are dealt to the 8 XCDs
+// round-robin), which keeps the overlapping columns in that XCD's L2.
 //
 // The cell update keeps the operation order of examples/heat_equation.cr:47
 // and :50 extended to two axes, with no contraction into FMAs:
@@ -56,191 +55,29 @@ struct HeatArgs {
   int64_t per_xcd;   // workgroups per XCD slice
 };
 
-// Row y of the strip: VEC values per lane plus the west/east halo cells.
-template <typename T>
-__device__ __forceinline__ void load_row(const T *__restrict__ u, const HeatArgs &a, int64_t y,
-                                         int64_t x0, int lane, bool vec_ok, T *v, T *west,
-                                         T *east) {
-  constexpr int V = Vec<T>::n;
-  const int64_t x = x0 + (int64_t)lane * V;
-  if (y < 0 || y >= a.ny) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = T(0);
-    *west = T(0);
-    *east = T(0);
-    return;
-  }
-  const T *row = u + y * a.nx;
-  if (vec_ok && x + V <= a.nx) {
-    unpack<T>(*(const typename Vec<T>::type *)(row + x), v);
-  } else {
-#pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = (x + i < a.nx) ? row[x + i] : T(0);
-  }
-  *west = (lane == 0 && x0 > 0) ? row[x0 - 1] : T(0);
-  const int64_t xe = x0 + 64 * V;
-  *east = (lane == 63 && xe < a.nx) ? row[xe] : T(0);
-}
-
-template <typename T>
-__global__ __launch_bounds__(64 * kWaves) void k_heat_step(const T *__restrict__ u,
-                                                           T *__restrict__ out, HeatArgs a, T r,
-                                                           int vec_ok) {
-  constexpr int V = Vec<T>::n;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  // XCD-aware renumbering: block b runs on XCD b % 8; give each XCD a
-  // contiguous range of logical blocks.
-  const int64_t b = blockIdx.x;
-  const int64_t lb = (b % 8) * a.per_xcd + b / 8;
-  if (lb >= a.nblocks) return;
-  const int64_t strip = (lb % ((a.nstrips + kWaves - 1) / kWaves)) * kWaves + wave;
-  const int64_t band = lb / ((a.nstrips + kWaves - 1) / kWaves);
-  if (strip >= a.nstrips) return;
-  const int64_t x0 = strip * 64 * V;
-  const int64_t y0 = band * a.ty;
-  const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
-  T n[V], c[V], cw, ce, dummy_w, dummy_e;
-  load_row<T>(u, a, y0 - 1, x0, lane, vec_ok, n, &dummy_w, &dummy_e);
-  load_row<T>(u, a, y0, x0, lane, vec_ok, c, &cw, &ce);
-  const T four = T(4);
-  const int64_t x = x0 + (int64_t)lane * V;
-  // kAhead south rows are loaded together before any of them is used, so a
-  // wavefront keeps kAhead row loads in flight instead of one.
-  constexpr int kAhead = 4;
-  for (int64_t y = y0; y < y1; y += kAhead) {
-    T s[kAhead][V], sw[kAhead], se[kAhead];
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      if (y + 1 + k <= y1) load_row<T>(u, a, y + 1 + k, x0, lane, vec_ok, s[k], &sw[k], &se[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      if (y + k >= y1) break;
-      // west neighbour of c[0] is the previous lane's c[V-1]; east of c[V-1]
-      // is the next lane's c[0]
-      T wl = __shfl_up(c[V - 1], 1, 64);
-      T er = __shfl_down(c[0], 1, 64);
-      if (lane == 0) wl = cw;
-      if (lane == 63) er = ce;
-      T o[V];
-#pragma unroll
-      for (int i = 0; i < V; ++i) {
-        const T w = i == 0 ? wl : c[i - 1];
-        const T e = i == V - 1 ? er : c[i + 1];
-        const T d = (((n[i] - four * c[i]) + s[k][i]) + w) + e;
-        o[i] = c[i] + d * r;
-      }
-      T *orow = out + (y + k) * a.nx;
-      if (vec_ok && x + V <= a.nx) {
-        *(typename Vec<T>::type *)(orow + x) = pack<T>(o);
-      } else {
-#pragma unroll
-        for (int i = 0; i < V; ++i)
-          if (x + i < a.nx) orow[x + i] = o[i];
-      }
-#pragma unroll
-      for (int i = 0; i < V; ++i) {
-        n[i] = c[i];
-        c[i] = s[k][i];
-      }
-      cw = sw[k];
-      ce = se[k];
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Two time steps per pass (temporal blocking).  The wavefront sweeps its strip
-// keeping three rows of level t0 (read from memory) and three rows of level
-// t1 (computed) in registers; level t2 is written.  Level-t1 values are also
-// needed one column beyond each strip edge, so lanes 0 and 63 carry two halo
-// columns of t0 and one of t1.  Every t1 and t2 value is computed with the
-// single-step formula, so the result is bit-identical to two single steps;
-// cells outside the grid stay 0 at every level.
+// K time steps per pass (temporal blocking, 1 <= K <= VEC).
+//
+// Columns: strips overlap.  A wavefront holds 64*VEC consecutive columns
+// starting VEC columns left of the 62*VEC columns it writes (lanes 1..62).
+// West/east neighbours come from the adjacent lane by a shuffle; lanes 0 and
+// 63 have no neighbour beyond the wavefront, so their values go wrong by one
+// column per level, and after K <= VEC levels only lanes 0 and 63 are wrong:
+// they are never stored.
+// Rows: level l+1 of row y is computed from level-l rows y-1, y, y+1.  The
+// wavefront keeps, per level, the last two rows in registers; each loaded
+// level-0 row y+K yields level-1 row y+K-1, …, level-K row y.  The sweep
+// starts 2K rows above the band, so every level is exact before the band's
+// first output row (level l is exact 2l rows after the start).
+// Cells outside the grid are 0 at every level, and each value is computed
+// with the single-step formula, so K steps per pass are bit-identical to K
+// single steps.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct Row0 {  // a t0 row of the strip plus two halo columns on each side
-  T v[Vec<T>::n];
-  T w1, w2, e1, e2;  // x0-1, x0-2, x0+W, x0+W+1
-};
-template <typename T>
-struct Row1 {  // a t1 row plus one halo column on each side
-  T v[Vec<T>::n];
-  T w, e;  // x0-1, x0+W
-};
-
-template <typename T>
-__device__ __forceinline__ void load_row0(const T *__restrict__ u, const HeatArgs &a, int64_t y, int64_t x0,
-                                          int lane, bool vec_ok, Row0<T> &r) {
+template <typename T, int K, int kAhead>
+__global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict__ u, T *__restrict__ out,
+                                                            HeatArgs a, T r, int vec_ok) {
   constexpr int V = Vec<T>::n;
-  const int64_t x = x0 + (int64_t)lane * V;
-  r.w1 = r.w2 = r.e1 = r.e2 = T(0);
-  if (y < 0 || y >= a.ny) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) r.v[i] = T(0);
-    return;
-  }
-  const T *row = u + y * a.nx;
-  if (vec_ok && x + V <= a.nx) {
-    unpack<T>(*(const typename Vec<T>::type *)(row + x), r.v);
-  } else {
-#pragma unroll
-    for (int i = 0; i < V; ++i) r.v[i] = (x + i < a.nx) ? row[x + i] : T(0);
-  }
-  const int64_t xe = x0 + 64 * V;
-  if (lane == 0) {
-    r.w1 = x0 - 1 >= 0 ? row[x0 - 1] : T(0);
-    r.w2 = x0 - 2 >= 0 ? row[x0 - 2] : T(0);
-  }
-  if (lane == 63) {
-    r.e1 = xe < a.nx ? row[xe] : T(0);
-    r.e2 = xe + 1 < a.nx ? row[xe + 1] : T(0);
-  }
-}
-
-// t1 row from t0 rows n (above), c, s (below); row index y1row.
-template <typename T>
-__device__ __forceinline__ void step_row(const Row0<T> &n, const Row0<T> &c, const Row0<T> &s, T r,
-                                         const HeatArgs &a, int64_t yrow, int64_t x0, int lane, Row1<T> &o) {
-  constexpr int V = Vec<T>::n;
-  const T four = T(4);
-  if (yrow < 0 || yrow >= a.ny) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) o.v[i] = T(0);
-    o.w = o.e = T(0);
-    return;
-  }
-  T wl = __shfl_up(c.v[V - 1], 1, 64);
-  T er = __shfl_down(c.v[0], 1, 64);
-  if (lane == 0) wl = c.w1;
-  if (lane == 63) er = c.e1;
-  const int64_t x = x0 + (int64_t)lane * V;
-#pragma unroll
-  for (int i = 0; i < V; ++i) {
-    const T w = i == 0 ? wl : c.v[i - 1];
-    const T e = i == V - 1 ? er : c.v[i + 1];
-    const T d = (((n.v[i] - four * c.v[i]) + s.v[i]) + w) + e;
-    o.v[i] = (x + i < a.nx) ? c.v[i] + d * r : T(0);
-  }
-  o.w = T(0);
-  o.e = T(0);
-  if (lane == 0 && x0 - 1 >= 0) {  // column x0-1: west x0-2, east x0 (own v[0])
-    const T d = (((n.w1 - four * c.w1) + s.w1) + c.w2) + c.v[0];
-    o.w = c.w1 + d * r;
-  }
-  const int64_t xe = x0 + 64 * V;
-  if (lane == 63 && xe < a.nx) {  // column x0+W: west x0+W-1 (own v[V-1]), east x0+W+1
-    const T d = (((n.e1 - four * c.e1) + s.e1) + c.v[V - 1]) + c.e2;
-    o.e = c.e1 + d * r;
-  }
-}
-
-template <typename T, int kAhead>
-__global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict__ u,
-                                                            T *__restrict__ out, HeatArgs a, T r,
-                                                            int vec_ok) {
-  constexpr int V = Vec<T>::n;
+  static_assert(K >= 1 && K <= V, "K steps per pass need K <= VEC halo columns");
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t b = blockIdx.x;
@@ -250,56 +87,79 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_step2(const T *__restrict_
   const int64_t strip = (lb % gpb) * kWaves + wave;
   const int64_t band = lb / gpb;
   if (strip >= a.nstrips) return;
-  const int64_t x0 = strip * 64 * V;
+  const int64_t x = strip * 62 * V - V + (int64_t)lane * V;  // this lane's first column
   const int64_t y0 = band * a.ty;
   const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
-  const int64_t x = x0 + (int64_t)lane * V;
-  const T four = T(4);
-  // start-up: t0 rows y0-2 .. y0+1, t1 rows y0-1, y0
-  Row0<T> t0a, t0b, t0c, t0d;
-  load_row0<T>(u, a, y0 - 2, x0, lane, vec_ok, t0a);
-  load_row0<T>(u, a, y0 - 1, x0, lane, vec_ok, t0b);
-  load_row0<T>(u, a, y0, x0, lane, vec_ok, t0c);
-  load_row0<T>(u, a, y0 + 1, x0, lane, vec_ok, t0d);
-  Row1<T> P, Q;
-  step_row<T>(t0a, t0b, t0c, r, a, y0 - 1, x0, lane, P);
-  step_row<T>(t0b, t0c, t0d, r, a, y0, x0, lane, Q);
-  Row0<T> A = t0c, B = t0d;
-  for (int64_t y = y0; y < y1; y += kAhead) {
-    Row0<T> C[kAhead];
+  const bool store_lane = lane != 0 && lane != 63;
+  const bool full = vec_ok && x >= 0 && x + V <= a.nx;  // whole aligned vector inside the row
+  bool colok[V];
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k)
-      if (y + k < y1) load_row0<T>(u, a, y + 2 + k, x0, lane, vec_ok, C[k]);
+  for (int i = 0; i < V; ++i) colok[i] = x + i >= 0 && x + i < a.nx;
+  const T four = T(4);
+  T prev[K][2][V];  // per level l: rows y+K-l-2 and y+K-l-1
+#pragma unroll
+  for (int l = 0; l < K; ++l)
+#pragma unroll
+    for (int i = 0; i < V; ++i) prev[l][0][i] = prev[l][1][i] = T(0);
+  for (int64_t y = y0 - 2 * K; y < y1; y += kAhead) {
+    T L[kAhead][V];  // level-0 rows y+K .. y+K+kAhead-1
 #pragma unroll
     for (int k = 0; k < kAhead; ++k) {
-      if (y + k >= y1) break;
-      Row1<T> R;
-      step_row<T>(A, B, C[k], r, a, y + k + 1, x0, lane, R);
-      // level t2 row y+k from t1 rows P (above), Q, R (below)
-      T wl = __shfl_up(Q.v[V - 1], 1, 64);
-      T er = __shfl_down(Q.v[0], 1, 64);
-      if (lane == 0) wl = Q.w;
-      if (lane == 63) er = Q.e;
-      T o[V];
+      const int64_t ry = y + k + K;
+      if (y + k < y1 && ry >= 0 && ry < a.ny) {
+        const T *row = u + ry * a.nx;
+        if (full) {
+          unpack<T>(*(const typename Vec<T>::type *)(row + x), L[k]);
+        } else {
 #pragma unroll
-      for (int i = 0; i < V; ++i) {
-        const T w = i == 0 ? wl : Q.v[i - 1];
-        const T e = i == V - 1 ? er : Q.v[i + 1];
-        const T d = (((P.v[i] - four * Q.v[i]) + R.v[i]) + w) + e;
-        o[i] = Q.v[i] + d * r;
-      }
-      T *orow = out + (y + k) * a.nx;
-      if (vec_ok && x + V <= a.nx) {
-        *(typename Vec<T>::type *)(orow + x) = pack<T>(o);
+          for (int i = 0; i < V; ++i) L[k][i] = colok[i] ? row[x + i] : T(0);
+        }
       } else {
 #pragma unroll
-        for (int i = 0; i < V; ++i)
-          if (x + i < a.nx) orow[x + i] = o[i];
+        for (int i = 0; i < V; ++i) L[k][i] = T(0);
       }
-      A = B;
-      B = C[k];
-      P = Q;
-      Q = R;
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int64_t yy = y + k;
+      if (yy >= y1) break;
+      T cur[V];
+#pragma unroll
+      for (int i = 0; i < V; ++i) cur[i] = L[k][i];
+#pragma unroll
+      for (int l = 0; l < K; ++l) {
+        // level l+1 of row yy+K-l-1: north prev[l][0], centre prev[l][1], south cur
+        const int64_t row = yy + K - l - 1;
+        const bool rowok = row >= 0 && row < a.ny;
+        const T wl = __shfl_up(prev[l][1][V - 1], 1, 64);
+        const T er = __shfl_down(prev[l][1][0], 1, 64);
+        T nxt[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          const T c = prev[l][1][i];
+          const T w = i == 0 ? wl : prev[l][1][i - 1];
+          const T e = i == V - 1 ? er : prev[l][1][i + 1];
+          const T d = (((prev[l][0][i] - four * c) + cur[i]) + w) + e;
+          nxt[i] = (rowok && colok[i]) ? c + d * r : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          prev[l][0][i] = prev[l][1][i];
+          prev[l][1][i] = cur[i];
+          cur[i] = nxt[i];
+        }
+      }
+      // cur: level K of row yy
+      if (yy >= y0 && store_lane) {
+        T *orow = out + yy * a.nx;
+        if (full) {
+          *(typename Vec<T>::type *)(orow + x) = pack<T>(cur);
+        } else {
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+            if (colok[i]) orow[x + i] = cur[i];
+        }
+      }
     }
   }
 }
@@ -408,27 +268,41 @@ int fp64_sum(const TA *a, const TB *b, int64_t n, double *out) {
   return ph::check_launch("k_sum");
 }
 
+template <typename T, int K>
+int launch_multi(const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int ahead) {
+  const unsigned grid = (unsigned)(a.per_xcd * 8);
+  if (ahead >= 2)
+    hipLaunchKernelGGL((k_heat_multi<T, K, 2>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
+                       vec_ok);
+  else
+    hipLaunchKernelGGL((k_heat_multi<T, K, 1>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
+                       vec_ok);
+  return ph::check_launch("k_heat_multi");
+}
+
 template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
              double *energy) {
   constexpr int V = Vec<T>::n;
+  // Steps per pass, band height and row prefetch depth, per precision
+  // (measured on MI355X at 8192^2: 4 steps; FP64 32-row bands, FP32 64-row
+  // bands; 2 rows loaded ahead); PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD
+  // override them for tuning.
+  int64_t kmax = V;
+  int64_t ty = sizeof(T) == 4 ? 64 : 32;
+  int64_t ahead = 2;
+  if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(V, std::max<int64_t>(1, std::atol(e)));
+  if (const char *e = std::getenv("PH_HEAT_TY")) ty = std::max<int64_t>(8, std::atol(e));
+  if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead = std::atol(f);
   HeatArgs a;
   a.ny = ny;
   a.nx = nx;
-  a.nstrips = (nx + 64 * V - 1) / (64 * V);
-  // Band height and row prefetch depth of the two-step kernel, per precision
-  // (measured on MI355X at 8192^2: FP32 32 rows / no prefetch, FP64 16 rows /
-  // 2 rows ahead); PH_HEAT_TY and PH_HEAT_AHEAD override them for tuning.
-  int64_t ty_env = sizeof(T) == 4 ? 32 : 16;
-  int64_t ahead_env = sizeof(T) == 4 ? 1 : 2;
-  if (const char *e = std::getenv("PH_HEAT_TY")) ty_env = std::max<int64_t>(8, std::atol(e));
-  if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead_env = std::atol(f);
-  a.ty = ty_env;
+  a.nstrips = (nx + 62 * V - 1) / (62 * V);  // 62 of a wavefront's 64 lanes are written
+  a.ty = ty;
   a.nbands = (ny + a.ty - 1) / a.ty;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
   a.nblocks = groups_per_band * a.nbands;
   a.per_xcd = (a.nblocks + 7) / 8;
-  const int64_t grid = a.per_xcd * 8;
   constexpr uintptr_t kAlign = sizeof(typename Vec<T>::type);
   const int vec_ok = (nx % V == 0) && ((uintptr_t)u % kAlign == 0) && ((uintptr_t)tmp % kAlign == 0);
   const T rr = (T)r;
@@ -439,36 +313,24 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
     if (rc) return rc;
   }
-  // Steps run two at a time (k_heat_step2) up to each energy checkpoint; an
-  // odd remainder runs as a single step.
+  // Steps run kmax at a time up to each energy checkpoint; the remainder
+  // before a checkpoint runs as one shorter pass.
   int64_t t = 0;
   while (t < steps) {
     int64_t until = steps;
     if (energy && every > 0) until = std::min<int64_t>(steps, (t / every + 1) * every);
-    const bool pair = until - t >= 2;
+    const int k = (int)std::min<int64_t>(kmax, until - t);
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
-    if (pair) {
-      if (ahead_env == 4)
-        hipLaunchKernelGGL((k_heat_step2<T, 4>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                           (const T *)cur, nxt, a, rr, vec_ok);
-      else if (ahead_env == 1)
-        hipLaunchKernelGGL((k_heat_step2<T, 1>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                           (const T *)cur, nxt, a, rr, vec_ok);
-      else
-        hipLaunchKernelGGL((k_heat_step2<T, 2>), dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                           (const T *)cur, nxt, a, rr, vec_ok);
-      rc = ph::check_launch("k_heat_step2");
-    } else {
-      hipLaunchKernelGGL(k_heat_step<T>, dim3((unsigned)grid), dim3(64 * kWaves), 0, ph::stream(),
-                         (const T *)cur, nxt, a, rr, vec_ok);
-      rc = ph::check_launch("k_heat_step");
-    }
+    if (k == 4) rc = launch_multi<T, 4>(cur, nxt, a, rr, vec_ok, (int)ahead);
+    else if (k == 3) rc = launch_multi<T, 3>(cur, nxt, a, rr, vec_ok, (int)ahead);
+    else if (k == 2) rc = launch_multi<T, 2>(cur, nxt, a, rr, vec_ok, (int)ahead);
+    else rc = launch_multi<T, 1>(cur, nxt, a, rr, vec_ok, (int)ahead);
     if (rc) return rc;
     T *sw = cur;
     cur = nxt;
     nxt = sw;
-    t += pair ? 2 : 1;
+    t += k;
     if (energy && every > 0 && t % every == 0) {
       rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
       if (rc) return rc;

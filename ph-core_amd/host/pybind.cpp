@@ -468,6 +468,8 @@ PYBIND11_MODULE(_phase, m) {
       })
       .def("reverse", &NArray::reverse)
       .def("clone", &NArray::clone)
+      .def_property_readonly("device_ptr", [](const NArray &a) { return (uintptr_t)a.device_ptr(); },
+                             "address of the device buffer (for C-ABI calls)")
       .def("to_json", [](const NArray &a) { return to_json(a); })
       .def("to_yaml", [](const NArray &a) { return to_yaml(a); })
       .def_static("from_json", &narray_from_json, py::arg("text"), py::arg("dtype"))
