@@ -312,8 +312,8 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
  * the 2-D generalisation of examples/heat_equation.cr:38-51 (same operation
  * order as :47, `(s[i-1] - 2*s[i] + s[i+1]) * COEFF`, then `el + diff` :50).
  * dtype PH_F32 or PH_F64; u holds the initial field and receives the final
- * one; tmp is scratch of the same size.  Up to four steps run per pass over
- * the grid (temporal blocking), bit-identical to single steps.  If
+ * one; tmp is scratch of the same size.  Up to eight steps run per pass
+ * over the grid (temporal blocking), bit-identical to single steps.  If
  * energy_every > 0, energy_out
  * (device, steps/energy_every + 1 doubles) receives the FP64 sum of u at
  * steps 0, energy_every, 2*energy_every, ...

@@ -4,7 +4,7 @@
 // Step kernel (k_heat_multi): K time steps per pass over the grid.  A
 // wavefront owns a strip of columns and sweeps a band of rows top to bottom,
 // keeping two rows per time level in registers, so each pass reads the grid
-// once (plus 2K halo rows per band and one overlapping lane per strip edge)
+// once (plus 2K halo rows per band and two overlapping lanes per strip edge)
 // and writes it once.  Loads and stores are VEC-wide (16 or 32 bytes per
 // lane) and coalesced.  Workgroups are numbered so that strips adjacent in
 // the grid run on the same XCD (workgroups are dealt to the 8 XCDs
@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -46,6 +47,24 @@ __device__ __forceinline__ typename Vec<T>::type pack(const T *o) {
   return v;
 }
 
+// Whole-wavefront lane shifts by one (DPP wave_shr:1 / wave_shl:1): lane i
+// receives lane i-1's (shr) or lane i+1's (shl) value; the lane at the end
+// receives 0.  One VALU move per 32 bits instead of an LDS-crossbar
+// permute.  Only lanes 0 and 63 see the 0, and the stencil never stores them.
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+template <int kCtrl>
+__device__ __forceinline__ float lane_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, true));
+}
+template <int kCtrl>
+__device__ __forceinline__ double lane_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), kCtrl, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 struct HeatArgs {
   int64_t ny, nx;
   int64_t ty;        // rows per band
@@ -56,14 +75,14 @@ struct HeatArgs {
 };
 
 // ---------------------------------------------------------------------------
-// K time steps per pass (temporal blocking, 1 <= K <= VEC).
+// K time steps per pass (temporal blocking, 1 <= K <= kHalo * VEC).
 //
-// Columns: strips overlap.  A wavefront holds 64*VEC consecutive columns
-// starting VEC columns left of the 62*VEC columns it writes (lanes 1..62).
-// West/east neighbours come from the adjacent lane by a shuffle; lanes 0 and
-// 63 have no neighbour beyond the wavefront, so their values go wrong by one
-// column per level, and after K <= VEC levels only lanes 0 and 63 are wrong:
-// they are never stored.
+// Columns: strips overlap.  A wavefront holds 64*VEC consecutive columns and
+// writes the (64 - 2*kHalo)*VEC in the middle (lanes kHalo .. 63-kHalo).
+// West/east neighbours come from the adjacent lane (a DPP lane shift); the
+// edge lanes have no neighbour beyond the wavefront, so their values go
+// wrong by one column per level, and after K <= kHalo*VEC levels only the
+// kHalo lanes at each edge are wrong: they are never stored.
 // Rows: level l+1 of row y is computed from level-l rows y-1, y, y+1.  The
 // wavefront keeps, per level, the last two rows in registers; each loaded
 // level-0 row y+K yields level-1 row y+K-1, …, level-K row y.  The sweep
@@ -73,11 +92,11 @@ struct HeatArgs {
 // with the single-step formula, so K steps per pass are bit-identical to K
 // single steps.
 // ---------------------------------------------------------------------------
-template <typename T, int K, int kAhead>
+template <typename T, int K, int kAhead, int kHalo>
 __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict__ u, T *__restrict__ out,
                                                             HeatArgs a, T r, int vec_ok) {
   constexpr int V = Vec<T>::n;
-  static_assert(K >= 1 && K <= V, "K steps per pass need K <= VEC halo columns");
+  static_assert(K >= 1 && K <= kHalo * V, "K steps per pass need K <= kHalo * VEC halo columns");
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t b = blockIdx.x;
@@ -87,81 +106,95 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict_
   const int64_t strip = (lb % gpb) * kWaves + wave;
   const int64_t band = lb / gpb;
   if (strip >= a.nstrips) return;
-  const int64_t x = strip * 62 * V - V + (int64_t)lane * V;  // this lane's first column
+  const int64_t x = strip * (64 - 2 * kHalo) * V - kHalo * V + (int64_t)lane * V;  // this lane's first column
   const int64_t y0 = band * a.ty;
   const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
-  const bool store_lane = lane != 0 && lane != 63;
+  const bool store_lane = lane >= kHalo && lane < 64 - kHalo;
   const bool full = vec_ok && x >= 0 && x + V <= a.nx;  // whole aligned vector inside the row
   bool colok[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) colok[i] = x + i >= 0 && x + i < a.nx;
   const T four = T(4);
-  T prev[K][2][V];  // per level l: rows y+K-l-2 and y+K-l-1
+  // A wavefront whose every lane and every row it touches lies inside the
+  // grid runs the sweep without the edge masks (one select per value less).
+  const bool interior = vec_ok && x - (int64_t)lane * V >= 0 && x - (int64_t)lane * V + 64 * V <= a.nx &&
+                        y0 - 2 * K >= 0 && y1 + K <= a.ny;
+  auto sweep = [&](auto masked_tag) {
+    constexpr bool kMasked = decltype(masked_tag)::value;
+    T prev[K][2][V];  // per level l: rows y+K-l-2 and y+K-l-1
 #pragma unroll
-  for (int l = 0; l < K; ++l)
+    for (int l = 0; l < K; ++l)
 #pragma unroll
-    for (int i = 0; i < V; ++i) prev[l][0][i] = prev[l][1][i] = T(0);
-  for (int64_t y = y0 - 2 * K; y < y1; y += kAhead) {
-    T L[kAhead][V];  // level-0 rows y+K .. y+K+kAhead-1
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int64_t ry = y + k + K;
-      if (y + k < y1 && ry >= 0 && ry < a.ny) {
+      for (int i = 0; i < V; ++i) prev[l][0][i] = prev[l][1][i] = T(0);
+    // Level-0 rows are prefetched kAhead rows ahead through a ring of kAhead
+    // slots: each slot is refilled right after it is consumed, so kAhead row
+    // loads stay in flight all the time.
+    auto load_level0 = [&](int64_t ry, T *dst) {
+      if ((!kMasked || (ry >= 0 && ry < a.ny)) && ry < y1 + K) {
         const T *row = u + ry * a.nx;
-        if (full) {
-          unpack<T>(*(const typename Vec<T>::type *)(row + x), L[k]);
+        if (!kMasked || full) {
+          unpack<T>(*(const typename Vec<T>::type *)(row + x), dst);
         } else {
 #pragma unroll
-          for (int i = 0; i < V; ++i) L[k][i] = colok[i] ? row[x + i] : T(0);
+          for (int i = 0; i < V; ++i) dst[i] = colok[i] ? row[x + i] : T(0);
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < V; ++i) L[k][i] = T(0);
+        for (int i = 0; i < V; ++i) dst[i] = T(0);
+      }
+    };
+    T L[kAhead][V];  // slot k holds level-0 row y+k+K
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) load_level0(y0 - 2 * K + k + K, L[k]);
+    for (int64_t y = y0 - 2 * K; y < y1; y += kAhead) {
+#pragma unroll
+      for (int k = 0; k < kAhead; ++k) {
+        const int64_t yy = y + k;
+        if (yy >= y1) break;
+        T cur[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) cur[i] = L[k][i];
+        load_level0(yy + kAhead + K, L[k]);  // refill the slot kAhead rows ahead
+#pragma unroll
+        for (int l = 0; l < K; ++l) {
+          // level l+1 of row yy+K-l-1: north prev[l][0], centre prev[l][1], south cur
+          const int64_t row = yy + K - l - 1;
+          const bool rowok = !kMasked || (row >= 0 && row < a.ny);
+          const T wl = lane_shift<kDppWaveShr1>(prev[l][1][V - 1]);  // lane-1's last column
+          const T er = lane_shift<kDppWaveShl1>(prev[l][1][0]);      // lane+1's first column
+          T nxt[V];
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            const T c = prev[l][1][i];
+            const T w = i == 0 ? wl : prev[l][1][i - 1];
+            const T e = i == V - 1 ? er : prev[l][1][i + 1];
+            const T d = (((prev[l][0][i] - four * c) + cur[i]) + w) + e;
+            if (kMasked) nxt[i] = (rowok && colok[i]) ? c + d * r : T(0);
+            else nxt[i] = c + d * r;
+          }
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            prev[l][0][i] = prev[l][1][i];
+            prev[l][1][i] = cur[i];
+            cur[i] = nxt[i];
+          }
+        }
+        // cur: level K of row yy
+        if (yy >= y0 && store_lane) {
+          T *orow = out + yy * a.nx;
+          if (!kMasked || full) {
+            *(typename Vec<T>::type *)(orow + x) = pack<T>(cur);
+          } else {
+#pragma unroll
+            for (int i = 0; i < V; ++i)
+              if (colok[i]) orow[x + i] = cur[i];
+          }
+        }
       }
     }
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int64_t yy = y + k;
-      if (yy >= y1) break;
-      T cur[V];
-#pragma unroll
-      for (int i = 0; i < V; ++i) cur[i] = L[k][i];
-#pragma unroll
-      for (int l = 0; l < K; ++l) {
-        // level l+1 of row yy+K-l-1: north prev[l][0], centre prev[l][1], south cur
-        const int64_t row = yy + K - l - 1;
-        const bool rowok = row >= 0 && row < a.ny;
-        const T wl = __shfl_up(prev[l][1][V - 1], 1, 64);
-        const T er = __shfl_down(prev[l][1][0], 1, 64);
-        T nxt[V];
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-          const T c = prev[l][1][i];
-          const T w = i == 0 ? wl : prev[l][1][i - 1];
-          const T e = i == V - 1 ? er : prev[l][1][i + 1];
-          const T d = (((prev[l][0][i] - four * c) + cur[i]) + w) + e;
-          nxt[i] = (rowok && colok[i]) ? c + d * r : T(0);
-        }
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-          prev[l][0][i] = prev[l][1][i];
-          prev[l][1][i] = cur[i];
-          cur[i] = nxt[i];
-        }
-      }
-      // cur: level K of row yy
-      if (yy >= y0 && store_lane) {
-        T *orow = out + yy * a.nx;
-        if (full) {
-          *(typename Vec<T>::type *)(orow + x) = pack<T>(cur);
-        } else {
-#pragma unroll
-          for (int i = 0; i < V; ++i)
-            if (colok[i]) orow[x + i] = cur[i];
-        }
-      }
-    }
-  }
+  };
+  if (interior) sweep(std::false_type{});
+  else sweep(std::true_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -270,34 +303,49 @@ int fp64_sum(const TA *a, const TB *b, int64_t n, double *out) {
 
 template <typename T, int K>
 int launch_multi(const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int ahead) {
+  constexpr int H = 2;  // halo lanes on each side of a wavefront's strip
   const unsigned grid = (unsigned)(a.per_xcd * 8);
-  if (ahead >= 2)
-    hipLaunchKernelGGL((k_heat_multi<T, K, 2>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
-                       vec_ok);
+  if (ahead >= 3)
+    hipLaunchKernelGGL((k_heat_multi<T, K, 3, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a,
+                       r, vec_ok);
   else
-    hipLaunchKernelGGL((k_heat_multi<T, K, 1>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
-                       vec_ok);
+    hipLaunchKernelGGL((k_heat_multi<T, K, 2, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a,
+                       r, vec_ok);
   return ph::check_launch("k_heat_multi");
+}
+
+template <typename T>
+int launch_k(int k, const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int ahead) {
+  switch (k) {
+    case 8: return launch_multi<T, 8>(cur, nxt, a, r, vec_ok, ahead);
+    case 7: return launch_multi<T, 7>(cur, nxt, a, r, vec_ok, ahead);
+    case 6: return launch_multi<T, 6>(cur, nxt, a, r, vec_ok, ahead);
+    case 5: return launch_multi<T, 5>(cur, nxt, a, r, vec_ok, ahead);
+    case 4: return launch_multi<T, 4>(cur, nxt, a, r, vec_ok, ahead);
+    case 3: return launch_multi<T, 3>(cur, nxt, a, r, vec_ok, ahead);
+    case 2: return launch_multi<T, 2>(cur, nxt, a, r, vec_ok, ahead);
+    default: return launch_multi<T, 1>(cur, nxt, a, r, vec_ok, ahead);
+  }
 }
 
 template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
              double *energy) {
   constexpr int V = Vec<T>::n;
-  // Steps per pass, band height and row prefetch depth, per precision
-  // (measured on MI355X at 8192^2: 4 steps; FP64 32-row bands, FP32 64-row
-  // bands; 2 rows loaded ahead); PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD
-  // override them for tuning.
-  int64_t kmax = V;
-  int64_t ty = sizeof(T) == 4 ? 64 : 32;
-  int64_t ahead = 2;
-  if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(V, std::max<int64_t>(1, std::atol(e)));
+  // Steps per pass, band height and row prefetch depth (measured on MI355X
+  // at 8192^2, both precisions: 8 steps, 64-row bands, 3 rows loaded
+  // ahead); PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD override them.
+  int64_t kmax = 2 * V;
+  int64_t ty = 64;
+  int64_t ahead = 3;
+  if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(2 * V, std::max<int64_t>(1, std::atol(e)));
   if (const char *e = std::getenv("PH_HEAT_TY")) ty = std::max<int64_t>(8, std::atol(e));
   if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead = std::atol(f);
   HeatArgs a;
   a.ny = ny;
   a.nx = nx;
-  a.nstrips = (nx + 62 * V - 1) / (62 * V);  // 62 of a wavefront's 64 lanes are written
+  const int64_t wcols = (64 - 2 * 2) * V;  // columns a wavefront writes (2 halo lanes per side)
+  a.nstrips = (nx + wcols - 1) / wcols;
   a.ty = ty;
   a.nbands = (ny + a.ty - 1) / a.ty;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
@@ -322,10 +370,7 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     const int k = (int)std::min<int64_t>(kmax, until - t);
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
-    if (k == 4) rc = launch_multi<T, 4>(cur, nxt, a, rr, vec_ok, (int)ahead);
-    else if (k == 3) rc = launch_multi<T, 3>(cur, nxt, a, rr, vec_ok, (int)ahead);
-    else if (k == 2) rc = launch_multi<T, 2>(cur, nxt, a, rr, vec_ok, (int)ahead);
-    else rc = launch_multi<T, 1>(cur, nxt, a, rr, vec_ok, (int)ahead);
+    rc = launch_k<T>(k, cur, nxt, a, rr, vec_ok, (int)ahead);
     if (rc) return rc;
     T *sw = cur;
     cur = nxt;
