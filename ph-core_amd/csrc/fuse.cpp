@@ -506,16 +506,17 @@ template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, 
   const F d = (ph_fabs(a.s) >= ph_fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
   a.c += d; a.s = t; a.m = (v > a.m) ? v : a.m;
 }
-// Transposing butterfly over a wavefront: each lane holds 16 partial values
-// (one per row of a 16-row group); after the fold, lane l holds the total over
-// all 64 lanes for row (l >> 2) & 15.  17 cross-lane exchanges for 16 rows.
 __device__ __forceinline__ float ph_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double ph_fmax(double a, double b) { return __builtin_fmax(a, b); }
 template <typename F, bool MAX> __device__ __forceinline__ F ph_comb2(F a, F b) {
   if (MAX) return ph_fmax(a, b);
   return a + b;
 }
-// As tfold16 for 8 rows: lane l ends with the total for row (l >> 3) & 7.
+// Transposing butterfly over a wavefront: each lane holds 8 partial values
+// (one per row of an 8-row group); at each exchange a lane keeps half of its
+// rows and sends the other half, so after the fold lane l holds the total
+// over all 64 lanes for row (l >> 3) & 7.  7 cross-lane exchanges for 8 rows
+// (a plain butterfly per row would take 48).
 template <typename F, bool MAX> __device__ __forceinline__ F tfold8(F (&v)[8]) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -531,24 +532,6 @@ template <typename F, bool MAX> __device__ __forceinline__ F tfold8(F (&v)[8]) {
   }
   F x = v[0];
   x = ph_comb2<F, MAX>(x, __shfl_xor(x, 4, 64));
-  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
-  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
-  return x;
-}
-template <typename F, bool MAX> __device__ __forceinline__ F tfold16(F (&v)[16]) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int half = 8, m = 32; half >= 1; half >>= 1, m >>= 1) {
-    const bool b = (lane & m) != 0;
-#pragma unroll
-    for (int i = 0; i < half; ++i) {
-      const F keep = b ? v[i + half] : v[i];
-      const F send = b ? v[i] : v[i + half];
-      const F recv = __shfl_xor(send, m, 64);
-      v[i] = ph_comb2<F, MAX>(keep, recv);
-    }
-  }
-  F x = v[0];
   x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
   x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
   return x;
@@ -711,8 +694,15 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += "            if (" + ccheck("c") + ") {\n";
   s += body;
   s += fmt("              const F val = (F)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
-  if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], mval);\n";
-  if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, mval);\n";
+  if (full_rows && full_cols) {
+    // no bounds checks: the first row / column of a chunk starts the maximum
+    // (as Enumerable#max starts from the first element)
+    if (P.do_cols) s += "              cp[v] += val; cm[v] = k == 0 ? mval : ph_fmax(cm[v], mval);\n";
+    if (P.do_rows) s += "              ls += val; lm = v == 0 ? mval : ph_fmax(lm, mval);\n";
+  } else {
+    if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], mval);\n";
+    if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, mval);\n";
+  }
   s += "            }\n          }\n        }\n";
   if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
   s += "      }\n";
@@ -900,8 +890,8 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   }
   P.ntc = (P.cols + 1023) / 1024;
   // rows per tile: aim for ~512 workgroups (fewer, taller tiles write fewer
-  // column partial states; measured best on 16384^2), a multiple of 16 (rows
-  // are folded 8 and 16 at a time), at most 1024 rows (256 for integer
+  // column partial states; measured best on 16384^2), a multiple of 16 (the
+  // float tile folds rows 8 at a time), at most 1024 rows (256 for integer
   // states, whose per-row LDS states are larger)
   int64_t tr = P.rows * P.ntc / 512;
   if (const char *e = std::getenv("PH_RED_TR")) tr = std::atol(e);  // tuning override
