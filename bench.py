@@ -137,6 +137,65 @@ def cpu_baseline(xn, lits_parsed, n, alg_bytes):
                       f"{dt * 1e3:.1f} ms/step"}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    return O
+
+
+def cpu_c1(a, b, c):
+    """ph-core's (a + b.tile(1024, 1)) * c restated by the oracle, 1 core."""
+    import numpy as np
+    O = _oracle()
+    bt = np.tile(b, (a.shape[0], 1))
+    reps, t0 = 0, time.perf_counter()
+    while reps < 5 or time.perf_counter() - t0 < 2.0:
+        t, _ = O.eval_expr("(add x0 x1)", [a, bt])
+        O.eval_expr("(mul x0 x1)", [t, c])
+        reps += 1
+        if reps >= 50:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    by = 24 * a.size + 8 * b.size
+    return {"value": round(by / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} evaluations of the two elementwise ops at 1024^2 (tile materialised), {dt * 1e3:.1f} ms each"}
+
+
+def cpu_c3(x):
+    """ph-core's ProcView block, then sum / max in full and per row / column
+    (slices), restated by the oracle on the first 512 rows, 1 core."""
+    import numpy as np
+    from crystal import trace
+    O = _oracle()
+    rows = 512
+    xs = np.ascontiguousarray(x[:rows])
+    t0 = time.perf_counter()
+    f, _ = O.eval_expr(trace(lambda v: (v + 4) ** 2 * 0.5 - v), [xs])
+    O.sum_seq(f)
+    O.max_seq(f)
+    O.slices_reduce(f, 0, 0)
+    O.slices_reduce(f, 0, 1)
+    O.slices_reduce(f, 1, 0)
+    O.slices_reduce(f, 1, 1)
+    dt = time.perf_counter() - t0
+    by = 4 * xs.size
+    return {"value": round(by / dt / 1e9, 3), "unit": "GB/s (input bytes)", "cores": 1, "kind": "port",
+            "sample": f"the first {rows} of 16384 rows: block evaluation + sequential sum, max and the four "
+                      f"slice reductions, {dt * 1e3:.0f} ms; about {dt * 16384 / rows:.1f} s for the full array"}
+
+
+def cpu_c4(u0):
+    """The heat step restated by the oracle on the full 8192^2 FP64 grid for
+    2 steps, 1 core; Gcell/s."""
+    O = _oracle()
+    steps = 2
+    t0 = time.perf_counter()
+    O.heat2d(u0, 0.2, steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(u0.size * steps / dt / 1e9, 4), "unit": "Gcell/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} FP64 steps of the full 8192^2 grid, {dt:.2f} s"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -281,6 +340,31 @@ def other_workloads(phase, ev, args):
     tr = load_traffic() or {}
     out["c1_broadcast_f64_16384"]["fused"]["traffic"] = tr.get("ph_map_fused_c1")
 
+    # C1 at its parity shape, 1024^2 FP64 (25 MB: cache-resident, launch-bound)
+    m = 1024
+    a = phase.NArray.random([m, m], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
+    b = phase.NArray.random([1, m], D.F64, seed=0, offset=m * m, a=-1.0, b=1.0)
+    c = phase.NArray.random([m, m], D.F64, seed=0, offset=m * m + m, a=-1.0, b=1.0)
+    bt = b.tile(m, 1)
+    f1 = lambda: (a + bt) * c  # noqa: E731
+    f1()
+    ms1 = time_loop(ev, f1, 50, phase)
+    out["c1_broadcast_f64_1024"] = {"ms": round(ms1, 4), "bytes": 24 * m * m + 8 * m,
+                                    "achieved_gbs": round((24 * m * m + 8 * m) / ms1 / 1e6, 1),
+                                    "note": "eager (a + b.tile) * c, two kernels; inputs fit the 256 MB MALL"}
+    if not args.no_cpu:
+        out["c1_broadcast_f64_1024"]["cpu"] = cpu_c1(a.to_numpy(), b.to_numpy(), c.to_numpy())
+    del a, b, c, bt
+
+    # the device's copy rate for a working set well beyond the MALL (2 GiB)
+    big = phase.NArray.random([1 << 29], D.I32, seed=9, dist="int31")
+    big.clone()
+    msb = time_loop(ev, lambda: big.clone(), 5, phase)
+    out["copy_2gib"] = {"kernel": "k_copy_rows (contiguous)", "bytes": 2 << 31, "ms": round(msb, 4),
+                        "achieved_gbs": round((2 << 31) / msb / 1e6, 1),
+                        "note": "practical HBM copy ceiling for multi-GB working sets"}
+    del big
+
     # C3 fused ProcView + six reductions (F32 and BF16 storage), one pass
     n = 16384
     for st, esz in [("F32", 4), ("BF16", 2)]:
@@ -289,6 +373,8 @@ def other_workloads(phase, ev, args):
         pv.reduce2d_all_device()
         ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10, phase)
         by = esz * n * n
+        if not args.no_cpu and st == "F32":
+            out["c3_cpu"] = cpu_c3(x.to_numpy())
         out[f"c3_procview_reduce_{st.lower()}_16384"] = {
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
             "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
@@ -342,6 +428,8 @@ def other_workloads(phase, ev, args):
                            "traffic_per_launch_of_four_steps": {"f64": tr.get("k_heat_multi_f64"),
                                                                 "f32": tr.get("k_heat_multi_f32")}}
     del fields, u
+    if not args.no_cpu:
+        out["c4_heat_8192"]["cpu"] = cpu_c4(u0)
     out["c5_batched_20000"] = batched_workload(phase, ev, args)
 
     # NArray.concatenate of 4 x [4096, 4096] Float32 along axis 1 (one
