@@ -1,0 +1,131 @@
+"""Edge cases of the device path against the oracle: tiny and ragged heat
+grids (every steps-per-pass remainder), reductions over ragged shapes of
+every element type and axis, Int32 running-sum overflow, empty inputs, and
+arrays beyond 2^31 elements (64-bit indexing)."""
+import numpy as np
+import pytest
+
+import oracle_api as O
+from crystal import parse_literal
+
+
+def O_lit(lit):
+    """(first, last, step, exclusive) of a range literal."""
+    return parse_literal(lit)[1:]
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_heat_tiny_and_ragged_grids(ph, dt):
+    rng = np.random.default_rng(17)
+    for ny, nx in [(1, 1), (1, 7), (7, 1), (2, 3), (3, 1030), (130, 5), (65, 257), (200, 249)]:
+        for steps in (1, 3, 9, 17):
+            u0 = rng.random((ny, nx)).astype(dt)
+            u = ph.NArray.from_numpy(u0)
+            e = ph.heat2d(u, 0.2, steps, every=4)
+            ref, eref = O.heat2d(u0, 0.2, steps, every=4)
+            assert np.array_equal(u.to_numpy(), ref), (ny, nx, steps)
+            assert np.allclose(e, eref, rtol=1e-13, atol=1e-300), (ny, nx, steps)
+
+
+def _device(ph, a, storage):
+    """The device array and the values it holds (BF16: the truncated bits)."""
+    if storage == "BF16":
+        bits = (a.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+        return ph.NArray.from_numpy(bits, ph.DType.BF16), (bits.astype(np.uint32) << 16).view(np.float32)
+    return ph.NArray.from_numpy(a), a
+
+
+@pytest.mark.parametrize("storage", ["F64", "F32", "BF16", "I32"])
+def test_reductions_ragged_shapes(ph, storage):
+    rng = np.random.default_rng(23)
+    np_dt = {"F64": np.float64, "F32": np.float32, "BF16": np.float32, "I32": np.int32}[storage]
+    for shape in [(1,), (5,), (1, 1), (1, 1027), (1027, 1), (513, 1029), (3, 5, 7), (2, 1, 1030), (17, 3, 2, 9)]:
+        if storage == "I32":
+            a = rng.integers(-1000, 1000, shape).astype(np.int32)
+        else:
+            a = rng.standard_normal(shape).astype(np_dt)
+        d, host = _device(ph, a, storage)
+        h64 = host.astype(np.float64)
+        # full sum and max
+        s = d.sum()
+        if storage == "I32":
+            assert s == int(a.astype(np.int64).sum())
+        else:
+            ex = O.sum_exact(h64)
+            assert abs(s - ex) <= 1e-6 * np.abs(h64).sum() + 1e-30, (shape, s, ex)
+        assert float(d.max()) == float(h64.max()), shape
+        # slices(k).map(&.sum) / (&.max) along every axis
+        for ax in range(len(shape)):
+            others = tuple(i for i in range(len(shape)) if i != ax)
+            ss = np.array(d.slices_sum(ax))
+            sm = np.array(d.slices_max(ax))
+            assert np.array_equal(sm, h64.max(axis=others) if others else h64), (shape, ax)
+            if storage == "I32":
+                assert np.array_equal(ss, h64.sum(axis=others) if others else h64), (shape, ax)
+            else:
+                want = np.array([O.sum_exact(np.take(h64, j, axis=ax)) for j in range(shape[ax])])
+                scale = np.array([np.abs(np.take(h64, j, axis=ax)).sum() for j in range(shape[ax])])
+                assert np.all(np.abs(ss - want) <= 1e-6 * scale + 1e-30), (shape, ax)
+
+
+def test_int32_running_sum_overflow(ph):
+    """Enumerable#sum with checked Int32 `+` raises as soon as the running sum
+    leaves the Int32 range, even if later terms bring it back."""
+    big = 2 ** 31 - 1
+    ok = ph.NArray.from_numpy(np.array([big - 5, 5, -7], np.int32))
+    assert ok.sum() == big - 7
+    back = ph.NArray.from_numpy(np.array([big - 5, 6, -7], np.int32))  # crosses, then returns
+    with pytest.raises(ph.OverflowError):
+        back.sum()
+    low = ph.NArray.from_numpy(np.array([-2 ** 31, 1, -1, -1], np.int32))
+    with pytest.raises(ph.OverflowError):
+        low.sum()
+    # across a tile boundary of a 2-D reduction
+    a = np.zeros((40, 3000), np.int32)
+    a[0, 0] = big
+    a[20, 2999] = 1
+    a[39, 0] = -5
+    with pytest.raises(ph.OverflowError):
+        ph.NArray.from_numpy(a).sum()
+    a[20, 2999] = -1
+    assert ph.NArray.from_numpy(a).sum() == big - 6
+
+
+def test_empty_inputs(ph):
+    e = ph.NArray.fill([0, 5], 1.5)
+    assert (e + 1.0).to_numpy().shape == (0, 5)
+    assert e.sum() == 0.0
+    with pytest.raises(ph.EmptyError):
+        e.max()
+    assert list(e.slices_sum(1)) == [0.0] * 5
+    # any range over an empty axis raises, as canonicalize_range does
+    # (range_syntax.cr:145-150); the whole-array view is empty
+    for lit in ["..", "0...0"]:
+        with pytest.raises(O.OracleError):
+            O.canonicalize_range(("range",) + tuple(O_lit(lit)), 0)
+        with pytest.raises(IndexError):
+            e.view(lit, "1..2")
+    assert e.view().to_narr().size == 0 and e.view().to_narr().shape == [0, 5]
+    z = ph.NArray.fill([0], 0)
+    assert z.sum() == 0
+    h = ph.NArray.fill([4, 4], 1.0)
+    assert np.array_equal(np.array(ph.heat2d(h, 0.2, 0, every=1)), [16.0])  # zero steps: energy only
+
+
+def test_beyond_2_31_elements(ph):
+    """A 2^31 + 4100-element Int32 array (8.6 GB): strided gather and scatter
+    with offsets past 2^31 take the 64-bit index path."""
+    n = 2 ** 31 + 4100
+    x = ph.NArray.random([n], ph.DType.I32, seed=1, dist="int31")
+    xh = x.to_numpy()
+    g = x.view("2147480000..3..").to_narr().to_numpy()
+    assert np.array_equal(g, xh[2147480000::3])
+    tail = ph.NArray.from_numpy(np.arange(1000, dtype=np.int32))
+    x[["-2000..2.."]] = tail
+    got = x.to_numpy()
+    exp = xh.copy()
+    exp[n - 2000::2] = np.arange(1000, dtype=np.int32)
+    assert np.array_equal(got[n - 4096:], exp[n - 4096:]) and np.array_equal(got[:4096], exp[:4096])
+    assert x.view("-3..").to_narr().to_numpy().tolist() == exp[-3:].tolist()
