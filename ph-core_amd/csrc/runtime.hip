@@ -206,6 +206,7 @@ int ph_init(int device) {
 
 int ph_device_count(int *n) {
   PH_GUARD();
+  if (!n) return ph::set_error(PH_E_ARGUMENT, "ph_device_count: null output");
   *n = 0;
   hipError_t e = hipGetDeviceCount(n);
   if (e != hipSuccess) {
