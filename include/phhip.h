@@ -279,9 +279,11 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins,
  * keep_axis = k : shape[k] results, result j reducing the slice with
  *   coordinate k fixed at j (`slices(k).map(&.sum)`, src/multi_indexable.cr:
  *   742-786).
- * ops: bit 0 sum, bit 1 max.  acc: accumulation mode for float values,
- *   PH_ACC_F32C (FP32 Neumaier-compensated), PH_ACC_F64 (FP64 Neumaier-
- *   compensated).  Int32/Bool values are summed exactly in 64 bits; an Int32
+ * ops: bit 0 sum, bit 1 max.  acc: accumulation mode for float values.
+ *   Each tile of at most 1024 rows sums its values in FP64; the tiles'
+ *   partial sums are carried as compensated pairs and combined with
+ *   error-free TwoSum, pairs of floats under PH_ACC_F32C and of doubles
+ *   under PH_ACC_F64.  Int32/Bool values are summed exactly in 64 bits; an Int32
  *   sum whose sequential (lexicographic) running total ever leaves the Int32
  *   range sets PH_FLAG_OVERFLOW, as Crystal's checked `+` would raise.
  * out_sum / out_max: device arrays of doubles (count results each; may be

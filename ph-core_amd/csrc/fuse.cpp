@@ -12,15 +12,17 @@
 // tile0 + e*256 + t (coalesced for every unrolled e); all loads of a tile
 // are issued before any arithmetic.
 //
-// Reduction kernel (2-D view rows x cols): a workgroup owns TR rows x 1024
-// columns; wavefront w owns columns [256w, 256w+256), lane l owns VEC=4
-// consecutive columns.  Per element the program's value feeds (a) the
-// thread's column states, accumulated row after row, and (b) a per-row lane
-// state that an ordered shuffle tree folds into one state per wavefront and
-// row.  Second pass: per-row and per-column states are folded across tiles in
-// tile order, then the full result across rows in row order.  Every fold is
-// in lexicographic order, so Int32 running-sum overflow is detected exactly as
-// a sequential checked sum would, and every result is deterministic.
+// Reduction kernel (2-D view rows x cols): a workgroup owns TR rows x
+// 256*VEC columns; wavefront w owns columns [64w*VEC, 64(w+1)*VEC), lane l
+// owns VEC consecutive columns (VEC = 4, or 8 for BFloat16 and Bool inputs).
+// Per element the program's value feeds (a) the thread's column sums and
+// maxima, accumulated row after row, and (b) a per-row lane value that a
+// fixed cross-lane tree folds into one value per wavefront and row.  Second
+// pass: per-row and per-column states are folded across tiles in tile order,
+// then the full result across rows in row order.  Integer folds are in
+// lexicographic order, so Int32 running-sum overflow is detected exactly as a
+// sequential checked sum would; every result is deterministic.
+#include <algorithm>
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
@@ -501,48 +503,104 @@ struct SI { i64 s, hi, lo; int m; int n; };
 __device__ __forceinline__ void st_init(SF &a) { a.s = 0.f; a.c = 0.f; a.m = -__builtin_inf(); }
 __device__ __forceinline__ void st_init(SD &a) { a.s = 0.0; a.c = 0.0; a.m = -__builtin_inf(); }
 __device__ __forceinline__ void st_init(SI &a) { a.s = 0; a.hi = -(1LL << 62); a.lo = (1LL << 62); a.m = 0; a.n = 0; }
-template <typename S, typename F> __device__ __forceinline__ void st_addf(S &a, F v) {
-  const F t = a.s + v;
-  const F d = (ph_fabs(a.s) >= ph_fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
-  a.c += d; a.s = t; a.m = (v > a.m) ? v : a.m;
+// A state from an FP64 partial sum and maximum: SF keeps the sum as an
+// unevaluated pair of floats (s + c, about 48 significant bits).
+__device__ __forceinline__ void st_from(SF &a, double s, double m) { a.s = (float)s; a.c = (float)(s - (double)a.s); a.m = m; }
+__device__ __forceinline__ void st_from(SD &a, double s, double m) { a.s = s; a.c = 0.0; a.m = m; }
+// V consecutive elements of one row (address aligned to min(V * size, 16)
+// bytes), loaded with 16-byte (or, for fewer bytes, one) vector loads.
+template <int V> __device__ __forceinline__ void ph_ldv(const float *p, float (&x)[V]) {
+#pragma unroll
+  for (int q = 0; q < V / 4; ++q) { const float4 t = ((const float4 *)p)[q]; x[4 * q] = t.x; x[4 * q + 1] = t.y; x[4 * q + 2] = t.z; x[4 * q + 3] = t.w; }
 }
+template <int V> __device__ __forceinline__ void ph_ldv(const double *p, double (&x)[V]) {
+#pragma unroll
+  for (int q = 0; q < V / 2; ++q) { const double2 t = ((const double2 *)p)[q]; x[2 * q] = t.x; x[2 * q + 1] = t.y; }
+}
+template <int V> __device__ __forceinline__ void ph_ldv(const int *p, int (&x)[V]) {
+#pragma unroll
+  for (int q = 0; q < V / 4; ++q) { const int4 t = ((const int4 *)p)[q]; x[4 * q] = t.x; x[4 * q + 1] = t.y; x[4 * q + 2] = t.z; x[4 * q + 3] = t.w; }
+}
+// BFloat16 and Bool elements stay packed in 32-bit words (2 and 4 per word,
+// the lower address in the low bits) until they are used.
+template <int V> __device__ __forceinline__ void ph_ldv(const unsigned short *p, unsigned (&w)[V / 2]) {
+  if constexpr (V == 8) { const uint4 t = *(const uint4 *)p; w[0] = t.x; w[1] = t.y; w[2] = t.z; w[3] = t.w; }
+  else { const uint2 t = *(const uint2 *)p; w[0] = t.x; w[1] = t.y; }
+}
+template <int V> __device__ __forceinline__ void ph_ldv(const unsigned char *p, unsigned (&w)[V / 4]) {
+  if constexpr (V == 8) { const uint2 t = *(const uint2 *)p; w[0] = t.x; w[1] = t.y; }
+  else { w[0] = *(const unsigned *)p; }
+}
+__device__ __forceinline__ float ph_bf(unsigned w, int h) { return __uint_as_float(h ? (w & 0xffff0000u) : (w << 16)); }
+__device__ __forceinline__ bool ph_bb(unsigned w, int b) { return ((w >> (8 * b)) & 0xffu) != 0; }
 __device__ __forceinline__ float ph_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double ph_fmax(double a, double b) { return __builtin_fmax(a, b); }
 template <typename F, bool MAX> __device__ __forceinline__ F ph_comb2(F a, F b) {
   if (MAX) return ph_fmax(a, b);
   return a + b;
 }
+// Cross-lane moves.  v_permlane32_swap(a, b) exchanges lanes 32-63 of a with
+// lanes 0-31 of b, v_permlane16_swap(a, b) the odd 16-lane rows of a with the
+// even rows of b: with a = the lane's value for one row and b = its value for
+// another, each lane ends up holding its own value and its partner's for the
+// row it keeps, with no selects.  DPP moves serve the exchanges inside a row.
+__device__ __forceinline__ void ph_swap32(float &a, float &b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]); b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void ph_swap16(float &a, float &b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]); b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void ph_swap32(double &a, double &b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a), ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void ph_swap16(double &a, double &b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a), ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+// DPP controls: row_ror:8 (lane l reads l ^ 8 in its 16-lane row),
+// row_half_mirror (l reads 7 - l in its 8-lane half row), quad_perm [2,3,0,1]
+// (l ^ 2) and [1,0,3,2] (l ^ 1).
+template <int C> __device__ __forceinline__ float ph_dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), C, 0xF, 0xF, true));
+}
+template <int C> __device__ __forceinline__ double ph_dpp(double x) {
+  const long long u = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, C, 0xF, 0xF, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)((unsigned long long)u >> 32), C, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // Transposing butterfly over a wavefront: each lane holds 8 partial values
 // (one per row of an 8-row group); at each exchange a lane keeps half of its
-// rows and sends the other half, so after the fold lane l holds the total
-// over all 64 lanes for row (l >> 3) & 7.  7 cross-lane exchanges for 8 rows
-// (a plain butterfly per row would take 48).
+// rows and receives its partner's values for them, so after three exchanges
+// (lane bits 5, 4, 3) lane l holds 8 lanes' totals for row (l >> 3) & 7, and
+// a fixed three-level tree inside each 8-lane group (half mirror, then lane
+// bits 1 and 0) completes the row over all 64 lanes.
 template <typename F, bool MAX> __device__ __forceinline__ F tfold8(F (&v)[8]) {
   const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int half = 4, m = 32; half >= 1; half >>= 1, m >>= 1) {
-    const bool b = (lane & m) != 0;
-#pragma unroll
-    for (int i = 0; i < half; ++i) {
-      const F keep = b ? v[i + half] : v[i];
-      const F send = b ? v[i] : v[i + half];
-      const F recv = __shfl_xor(send, m, 64);
-      v[i] = ph_comb2<F, MAX>(keep, recv);
-    }
-  }
-  F x = v[0];
-  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 4, 64));
-  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 2, 64));
-  x = ph_comb2<F, MAX>(x, __shfl_xor(x, 1, 64));
+  F a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], b0 = v[4], b1 = v[5], b2 = v[6], b3 = v[7];
+  ph_swap32(a0, b0); ph_swap32(a1, b1); ph_swap32(a2, b2); ph_swap32(a3, b3);
+  // lanes 0-31: rows 0-3, lanes 32-63: rows 4-7
+  F w0 = ph_comb2<F, MAX>(a0, b0), w1 = ph_comb2<F, MAX>(a1, b1);
+  F w2 = ph_comb2<F, MAX>(a2, b2), w3 = ph_comb2<F, MAX>(a3, b3);
+  ph_swap16(w0, w2); ph_swap16(w1, w3);
+  const F y0 = ph_comb2<F, MAX>(w0, w2), y1 = ph_comb2<F, MAX>(w1, w3);
+  const bool b8 = (lane & 8) != 0;
+  const F keep = b8 ? y1 : y0, send = b8 ? y0 : y1;
+  F x = ph_comb2<F, MAX>(keep, ph_dpp<0x128>(send));
+  x = ph_comb2<F, MAX>(x, ph_dpp<0x141>(x));
+  x = ph_comb2<F, MAX>(x, ph_dpp<0x4E>(x));
+  x = ph_comb2<F, MAX>(x, ph_dpp<0xB1>(x));
   return x;
 }
-template <typename S, typename F> __device__ __forceinline__ void st_addsum(S &a, F v) {
-  const F t = a.s + v;
-  const F d = (ph_fabs(a.s) >= ph_fabs(v)) ? ((a.s - t) + v) : ((v - t) + a.s);
-  a.c += d; a.s = t;
-}
-__device__ __forceinline__ void st_add(SF &a, float v) { st_addf<SF, float>(a, v); }
-__device__ __forceinline__ void st_add(SD &a, double v) { st_addf<SD, double>(a, v); }
 __device__ __forceinline__ void st_add(SI &a, i64 v) {
   a.s += v; a.hi = a.s > a.hi ? a.s : a.hi; a.lo = a.s < a.lo ? a.s : a.lo;
   a.m = (a.n == 0 || (int)v > a.m) ? (int)v : a.m; a.n = 1;
@@ -615,31 +673,120 @@ struct RedPlan {
   bool vec_ok[PH_MAX_INPUTS];  // whole-vector loads of the lane's VEC columns
 };
 
-// Tile kernel for floating-point accumulation (states SF / SD).  Column
-// states: Neumaier-compensated, one per thread column; each chunk of four
-// rows is summed plainly first (a pairwise level) and then added with
-// compensation.  Row partials: the lane's VEC values summed, then 8 rows at a
-// time folded across the wavefront by a transposing butterfly (pairwise
-// summation within a 256-column chunk); the four wavefronts of the workgroup
-// and the column tiles are then combined with error-free TwoSum.  Loads for
-// four rows are issued before any of them is used.  Bounds checks are emitted
-// only when the shape is not a whole number of tiles, so the interior code is
-// free of predication.
+// Tile kernel for floating-point accumulation (states SF / SD).  A
+// workgroup owns TR rows x 256*VEC columns; lane l of wavefront w owns VEC
+// consecutive columns.  The value of every element is summed in FP64 twice:
+// into the lane's column sums (one per column, running over the tile's rows)
+// and into a per-row lane sum over its VEC columns.  Row sums of 8 rows at a
+// time are folded across the wavefront by a transposing butterfly; the four
+// wavefronts of the workgroup and, in the finish kernel, the tiles are then
+// combined as compensated states with error-free TwoSum.  Maxima are taken in
+// the value's own type (exact).  Rows are processed in groups of 8: the loads
+// of the next group are issued before the current one is computed (two
+// register buffers, alternating).  Bounds checks are emitted only when the
+// shape is not a whole number of tiles, so the interior code is free of
+// predication.
 std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
                            const int64_t *cs, const RedPlan &P) {
   const int V = P.vec;
   const bool sd = std::string(P.state) == "SD";
-  const char *F = sd ? "double" : "float";
-  // maxima are taken in the value's own type (exact), sums in F
   const char *M = (sd || p->out_dtype == PH_F64) ? "double" : "float";
   const bool full_cols = P.cols % (256 * V) == 0;
   const bool full_rows = P.rows % P.tr == 0;
   const std::string C = lit_i64(P.cols);
   auto ccheck = [&](const std::string &expr) { return full_cols ? std::string("true") : "(" + expr + " < " + C + ")"; };
   const std::string rcheck = full_rows ? "true" : "(r < r1)";
+  // loads of the 8 rows starting at row `g` into buffer `b`
+  // (rows past the tile's end read its last row again: every group's loads
+  // are issued unconditionally, so no branch separates them from the wait for
+  // the previous group's data)
+  auto loads = [&](const std::string &b, const std::string &g) {
+    std::string o;
+    o += "#pragma unroll\n    for (int k = 0; k < 8; ++k) {\n      const i64 r = (" + g + " + k < r1) ? " + g + " + k : r1 - 1;\n";
+    o += "      {\n";
+    for (int j = 0; j < n_in; ++j) {
+      o += fmt("        const %s *rp%d = (const %s *)A.in[%d] + (A.in_off[%d] + r * %s);\n", stype(in_dt[j]), j,
+               stype(in_dt[j]), j, j, lit_i64(rs[j]).c_str());
+      if (P.vec_ok[j]) {
+        // the row's address is uniform, the lane's byte offset 32-bit
+        o += "        if (" + ccheck(fmt("col0 + %d", V - 1)) +
+             fmt(") ph_ldv<%d>((const %s *)((const char *)(rp%d + ct * %d) + lane_b%d), %s%d[k]);\n", V, stype(in_dt[j]), j,
+                 256 * V, j, b.c_str(), j);
+        if (full_cols) continue;
+        o += "        else\n";
+      }
+      const std::string CS = lit_i64(cs[j]);
+      if (in_dt[j] == PH_BF16 || in_dt[j] == PH_BOOL) {
+        // element by element into the packed words (absent columns read 0)
+        const int per = in_dt[j] == PH_BF16 ? 2 : 4, bits = in_dt[j] == PH_BF16 ? 16 : 8;
+        o += fmt("#pragma unroll\n        for (int q = 0; q < %d; ++q) {\n          unsigned w = 0;\n", V / per);
+        o += fmt("#pragma unroll\n          for (int e = 0; e < %d; ++e) {\n            const i64 c = col0 + q * %d + e;\n", per, per);
+        o += "            if (" + ccheck("c") + fmt(") w |= (unsigned)rp%d[c * %s] << (%d * e);\n", j, CS.c_str(), bits);
+        o += fmt("          }\n          %s%d[k][q] = w;\n        }\n", b.c_str(), j);
+        continue;
+      }
+      o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = col0 + v;\n", V);
+      o += "          if (" + ccheck("c") + fmt(") %s%d[k][v] = ph_ld(rp%d, c * %s);\n", b.c_str(), j, j, CS.c_str());
+      o += "        }\n";
+    }
+    o += "      }\n    }\n";
+    return o;
+  };
+  // the 8 rows at row `g` held in buffer `b`
+  auto compute = [&](const std::string &b, const std::string &g) {
+    std::vector<std::string> inexpr;
+    for (int j = 0; j < n_in; ++j) {
+      if (in_dt[j] == PH_BF16)
+        inexpr.push_back(fmt("ph_bf(%s%d[k][v >> 1], v & 1)", b.c_str(), j));
+      else if (in_dt[j] == PH_BOOL)
+        inexpr.push_back(fmt("ph_bb(%s%d[k][v >> 2], v & 3)", b.c_str(), j));
+      else
+        inexpr.push_back(fmt("%s%d[k][v]", b.c_str(), j));
+    }
+    std::string body;
+    const std::string res = emit_prog(p, inexpr, &body, "");
+    std::string o = "    {\n";
+    if (P.do_rows) o += "    double rsv[8]; M rmv[8];\n";
+    o += "#pragma unroll\n    for (int k = 0; k < 8; ++k) {\n      const i64 r = " + g + " + k;\n";
+    o += "      double ls = 0.0; M lm = -__builtin_inf();\n";
+    o += "      if (" + rcheck + ") {\n";
+    o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = col0 + v;\n", V);
+    o += "          if (" + ccheck("c") + ") {\n";
+    o += body;
+    o += fmt("            const double val = (double)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
+    if (P.do_cols) o += "            csum[v] += val; cmax[v] = ph_fmax(cmax[v], mval);\n";
+    if (P.do_rows) {
+      // with every column present the lane's first column starts the row
+      // (the row's state still starts from +0.0 where it is combined)
+      if (full_cols)
+        o += "            ls = v == 0 ? val : ls + val; lm = v == 0 ? mval : ph_fmax(lm, mval);\n";
+      else
+        o += "            ls += val; lm = ph_fmax(lm, mval);\n";
+    }
+    o += "          }\n        }\n      }\n";
+    if (P.do_rows) o += "      rsv[k] = ls; rmv[k] = lm;\n";
+    if (P.do_cols) {
+      // The column sums and maxima are brought up to date row by row (sums
+      // every row, maxima every second row): an empty asm that takes the
+      // running value as in-out operand.  Left alone, the compiler defers
+      // the column chains to the end of the group and holds every element's
+      // value of 8 rows until then, which does not fit in registers.
+      o += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) __asm__ volatile(\"\" : \"+v\"(csum[v]));\n", V);
+      o += fmt("      if (k & 1) {\n#pragma unroll\n        for (int v = 0; v < %d; ++v) __asm__ volatile(\"\" : \"+v\"(cmax[v]));\n      }\n", V);
+    }
+    o += "    }\n";
+    if (P.do_rows) {
+      o += "    const double fs = tfold8<double, false>(rsv);\n    const M fm = tfold8<M, true>(rmv);\n";
+      o += "    const int rr = (lane >> 3) & 7;\n";
+      o += "    if ((lane & 7) == 0 && " + g + " + rr < r1) { rsum_sh[" + g + " - r0 + rr][wave] = fs; rmax_sh[" + g +
+           " - r0 + rr][wave] = fm; }\n";
+    }
+    o += "    }\n";
+    return o;
+  };
   std::string s;
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_tile(RedArgs A) {\n";
-  s += fmt("  typedef %s S; typedef %s F; typedef %s M;\n", P.state, F, M);
+  s += fmt("  typedef %s S; typedef %s M;\n", P.state, M);
   s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
            lit_i64(P.ntc).c_str());
   s += "  const int lane = threadIdx.x & 63; const int wave = threadIdx.x >> 6;\n";
@@ -648,82 +795,37 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
            lit_i64(P.rows).c_str());
   s += "  int fl = 0;\n";
-  s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
-  if (P.do_rows) s += fmt("  __shared__ F rsum_sh[%d][4]; __shared__ M rmax_sh[%d][4];\n", P.tr, P.tr);
-  s += "  for (i64 g0 = r0; g0 < r1; g0 += 8) {\n";
-  if (P.do_rows) s += "    F rsv[8]; M rmv[8];\n";
-  s += "#pragma unroll\n    for (int q = 0; q < 8; q += 4) {\n";
-  for (int j = 0; j < n_in; ++j) s += fmt("      %s x%d[4][%d];\n", ctype(compute_dt(in_dt[j])), j, V);
-  // loads of 4 rows
-  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
-  s += "        if (" + rcheck + ") {\n";
-  for (int j = 0; j < n_in; ++j) {
-    s += fmt("          const %s *rp%d = (const %s *)A.in[%d] + (A.in_off[%d] + r * %s);\n", stype(in_dt[j]), j,
-             stype(in_dt[j]), j, j, lit_i64(rs[j]).c_str());
-    if (P.vec_ok[j]) {
-      const char *vt = in_dt[j] == PH_F64 ? "double4" : in_dt[j] == PH_BF16 ? "ushort4"
-                       : in_dt[j] == PH_I32 ? "int4" : in_dt[j] == PH_BOOL ? "uchar4" : "float4";
-      s += "          if (" + ccheck(fmt("col0 + %d - 1", V)) + ") {\n";
-      s += fmt("            const %s t = *(const %s *)(rp%d + col0);\n", vt, vt, j);
-      s += fmt("            const %s tt[4] = {t.x, t.y, t.z, t.w};\n", stype(in_dt[j]));
-      s += fmt("#pragma unroll\n            for (int v = 0; v < 4; ++v) x%d[k][v] = ph_ld(tt, v);\n", j);
-      if (full_cols) {
-        s += "          }\n";
-        continue;
-      }
-      s += "          } else {\n";
-    } else {
-      s += "          {\n";
-    }
-    s += fmt("#pragma unroll\n            for (int v = 0; v < %d; ++v) {\n              const i64 c = col0 + v;\n", V);
-    s += "              if (" + ccheck("c") + fmt(") x%d[k][v] = ph_ld(rp%d, c * %s);\n", j, j, lit_i64(cs[j]).c_str());
-    s += "            }\n          }\n";
-  }
-  s += "        }\n      }\n";
-  // compute 4 rows
-  std::vector<std::string> inexpr;
-  for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[k][v]", j));
-  std::string body;
-  std::string res = emit_prog(p, inexpr, &body, "");
-  s += fmt("      F cp[%d]; M cm[%d];\n#pragma unroll\n      for (int v = 0; v < %d; ++v) { cp[v] = (F)0; cm[v] = -__builtin_inf(); }\n",
+  s += fmt("  double csum[%d]; M cmax[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) { csum[v] = 0.0; cmax[v] = -__builtin_inf(); }\n",
            V, V, V);
-  s += "#pragma unroll\n      for (int k = 0; k < 4; ++k) {\n        const i64 r = g0 + q + k;\n";
-  s += "        F ls = (F)0; M lm = -__builtin_inf();\n";
-  s += "        if (" + rcheck + ") {\n";
-  s += fmt("#pragma unroll\n          for (int v = 0; v < %d; ++v) {\n            const i64 c = col0 + v;\n", V);
-  s += "            if (" + ccheck("c") + ") {\n";
-  s += body;
-  s += fmt("              const F val = (F)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
-  if (full_rows && full_cols) {
-    // no bounds checks: the first row / column of a chunk starts the maximum
-    // (as Enumerable#max starts from the first element)
-    if (P.do_cols) s += "              cp[v] += val; cm[v] = k == 0 ? mval : ph_fmax(cm[v], mval);\n";
-    if (P.do_rows) s += "              ls += val; lm = v == 0 ? mval : ph_fmax(lm, mval);\n";
-  } else {
-    if (P.do_cols) s += "              cp[v] += val; cm[v] = ph_fmax(cm[v], mval);\n";
-    if (P.do_rows) s += "              ls += val; lm = ph_fmax(lm, mval);\n";
+  if (P.do_rows) s += fmt("  __shared__ double rsum_sh[%d][4]; __shared__ M rmax_sh[%d][4];\n", P.tr, P.tr);
+  for (int j = 0; j < n_in; ++j) {
+    if (in_dt[j] == PH_BF16 || in_dt[j] == PH_BOOL) {
+      const int w = in_dt[j] == PH_BF16 ? V / 2 : V / 4;
+      s += fmt("  unsigned xa%d[8][%d], xb%d[8][%d];\n", j, w, j, w);
+    } else {
+      const char *T = ctype(in_dt[j]);
+      s += fmt("  %s xa%d[8][%d], xb%d[8][%d];\n", T, j, V, j, V);
+    }
   }
-  s += "            }\n          }\n        }\n";
-  if (P.do_rows) s += "        rsv[q + k] = ls; rmv[q + k] = lm;\n";
-  s += "      }\n";
-  if (P.do_cols)
-    s += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) { st_addsum(cst[v], cp[v]); cst[v].m = ph_fmax(cst[v].m, (double)cm[v]); }\n", V);
-  s += "    }\n";
-  if (P.do_rows) {
-    s += "    const F fs = tfold8<F, false>(rsv);\n    const M fm = tfold8<M, true>(rmv);\n";
-    s += "    const int rr = (lane >> 3) & 7;\n";
-    s += "    if ((lane & 7) == 0 && g0 + rr < r1) { rsum_sh[g0 - r0 + rr][wave] = fs; rmax_sh[g0 - r0 + rr][wave] = fm; }\n";
-  }
+  for (int j = 0; j < n_in; ++j)
+    if (P.vec_ok[j])
+      s += fmt("  const unsigned lane_b%d = threadIdx.x * %du;\n", j, V * (int)ph::dtype_size(in_dt[j]));
+  s += "  {\n" + loads("xa", "r0") + "  }\n";
+  s += "  for (i64 g0 = r0; g0 < r1; g0 += 16) {\n";
+  s += "    {\n" + loads("xb", "(g0 + 8)") + "    }\n";
+  s += compute("xa", "g0");
+  s += "    {\n" + loads("xa", "(g0 + 16)") + "    }\n";
+  s += compute("xb", "(g0 + 8)");
   s += "  }\n";
   if (P.do_cols) {
     s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = col0 + v;\n", V);
-    s += "    if (" + ccheck("c") + fmt(") ((S *)A.colpart)[rt * %s + c] = cst[v];\n  }\n", C.c_str());
+    s += "    if (" + ccheck("c") + fmt(") { S st; st_from(st, csum[v], (double)cmax[v]); ((S *)A.colpart)[rt * %s + c] = st; }\n  }\n", C.c_str());
   }
   if (P.do_rows) {
     s += "  __syncthreads();\n";
     s += "  for (i64 i = threadIdx.x; i < r1 - r0; i += 256) {\n";
-    s += "    S a; a.s = rsum_sh[i][0]; a.c = (F)0; a.m = rmax_sh[i][0];\n";
-    s += "    for (int w = 1; w < 4; ++w) { S b; b.s = rsum_sh[i][w]; b.c = (F)0; b.m = rmax_sh[i][w]; a = st_comb(a, b); }\n";
+    s += "    S a; st_from(a, rsum_sh[i][0], (double)rmax_sh[i][0]);\n";
+    s += "    for (int w = 1; w < 4; ++w) { S b; st_from(b, rsum_sh[i][w], (double)rmax_sh[i][w]); a = st_comb(a, b); }\n";
     s += fmt("    ((S *)A.rowpart)[ct * %s + r0 + i] = a;\n  }\n", lit_i64(P.rows).c_str());
   }
   s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
@@ -857,6 +959,15 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   return s;
 }
 
+// Columns per lane of the reduction tile: 8 when every input is 2 bytes or
+// narrower (16-byte BFloat16 loads), else 4; the integer tile always takes 4.
+int red_vec(bool int_state, int n_in, const int *in_dt) {
+  if (int_state || n_in < 1) return 4;
+  for (int j = 0; j < n_in; ++j)
+    if (ph::dtype_size(in_dt[j]) > 2) return 4;
+  return 8;
+}
+
 // A 2-D view of every input: rows x cols with per-input row/col strides.
 struct View2D {
   int64_t rows, cols;
@@ -882,24 +993,29 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   } else {
     P.state = "SF"; P.state_bytes = sizeof(HostSF); P.acc_t = "float";
   }
+  const bool int_state = std::string(P.state) == "SI";
+  P.vec = red_vec(int_state, n_in, in_dt);
   for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = false;
   for (int j = 0; j < n_in; ++j) {
-    const size_t vbytes = 4 * ph::dtype_size(in_dt[j]);
-    P.vec_ok[j] = V.cs[j] == 1 && V.off[j] % 4 == 0 && (P.rows == 1 || V.rs[j] % 4 == 0) &&
-                  ((uintptr_t)ins[j] % vbytes) == 0;
+    // the float tile loads at most 16 bytes at a time, the integer tile one
+    // 4-element vector
+    const size_t vbytes = (size_t)P.vec * ph::dtype_size(in_dt[j]);
+    const size_t align = int_state ? vbytes : std::min<size_t>(vbytes, 16);
+    P.vec_ok[j] = V.cs[j] == 1 && V.off[j] % P.vec == 0 && (P.rows == 1 || V.rs[j] % P.vec == 0) &&
+                  ((uintptr_t)ins[j] % align) == 0;
   }
-  P.ntc = (P.cols + 1023) / 1024;
+  P.ntc = (P.cols + 256 * P.vec - 1) / (256 * P.vec);
   // rows per tile: aim for ~512 workgroups (fewer, taller tiles write fewer
   // column partial states; measured best on 16384^2), a multiple of 16 (the
   // float tile folds rows 8 at a time), at most 1024 rows (256 for integer
   // states, whose per-row LDS states are larger)
   int64_t tr = P.rows * P.ntc / 512;
   if (const char *e = std::getenv("PH_RED_TR")) tr = std::atol(e);  // tuning override
-  const int64_t tr_cap = std::string(P.state) == "SI" ? 256 : 1024;
+  const int64_t tr_cap = int_state ? 256 : 1024;
   if (tr > tr_cap) tr = tr_cap;
   tr = (tr + 15) / 16 * 16;
   if (tr < 16) tr = 16;
-  if (std::string(P.state) == "SI" && tr > P.rows) tr = P.rows;
+  if (int_state && tr > P.rows) tr = P.rows;
   if (tr < 1) tr = 1;
   P.tr = (int)tr;
   P.ntr = (P.rows + P.tr - 1) / P.tr;
@@ -908,7 +1024,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   key_add(&key, V.rs, sizeof(int64_t) * (size_t)n_in);
   key_add(&key, V.cs, sizeof(int64_t) * (size_t)n_in);
   key_add(&key, P.vec_ok, sizeof(bool) * (size_t)n_in);
-  const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.do_rows, P.do_cols, P.do_full};
+  const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.vec, P.do_rows, P.do_cols, P.do_full};
   key_add(&key, plan, sizeof(plan));
   key += P.state;
   hipFunction_t ktile, kfin, kfull;
@@ -1047,8 +1163,9 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   } else {
     P.state = "SF"; P.state_bytes = sizeof(HostSF); P.acc_t = "float";
   }
-  for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = j < n_in && cols % 4 == 0;
-  P.ntc = (cols + 1023) / 1024;
+  P.vec = red_vec(std::string(P.state) == "SI", n_in, in_dt);
+  for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = j < n_in && cols % P.vec == 0;
+  P.ntc = (cols + 256 * P.vec - 1) / (256 * P.vec);
   P.tr = 64;
   P.ntr = (rows + 63) / 64;
   return ph::jit_compile_only(red_source(prog, n_in, in_dt, V.rs, V.cs, P));

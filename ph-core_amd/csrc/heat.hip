@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <utility>
 #include <type_traits>
 
 #include "common.hpp"
@@ -74,23 +76,43 @@ struct HeatArgs {
   int64_t per_xcd;   // workgroups per XCD slice
 };
 
+// Calls f(std::integral_constant<int, G>) for G = 0 .. N-1.
+template <typename F, int... G>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, G...>) {
+  (f(std::integral_constant<int, G>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // ---------------------------------------------------------------------------
 // K time steps per pass (temporal blocking, 1 <= K <= kHalo * VEC).
 //
 // Columns: strips overlap.  A wavefront holds 64*VEC consecutive columns and
 // writes the (64 - 2*kHalo)*VEC in the middle (lanes kHalo .. 63-kHalo).
-// West/east neighbours come from the adjacent lane (a DPP lane shift); the
-// edge lanes have no neighbour beyond the wavefront, so their values go
-// wrong by one column per level, and after K <= kHalo*VEC levels only the
-// kHalo lanes at each edge are wrong: they are never stored.
+// West/east neighbours come from the adjacent lane (a DPP lane shift, which
+// hands the lanes at the wavefront's two ends a zero); those edge lanes have
+// no neighbour beyond the wavefront, so their values go wrong by one column
+// per level, and after K <= kHalo*VEC levels only the kHalo lanes at each
+// edge are wrong: they are not stored.  When rows allow whole-vector lanes
+// and hold at least one wavefront's columns (the usual case), the first and
+// last strips are placed flush with the grid's edges: there the zero of the
+// lane shift is exactly the zero outside the grid, so those edge lanes are
+// exact and stored, and no lane lies outside the grid.
 // Rows: level l+1 of row y is computed from level-l rows y-1, y, y+1.  The
 // wavefront keeps, per level, the last two rows in registers; each loaded
-// level-0 row y+K yields level-1 row y+K-1, …, level-K row y.  The sweep
-// starts 2K rows above the band, so every level is exact before the band's
-// first output row (level l is exact 2l rows after the start).
-// Cells outside the grid are 0 at every level, and each value is computed
-// with the single-step formula, so K steps per pass are bit-identical to K
-// single steps.
+// level-0 row y+K yields level-1 row y+K-1, ..., level-K row y.  A band's
+// sweep starts 2K rows above it, so every level is exact before the band's
+// first output row (level l is exact 2l rows after the start).  The band at
+// the top of the grid starts at the grid's edge instead, with every level of
+// the rows above the edge held at zero (the boundary value; for its first K
+// rows by masks fixed at compile time), and the band at the bottom does the
+// same sweeping upwards (north and south trade places in the registers, not
+// in the formula).  Bands that fit neither case, and grids whose rows do not
+// allow whole-vector lanes, take a sweep that masks every value outside the
+// grid.  Each value is computed with the single-step formula, so K steps per
+// pass are bit-identical to K single steps.
 // ---------------------------------------------------------------------------
 template <typename T, int K, int kAhead, int kHalo>
 __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict__ u, T *__restrict__ out,
@@ -106,71 +128,80 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict_
   const int64_t strip = (lb % gpb) * kWaves + wave;
   const int64_t band = lb / gpb;
   if (strip >= a.nstrips) return;
-  const int64_t x = strip * (64 - 2 * kHalo) * V - kHalo * V + (int64_t)lane * V;  // this lane's first column
   const int64_t y0 = band * a.ty;
   const int64_t y1 = (y0 + a.ty < a.ny) ? y0 + a.ty : a.ny;
-  const bool store_lane = lane >= kHalo && lane < 64 - kHalo;
-  const bool full = vec_ok && x >= 0 && x + V <= a.nx;  // whole aligned vector inside the row
-  bool colok[V];
-#pragma unroll
-  for (int i = 0; i < V; ++i) colok[i] = x + i >= 0 && x + i < a.nx;
   const T four = T(4);
-  // A wavefront whose every lane and every row it touches lies inside the
-  // grid runs the sweep without the edge masks (one select per value less).
-  const bool interior = vec_ok && x - (int64_t)lane * V >= 0 && x - (int64_t)lane * V + 64 * V <= a.nx &&
-                        y0 - 2 * K >= 0 && y1 + K <= a.ny;
-  auto sweep = [&](auto masked_tag) {
-    constexpr bool kMasked = decltype(masked_tag)::value;
-    T prev[K][2][V];  // per level l: rows y+K-l-2 and y+K-l-1
+  const bool flush = vec_ok && a.nx >= 64 * V;
+  int64_t xs = strip * (64 - 2 * kHalo) * V - kHalo * V;  // the wavefront's first column
+  if (flush) xs = xs < 0 ? 0 : (xs > a.nx - 64 * V ? a.nx - 64 * V : xs);
+  const int64_t x = xs + (int64_t)lane * V;  // this lane's first column
+  const bool west_edge = flush && xs == 0, east_edge = flush && xs + 64 * V == a.nx;
+  const bool store_lane = (lane >= kHalo || west_edge) && (lane < 64 - kHalo || east_edge);
+
+  // One level update of V cells: north n, centre c (with the lane's west and
+  // east neighbour columns wl, er), south s.
+  auto level = [&](const T (&n)[V], const T (&c)[V], const T (&s)[V], T wl, T er, T (&o)[V]) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const T w = i == 0 ? wl : c[i - 1];
+      const T e = i == V - 1 ? er : c[i + 1];
+      const T d = (((n[i] - four * c[i]) + s[i]) + w) + e;
+      o[i] = c[i] + d * r;
+    }
+  };
+
+  // ---- Sweep with no masks (flush strips).  Rows are counted in the
+  // sweep's direction, m = y downwards or m = ny-1-y upwards; the band is
+  // [m0, m1).  kEdge: the band starts at the grid's edge (m0 = 0).
+  auto clean = [&](auto up_tag, auto edge_tag, int64_t m0, int64_t m1) {
+    constexpr bool kUp = decltype(up_tag)::value;
+    constexpr bool kEdge = decltype(edge_tag)::value;
+    auto row_at = [&](auto *base, int64_t m) { return base + (kUp ? a.ny - 1 - m : m) * a.nx + x; };
+    T prev[K][2][V];  // per level l: rows m+K-l-2 and m+K-l-1 (in sweep order)
 #pragma unroll
     for (int l = 0; l < K; ++l)
 #pragma unroll
       for (int i = 0; i < V; ++i) prev[l][0][i] = prev[l][1][i] = T(0);
-    // Level-0 rows are prefetched kAhead rows ahead through a ring of kAhead
-    // slots: each slot is refilled right after it is consumed, so kAhead row
-    // loads stay in flight all the time.
-    auto load_level0 = [&](int64_t ry, T *dst) {
-      if ((!kMasked || (ry >= 0 && ry < a.ny)) && ry < y1 + K) {
-        const T *row = u + ry * a.nx;
-        if (!kMasked || full) {
-          unpack<T>(*(const typename Vec<T>::type *)(row + x), dst);
-        } else {
-#pragma unroll
-          for (int i = 0; i < V; ++i) dst[i] = colok[i] ? row[x + i] : T(0);
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < V; ++i) dst[i] = T(0);
-      }
+    // Level-0 rows come through a ring of kAhead slots, each refilled right
+    // after it is consumed, so kAhead row loads stay in flight.  Rows past
+    // m1 + K - 1 are never consumed; they read that row again, so every load
+    // is issued unconditionally (no branch separates it from the wait for
+    // the loads before it).
+    auto load_level0 = [&](int64_t m, T *dst) {
+      const int64_t mc = m < m1 + K ? m : m1 + K - 1;
+      unpack<T>(*(const typename Vec<T>::type *)row_at(u, mc), dst);
     };
-    T L[kAhead][V];  // slot k holds level-0 row y+k+K
+    const int64_t start = m0 - (kEdge ? K : 2 * K);
+    T L[kAhead][V];  // slot k holds level-0 row m+k+K
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) load_level0(y0 - 2 * K + k + K, L[k]);
-    for (int64_t y = y0 - 2 * K; y < y1; y += kAhead) {
+    for (int k = 0; k < kAhead; ++k) load_level0(start + k + K, L[k]);
+    // kAhead rows from m = y.  J0 (compile time): in an edge sweep, the
+    // index of the group's first row among the first K rows; K otherwise.
+    auto group = [&](int64_t y, auto j0_tag) {
+      constexpr int J0 = decltype(j0_tag)::value;
 #pragma unroll
       for (int k = 0; k < kAhead; ++k) {
-        const int64_t yy = y + k;
-        if (yy >= y1) break;
+        // the last group may run past m1: those rows are computed from values
+        // that are never stored, and not stored themselves
+        const int64_t mm = y + k;
         T cur[V];
 #pragma unroll
         for (int i = 0; i < V; ++i) cur[i] = L[k][i];
-        load_level0(yy + kAhead + K, L[k]);  // refill the slot kAhead rows ahead
+        load_level0(mm + kAhead + K, L[k]);  // refill the slot kAhead rows ahead
 #pragma unroll
         for (int l = 0; l < K; ++l) {
-          // level l+1 of row yy+K-l-1: north prev[l][0], centre prev[l][1], south cur
-          const int64_t row = yy + K - l - 1;
-          const bool rowok = !kMasked || (row >= 0 && row < a.ny);
-          const T wl = lane_shift<kDppWaveShr1>(prev[l][1][V - 1]);  // lane-1's last column
-          const T er = lane_shift<kDppWaveShl1>(prev[l][1][0]);      // lane+1's first column
+          // level l+1 of row mm+K-l-1 from prev[l][0], prev[l][1], cur.  In
+          // row j = J0 + k of an edge sweep, levels l >= j belong to rows
+          // beyond the edge: they stay 0.
           T nxt[V];
+          if (J0 + k < K && l >= J0 + k) {
 #pragma unroll
-          for (int i = 0; i < V; ++i) {
-            const T c = prev[l][1][i];
-            const T w = i == 0 ? wl : prev[l][1][i - 1];
-            const T e = i == V - 1 ? er : prev[l][1][i + 1];
-            const T d = (((prev[l][0][i] - four * c) + cur[i]) + w) + e;
-            if (kMasked) nxt[i] = (rowok && colok[i]) ? c + d * r : T(0);
-            else nxt[i] = c + d * r;
+            for (int i = 0; i < V; ++i) nxt[i] = T(0);
+          } else {
+            const T wl = lane_shift<kDppWaveShr1>(prev[l][1][V - 1]);  // lane-1's last column
+            const T er = lane_shift<kDppWaveShl1>(prev[l][1][0]);      // lane+1's first column
+            if constexpr (kUp) level(cur, prev[l][1], prev[l][0], wl, er, nxt);
+            else level(prev[l][0], prev[l][1], cur, wl, er, nxt);
           }
 #pragma unroll
           for (int i = 0; i < V; ++i) {
@@ -179,22 +210,101 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict_
             cur[i] = nxt[i];
           }
         }
-        // cur: level K of row yy
-        if (yy >= y0 && store_lane) {
-          T *orow = out + yy * a.nx;
-          if (!kMasked || full) {
-            *(typename Vec<T>::type *)(orow + x) = pack<T>(cur);
-          } else {
+        // cur: level K of row mm
+        if (mm >= m0 && mm < m1 && store_lane)
+          *(typename Vec<T>::type *)row_at(out, mm) = pack<T>(cur);
+      }
+    };
+    int64_t y = start;
+    if constexpr (kEdge) {
+      constexpr int kGroups = (K + kAhead - 1) / kAhead;
+      static_for<kGroups>([&](auto g) {
+        constexpr int G = decltype(g)::value;
+        group(y + G * kAhead, std::integral_constant<int, G * kAhead>{});
+      });
+      y += kGroups * kAhead;
+    }
+    for (; y < m1; y += kAhead) group(y, std::integral_constant<int, K>{});
+  };
+
+  if (flush) {
+    if (y0 - 2 * K >= 0 && y1 + K <= a.ny) {
+      clean(std::false_type{}, std::false_type{}, y0, y1);
+      return;
+    }
+    if (y0 == 0 && y1 + K <= a.ny) {
+      clean(std::false_type{}, std::true_type{}, 0, y1);
+      return;
+    }
+    if (y1 == a.ny && y0 >= K) {
+      clean(std::true_type{}, std::true_type{}, 0, a.ny - y0);
+      return;
+    }
+  }
+
+  // ---- Sweep with masks: every value outside the grid is held at zero.
+  const bool full = vec_ok && x >= 0 && x + V <= a.nx;  // whole aligned vector inside the row
+  bool colok[V];
 #pragma unroll
-            for (int i = 0; i < V; ++i)
-              if (colok[i]) orow[x + i] = cur[i];
-          }
+  for (int i = 0; i < V; ++i) colok[i] = x + i >= 0 && x + i < a.nx;
+  T prev[K][2][V];
+#pragma unroll
+  for (int l = 0; l < K; ++l)
+#pragma unroll
+    for (int i = 0; i < V; ++i) prev[l][0][i] = prev[l][1][i] = T(0);
+  auto load_level0 = [&](int64_t ry, T *dst) {
+    if (ry >= 0 && ry < a.ny && ry < y1 + K) {
+      const T *row = u + ry * a.nx;
+      if (full) {
+        unpack<T>(*(const typename Vec<T>::type *)(row + x), dst);
+      } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i) dst[i] = colok[i] ? row[x + i] : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < V; ++i) dst[i] = T(0);
+    }
+  };
+  T L[kAhead][V];
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) load_level0(y0 - 2 * K + k + K, L[k]);
+  for (int64_t y = y0 - 2 * K; y < y1; y += kAhead) {
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int64_t yy = y + k;
+      T cur[V];
+#pragma unroll
+      for (int i = 0; i < V; ++i) cur[i] = L[k][i];
+      load_level0(yy + kAhead + K, L[k]);
+#pragma unroll
+      for (int l = 0; l < K; ++l) {
+        const int64_t row = yy + K - l - 1;
+        const bool rowok = row >= 0 && row < a.ny;
+        const T wl = lane_shift<kDppWaveShr1>(prev[l][1][V - 1]);
+        const T er = lane_shift<kDppWaveShl1>(prev[l][1][0]);
+        T nxt[V];
+        level(prev[l][0], prev[l][1], cur, wl, er, nxt);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          nxt[i] = (rowok && colok[i]) ? nxt[i] : T(0);
+          prev[l][0][i] = prev[l][1][i];
+          prev[l][1][i] = cur[i];
+          cur[i] = nxt[i];
+        }
+      }
+      if (yy >= y0 && yy < y1 && store_lane) {
+        T *orow = out + yy * a.nx;
+        if (full) {
+          *(typename Vec<T>::type *)(orow + x) = pack<T>(cur);
+        } else {
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+            if (colok[i]) orow[x + i] = cur[i];
         }
       }
     }
-  };
-  if (interior) sweep(std::false_type{});
-  else sweep(std::true_type{});
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -301,9 +411,59 @@ int fp64_sum(const TA *a, const TB *b, int64_t n, double *out) {
   return ph::check_launch("k_sum");
 }
 
+// Workgroups of kernel `f` that are resident at once on the current device
+// (compute units x workgroups per compute unit at the kernel's register
+// use), cached per device and kernel.
+int64_t resident_groups(const void *f) {
+  static std::map<std::pair<int, const void *>, int64_t> cache;
+  const int dev = ph::device();
+  auto it = cache.find({dev, f});
+  if (it != cache.end()) return it->second;
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 64 * kWaves, 0) != hipSuccess || per < 1) per = 1;
+  return cache[{dev, f}] = (int64_t)cus * per;
+}
+
+// Band height for a pass of K steps.  Every band is a row of independent
+// workgroups, and each of them sweeps its band's rows plus 2K warm-up rows;
+// the workgroups run in ceil(blocks / resident) rounds of equal length, so
+// the pass takes about rounds * (ty + 2K) row sweeps.  The height minimises
+// that (the smallest band count among equals) over heights that keep every
+// band on a sweep without masks: two bands or more, of at least 2K rows, the
+// last of at least K rows.  Small grids where no height does take the
+// cheapest one.
+int64_t band_rows(int64_t ny, int64_t groups_per_band, int64_t resident, int k) {
+  int64_t best_ty = ny, best_cost = -1;
+  bool best_clean = false;
+  for (int64_t nb = 1; nb <= ny; ++nb) {
+    const int64_t ty = (ny + nb - 1) / nb;
+    if (ty < 8 && nb > 1) break;
+    const int64_t nbands = (ny + ty - 1) / ty;
+    const int64_t last = ny - (nbands - 1) * ty;
+    const bool clean = nbands >= 2 && ty >= 2 * k && last >= k;
+    const int64_t rounds = (groups_per_band * nbands + resident - 1) / resident;
+    const int64_t cost = rounds * (ty + 2 * k);
+    if (best_cost < 0 || (clean && !best_clean) || (clean == best_clean && cost < best_cost)) {
+      best_cost = cost;
+      best_ty = ty;
+      best_clean = clean;
+    }
+  }
+  return best_ty;
+}
+
+// `a` carries the grid and strip count; the band height is `ty` when
+// positive, else chosen by band_rows for this kernel's residency.
 template <typename T, int K>
-int launch_multi(const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int ahead) {
+int launch_multi(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok, int ahead) {
   constexpr int H = 2;  // halo lanes on each side of a wavefront's strip
+  const void *f = ahead >= 3 ? (const void *)k_heat_multi<T, K, 3, H> : (const void *)k_heat_multi<T, K, 2, H>;
+  const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
+  a.ty = ty > 0 ? ty : band_rows(a.ny, groups_per_band, resident_groups(f), K);
+  a.nbands = (a.ny + a.ty - 1) / a.ty;
+  a.nblocks = groups_per_band * a.nbands;
+  a.per_xcd = (a.nblocks + 7) / 8;
   const unsigned grid = (unsigned)(a.per_xcd * 8);
   if (ahead >= 3)
     hipLaunchKernelGGL((k_heat_multi<T, K, 3, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a,
@@ -315,16 +475,16 @@ int launch_multi(const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int a
 }
 
 template <typename T>
-int launch_k(int k, const T *cur, T *nxt, const HeatArgs &a, T r, int vec_ok, int ahead) {
+int launch_k(int k, const T *cur, T *nxt, const HeatArgs &a, int64_t ty, T r, int vec_ok, int ahead) {
   switch (k) {
-    case 8: return launch_multi<T, 8>(cur, nxt, a, r, vec_ok, ahead);
-    case 7: return launch_multi<T, 7>(cur, nxt, a, r, vec_ok, ahead);
-    case 6: return launch_multi<T, 6>(cur, nxt, a, r, vec_ok, ahead);
-    case 5: return launch_multi<T, 5>(cur, nxt, a, r, vec_ok, ahead);
-    case 4: return launch_multi<T, 4>(cur, nxt, a, r, vec_ok, ahead);
-    case 3: return launch_multi<T, 3>(cur, nxt, a, r, vec_ok, ahead);
-    case 2: return launch_multi<T, 2>(cur, nxt, a, r, vec_ok, ahead);
-    default: return launch_multi<T, 1>(cur, nxt, a, r, vec_ok, ahead);
+    case 8: return launch_multi<T, 8>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 7: return launch_multi<T, 7>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 6: return launch_multi<T, 6>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 5: return launch_multi<T, 5>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 4: return launch_multi<T, 4>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 3: return launch_multi<T, 3>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 2: return launch_multi<T, 2>(cur, nxt, a, ty, r, vec_ok, ahead);
+    default: return launch_multi<T, 1>(cur, nxt, a, ty, r, vec_ok, ahead);
   }
 }
 
@@ -332,11 +492,12 @@ template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
              double *energy) {
   constexpr int V = Vec<T>::n;
-  // Steps per pass, band height and row prefetch depth (measured on MI355X
-  // at 8192^2, both precisions: 8 steps, 64-row bands, 3 rows loaded
-  // ahead); PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD override them.
+  // Steps per pass and row prefetch depth (measured on MI355X at 8192^2,
+  // both precisions: 8 steps, 3 rows loaded ahead); the band height follows
+  // from the grid and the kernel's residency (band_rows).  PH_HEAT_K,
+  // PH_HEAT_TY and PH_HEAT_AHEAD override them.
   int64_t kmax = 2 * V;
-  int64_t ty = 64;
+  int64_t ty = 0;
   int64_t ahead = 3;
   if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(2 * V, std::max<int64_t>(1, std::atol(e)));
   if (const char *e = std::getenv("PH_HEAT_TY")) ty = std::max<int64_t>(8, std::atol(e));
@@ -346,11 +507,7 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   a.nx = nx;
   const int64_t wcols = (64 - 2 * 2) * V;  // columns a wavefront writes (2 halo lanes per side)
   a.nstrips = (nx + wcols - 1) / wcols;
-  a.ty = ty;
-  a.nbands = (ny + a.ty - 1) / a.ty;
-  const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
-  a.nblocks = groups_per_band * a.nbands;
-  a.per_xcd = (a.nblocks + 7) / 8;
+  a.ty = a.nbands = a.nblocks = a.per_xcd = 0;  // set per pass by launch_multi
   constexpr uintptr_t kAlign = sizeof(typename Vec<T>::type);
   const int vec_ok = (nx % V == 0) && ((uintptr_t)u % kAlign == 0) && ((uintptr_t)tmp % kAlign == 0);
   const T rr = (T)r;
@@ -370,7 +527,7 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     const int k = (int)std::min<int64_t>(kmax, until - t);
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
-    rc = launch_k<T>(k, cur, nxt, a, rr, vec_ok, (int)ahead);
+    rc = launch_k<T>(k, cur, nxt, a, ty, rr, vec_ok, (int)ahead);
     if (rc) return rc;
     T *sw = cur;
     cur = nxt;

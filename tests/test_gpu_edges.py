@@ -129,3 +129,19 @@ def test_beyond_2_31_elements(ph):
     exp[n - 2000::2] = np.arange(1000, dtype=np.int32)
     assert np.array_equal(got[n - 4096:], exp[n - 4096:]) and np.array_equal(got[:4096], exp[:4096])
     assert x.view("-3..").to_narr().to_numpy().tolist() == exp[-3:].tolist()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_heat_edge_placed_sweeps(ph, dt):
+    """Grids whose rows allow whole-vector lanes: strips flush with the west
+    and east edges, the top band swept down and the bottom band up from the
+    grid's edge, eight steps per pass (17 = 8 + 8 + 1), bit-exact."""
+    rng = np.random.default_rng(29)
+    for ny, nx in [(40, 256), (300, 1000), (1000, 2048), (97, 260)]:
+        for steps in (8, 17):
+            u0 = rng.random((ny, nx)).astype(dt)
+            u = ph.NArray.from_numpy(u0)
+            e = ph.heat2d(u, 0.2, steps, every=steps)
+            ref, eref = O.heat2d(u0, 0.2, steps, every=steps)
+            assert np.array_equal(u.to_numpy(), ref), (ny, nx, steps)
+            assert np.allclose(e, eref, rtol=1e-13, atol=1e-300), (ny, nx, steps)
