@@ -379,7 +379,7 @@ def other_workloads(phase, ev, args):
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
             "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
             "traffic": tr.get(f"ph_red_tile_{st.lower()}"),
-            "note": "one pass: sum and max in full, along slices(0) and slices(1), FP32 compensated"}
+            "note": "one pass: sum and max in full, along slices(0) and slices(1); FP64 sums inside each tile, compensated FP32 pairs across tiles"}
         del x, pv
 
     # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64

@@ -795,6 +795,7 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
            lit_i64(P.rows).c_str());
   s += "  int fl = 0;\n";
+  if (P.do_full) s += "  if (blockIdx.x == 0 && threadIdx.x == 0) A.flags[1] = 0;  // the finish kernel's ticket\n";
   s += fmt("  double csum[%d]; M cmax[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) { csum[v] = 0.0; cmax[v] = -__builtin_inf(); }\n",
            V, V, V);
   if (P.do_rows) s += fmt("  __shared__ double rsum_sh[%d][4]; __shared__ M rmax_sh[%d][4];\n", P.tr, P.tr);
@@ -851,6 +852,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
            lit_i64(P.rows).c_str());
   s += "  int fl = 0;\n";
+  if (P.do_full) s += "  if (blockIdx.x == 0 && threadIdx.x == 0) A.flags[1] = 0;  // the finish kernel's ticket\n";
   s += fmt("  S cst[%d];\n#pragma unroll\n  for (int v = 0; v < %d; ++v) st_init(cst[v]);\n", V, V);
   if (P.do_rows) s += fmt("  __shared__ S rowsh[%d][4];\n", P.tr);
   s += "  for (i64 r = r0; r < r1; ++r) {\n";
@@ -939,23 +941,28 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
              lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
     s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
     if (P.do_full) {
+      const int64_t nrb = (P.rows + 255) / 256;
       s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
       s += "    for (int off = 1; off < 256; off <<= 1) {\n      S o = sh[threadIdx.x];\n";
       s += "      if ((threadIdx.x & (2 * off - 1)) == 0) o = st_comb(o, sh[threadIdx.x + off]);\n";
       s += "      __syncthreads();\n      sh[threadIdx.x] = o;\n      __syncthreads();\n    }\n";
-      s += "    if (threadIdx.x == 0) ((S *)A.rowfinal)[rb] = sh[0];\n";
+      // The row block that finishes last (a ticket counter, reset by the
+      // tile kernel) folds the row blocks' states in order for the full
+      // result: one wavefront, each lane a run of consecutive blocks, then
+      // an ordered fold across the lanes.
+      s += "    __shared__ int last;\n";
+      s += "    if (threadIdx.x == 0) {\n      ((S *)A.rowfinal)[rb] = sh[0];\n      __threadfence();\n";
+      s += fmt("      last = atomicAdd(A.flags + 1, 1) == %s - 1;\n    }\n    __syncthreads();\n", lit_i64(nrb).c_str());
+      s += "    if (last && threadIdx.x < 64) {\n      __threadfence();\n";
+      s += fmt("      const i64 n = %s; const i64 per = (n + 63) / 64;\n", lit_i64(nrb).c_str());
+      s += "      const i64 b = threadIdx.x * per; const i64 e = (b + per < n) ? b + per : n;\n";
+      s += "      S f; st_init(f);\n      for (i64 j = b; j < e; ++j) f = st_comb(f, ((const S *)A.rowfinal)[j]);\n";
+      s += "      f = wave_fold(f);\n";
+      s += "      if (threadIdx.x == 0) {\n        if (A.full_sum) { A.full_sum[0] = st_sum(f); fl |= st_ovf(f); }\n";
+      s += "        if (A.full_max) A.full_max[0] = st_max(f);\n      }\n    }\n";
     }
   }
   s += "  }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
-  // --- full: one wavefront folds the row blocks' states in order
-  const int64_t nrb = (P.rows + 255) / 256;
-  s += "extern \"C\" __global__ void __launch_bounds__(64) ph_red_full(RedArgs A) {\n";
-  s += fmt("  typedef %s S;\n", P.state);
-  s += fmt("  const i64 n = %s; const i64 per = (n + 63) / 64;\n", lit_i64(nrb).c_str());
-  s += "  const i64 b = threadIdx.x * per; const i64 e = (b + per < n) ? b + per : n;\n";
-  s += "  S a; st_init(a);\n  for (i64 j = b; j < e; ++j) a = st_comb(a, ((const S *)A.rowfinal)[j]);\n";
-  s += "  a = wave_fold(a);\n";
-  s += "  if (threadIdx.x == 0) { if (A.full_sum) { A.full_sum[0] = st_sum(a); if (st_ovf(a)) atomicOr(A.flags, 1); } if (A.full_max) A.full_max[0] = st_max(a); }\n}\n";
   return s;
 }
 
@@ -1027,22 +1034,19 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.vec, P.do_rows, P.do_cols, P.do_full};
   key_add(&key, plan, sizeof(plan));
   key += P.state;
-  hipFunction_t ktile, kfin, kfull;
+  hipFunction_t ktile, kfin;
   int rc;
   auto hit = fused_cache().find(key);
   if (hit != fused_cache().end()) {
     ktile = hit->second.k[0];
     kfin = hit->second.k[1];
-    kfull = hit->second.k[2];
   } else {
     std::string src = red_source(prog, n_in, in_dt, V.rs, V.cs, P);
     rc = ph::jit_kernel(src, "ph_red_tile", &ktile);
     if (rc) return rc;
     rc = ph::jit_kernel(src, "ph_red_finish", &kfin);
     if (rc) return rc;
-    rc = ph::jit_kernel(src, "ph_red_full", &kfull);
-    if (rc) return rc;
-    fused_cache()[key] = FusedKernels{{ktile, kfin, kfull}, 0};
+    fused_cache()[key] = FusedKernels{{ktile, kfin, nullptr}, 0};
   }
   const bool may_flag = prog_may_flag(prog) || (std::string(P.state) == "SI" && (full_sum || row_sum || col_sum));
   // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [row blocks of 256] (sized [rows])
@@ -1084,8 +1088,6 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   const int64_t nfin = (P.do_cols ? (P.cols + 63) / 64 : 0) + (P.do_rows ? (P.rows + 255) / 256 : 0);
   if (nfin > 0)
     PH_HIP(hipModuleLaunchKernel(kfin, (unsigned)nfin, 1, 1, 256, 1, 1, 0, ph::stream(), nullptr, cfg));
-  if (P.do_full)
-    PH_HIP(hipModuleLaunchKernel(kfull, 1, 1, 1, 64, 1, 1, 0, ph::stream(), nullptr, cfg));
   return read_flags(dflags, may_flag, flags_out);
 }
 

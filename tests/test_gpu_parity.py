@@ -413,18 +413,18 @@ def test_reductions_small(ph, storage):
     res = pv.reduce2d_all()
     f = _c3_block(x32)
     full = O.sum_exact(f)
-    assert abs(res["full_sum"] - full) <= 1e-6 * abs(full)
+    assert abs(res["full_sum"] - full) <= 1e-11 * abs(full)
     assert res["full_max"] == float(f.max())  # max is order-free: exact
     rs = np.array([O.sum_exact(f[i]) for i in range(rows)])
     cs = np.array([O.sum_exact(f[:, j]) for j in range(cols)])
-    assert np.max(np.abs(np.array(res["row_sum"]) - rs) / np.abs(rs)) < 1e-6
-    assert np.max(np.abs(np.array(res["col_sum"]) - cs) / np.abs(cs)) < 1e-6
+    assert np.max(np.abs(np.array(res["row_sum"]) - rs) / np.abs(rs)) < 1e-11
+    assert np.max(np.abs(np.array(res["col_sum"]) - cs) / np.abs(cs)) < 1e-11
     assert np.array_equal(res["row_max"], f.max(axis=1))
     assert np.array_equal(res["col_max"], f.max(axis=0))
     # slices(k).map(&.sum) and the full sum through the generic entry point
-    assert np.allclose(pv.slices_sum(0), rs, rtol=1e-6)
-    assert np.allclose(pv.slices_sum(1), cs, rtol=1e-6)
-    assert pv.sum() == pytest.approx(full, rel=1e-6)
+    assert np.allclose(pv.slices_sum(0), rs, rtol=1e-11, atol=0)
+    assert np.allclose(pv.slices_sum(1), cs, rtol=1e-11, atol=0)
+    assert pv.sum() == pytest.approx(full, rel=1e-11)
     assert pv.sum(acc="f64") == pytest.approx(full, rel=1e-12)
     # max is exact (order-free)
     assert pv.max() == float(np.max(f))
