@@ -20,6 +20,7 @@
 // of array sizes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -65,6 +66,7 @@ struct Batch {
   int64_t chunk = 0;
   int grid = 0;
   int dynamic = 1;
+  int sub = 2048;  // elements per streaming step of a wavefront (1024 or 2048)
   // caller buffers
   const float *x = nullptr;
   float *y = nullptr, *z = nullptr;
@@ -134,9 +136,9 @@ __device__ __forceinline__ uint32_t slice_before(const BRec &R, uint32_t e) {
   return cnt;
 }
 
-// Elements a wavefront streams before gathering their slice elements: the
-// lines just read are still in L2 when they are gathered.
-constexpr uint32_t kSub = 1024;
+// Elements a wavefront streams before gathering their slice elements (kSub,
+// a template parameter of 1024 or 2048): the lines just read are still in L2
+// when they are gathered.
 
 // Persistent: each wavefront runs work items until none are left.  dynamic:
 // items come from a queue (one atomic per item, lane 0); static: wavefront w
@@ -147,6 +149,7 @@ constexpr uint32_t kSub = 1024;
 // elements in the packed layout, so a vector never leaves x; a partial last
 // vector is stored element by element), then the wavefront gathers the
 // slice elements sourced from [base, base + kSub) and writes them to z.
+template <uint32_t kSub>
 __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, int32_t nitems, int32_t *counter,
                                                int32_t dynamic, const float *__restrict__ x,
                                                float *__restrict__ y, float *__restrict__ z,
@@ -168,14 +171,15 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
     uint32_t zlo = R.zlo;
     for (uint32_t base = R.lo; base < R.hi; base += kSub) {
       const uint32_t end = min(R.hi, base + kSub);
-      f4 v[4];
+      constexpr int kVecs = (int)(kSub / 256);
+      f4 v[kVecs];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < kVecs; ++m) {
         const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
         if (e < end) v[m] = *(const f4 *)(xa + e);
       }
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < kVecs; ++m) {
         const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
         const f4 r = v[m] + R.scalar;
         if (e + 4 <= end) {
@@ -342,8 +346,14 @@ int launch_batch(Batch *B) {
   if (B->n == 0) return PH_OK;  // nothing to launch
   hipStream_t st = ph::stream();
   if (B->dynamic) hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, B->d_counter);
-  hipLaunchKernelGGL(k_batch, dim3((unsigned)B->grid), dim3(256), 0, st, (const BRec *)B->d_recs,
-                     (int32_t)B->nitems, B->d_counter, (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
+  int sub = B->sub;
+  if (const char *e = std::getenv("PH_BATCH_SUB")) sub = std::atoi(e);  // tuning override
+  if (sub == 2048)
+    hipLaunchKernelGGL(k_batch<2048>, dim3((unsigned)B->grid), dim3(256), 0, st, (const BRec *)B->d_recs,
+                       (int32_t)B->nitems, B->d_counter, (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
+  else
+    hipLaunchKernelGGL(k_batch<1024>, dim3((unsigned)B->grid), dim3(256), 0, st, (const BRec *)B->d_recs,
+                       (int32_t)B->nitems, B->d_counter, (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
   hipLaunchKernelGGL(k_batch_final, dim3((unsigned)((B->n + 255) / 256)), dim3(256), 0, st,
                      (const int32_t *)B->d_first, B->n, (const float2 *)B->d_partial, B->sums);
   return ph::check_launch("k_batch");

@@ -684,8 +684,9 @@ struct RedPlan {
 // the value's own type (exact).  Rows are processed in groups of 8: the loads
 // of the next group are issued before the current one is computed (two
 // register buffers, alternating).  Bounds checks are emitted only when the
-// shape is not a whole number of tiles, so the interior code is free of
-// predication.
+// shape is not a whole number of tiles, and then only in a second copy of
+// the body that the last row and column tiles run; whole tiles run one free
+// of predication.
 std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs,
                            const int64_t *cs, const RedPlan &P) {
   const int V = P.vec;
@@ -694,8 +695,25 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   const bool full_cols = P.cols % (256 * V) == 0;
   const bool full_rows = P.rows % P.tr == 0;
   const std::string C = lit_i64(P.cols);
-  auto ccheck = [&](const std::string &expr) { return full_cols ? std::string("true") : "(" + expr + " < " + C + ")"; };
-  const std::string rcheck = full_rows ? "true" : "(r < r1)";
+  // bounds checks emitted by the generators below (switched off for the body
+  // that whole tiles run)
+  bool chk_cols = !full_cols, chk_rows = !full_rows;
+  // Lane-granular column handling: when every input takes whole-vector loads
+  // and the row length is a multiple of VEC, each lane's columns lie wholly
+  // inside or wholly outside the row.  Lanes outside then load a clamped
+  // (valid) vector, accumulate column values that are never written, and
+  // give their rows zero and -inf: no per-element branches.
+  bool lane_ok = P.cols % V == 0;
+  for (int j = 0; j < n_in; ++j) lane_ok = lane_ok && P.vec_ok[j];
+  bool lane_cols = false;
+  // Column of a lane's v-th value: VEC consecutive columns when every input
+  // takes whole-vector loads; otherwise columns 64 apart, so that each
+  // element-wise load instruction reads 64 consecutive elements.
+  bool all_vec = true;
+  for (int j = 0; j < n_in; ++j) all_vec = all_vec && P.vec_ok[j];
+  auto colv = [&](const std::string &v) { return all_vec ? "col0 + " + v : "cb + lane + 64 * (" + v + ")"; };
+  auto ccheck = [&](const std::string &expr) { return chk_cols ? "(" + expr + " < " + C + ")" : std::string("true"); };
+  auto rcheck = [&]() { return chk_rows ? std::string("(r < r1)") : std::string("true"); };
   // loads of the 8 rows starting at row `g` into buffer `b`
   // (rows past the tile's end read its last row again: every group's loads
   // are issued unconditionally, so no branch separates them from the wait for
@@ -707,26 +725,32 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
     for (int j = 0; j < n_in; ++j) {
       o += fmt("        const %s *rp%d = (const %s *)A.in[%d] + (A.in_off[%d] + r * %s);\n", stype(in_dt[j]), j,
                stype(in_dt[j]), j, j, lit_i64(rs[j]).c_str());
-      if (P.vec_ok[j]) {
-        // the row's address is uniform, the lane's byte offset 32-bit
-        o += "        if (" + ccheck(fmt("col0 + %d", V - 1)) +
-             fmt(") ph_ldv<%d>((const %s *)((const char *)(rp%d + ct * %d) + lane_b%d), %s%d[k]);\n", V, stype(in_dt[j]), j,
-                 256 * V, j, b.c_str(), j);
-        if (full_cols) continue;
-        o += "        else\n";
+      if (lane_cols) {
+        o += fmt("        ph_ldv<%d>(rp%d + col0c, %s%d[k]);\n", V, j, b.c_str(), j);
+        continue;
       }
+      if (P.vec_ok[j] && !chk_cols) {
+        // the row's address is uniform, the lane's byte offset 32-bit
+        o += fmt("        ph_ldv<%d>((const %s *)((const char *)(rp%d + ct * %d) + lane_b%d), %s%d[k]);\n", V,
+                 stype(in_dt[j]), j, 256 * V, j, b.c_str(), j);
+        continue;
+      }
+      // element by element; with column checks a column past the row's end
+      // reads the row's last element (its values are masked where used)
       const std::string CS = lit_i64(cs[j]);
+      const std::string cc = chk_cols ? "(c < " + C + " ? c : " + lit_i64(P.cols - 1) + ")" : "c";
       if (in_dt[j] == PH_BF16 || in_dt[j] == PH_BOOL) {
-        // element by element into the packed words (absent columns read 0)
+        // into the packed words
         const int per = in_dt[j] == PH_BF16 ? 2 : 4, bits = in_dt[j] == PH_BF16 ? 16 : 8;
         o += fmt("#pragma unroll\n        for (int q = 0; q < %d; ++q) {\n          unsigned w = 0;\n", V / per);
-        o += fmt("#pragma unroll\n          for (int e = 0; e < %d; ++e) {\n            const i64 c = col0 + q * %d + e;\n", per, per);
-        o += "            if (" + ccheck("c") + fmt(") w |= (unsigned)rp%d[c * %s] << (%d * e);\n", j, CS.c_str(), bits);
+        o += fmt("#pragma unroll\n          for (int e = 0; e < %d; ++e) {\n            const i64 c = %s;\n", per,
+                 colv(fmt("q * %d + e", per)).c_str());
+        o += fmt("            w |= (unsigned)rp%d[%s * %s] << (%d * e);\n", j, cc.c_str(), CS.c_str(), bits);
         o += fmt("          }\n          %s%d[k][q] = w;\n        }\n", b.c_str(), j);
         continue;
       }
-      o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = col0 + v;\n", V);
-      o += "          if (" + ccheck("c") + fmt(") %s%d[k][v] = ph_ld(rp%d, c * %s);\n", b.c_str(), j, j, CS.c_str());
+      o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = %s;\n", V, colv("v").c_str());
+      o += fmt("          %s%d[k][v] = ph_ld(rp%d, %s * %s);\n", b.c_str(), j, j, cc.c_str(), CS.c_str());
       o += "        }\n";
     }
     o += "      }\n    }\n";
@@ -749,21 +773,24 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
     if (P.do_rows) o += "    double rsv[8]; M rmv[8];\n";
     o += "#pragma unroll\n    for (int k = 0; k < 8; ++k) {\n      const i64 r = " + g + " + k;\n";
     o += "      double ls = 0.0; M lm = -__builtin_inf();\n";
-    o += "      if (" + rcheck + ") {\n";
-    o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = col0 + v;\n", V);
-    o += "          if (" + ccheck("c") + ") {\n";
+    o += "      if (" + rcheck() + ") {\n";
+    o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = %s;\n", V, colv("v").c_str());
+    o += "          {\n";
     o += body;
     o += fmt("            const double val = (double)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
+    // (columns past the row's end accumulate values that are never written)
     if (P.do_cols) o += "            csum[v] += val; cmax[v] = ph_fmax(cmax[v], mval);\n";
     if (P.do_rows) {
       // with every column present the lane's first column starts the row
-      // (the row's state still starts from +0.0 where it is combined)
-      if (full_cols)
+      // (the row's state still starts from +0.0 where it is combined);
+      // otherwise columns past the row's end are masked out of it
+      if (!chk_cols)
         o += "            ls = v == 0 ? val : ls + val; lm = v == 0 ? mval : ph_fmax(lm, mval);\n";
       else
-        o += "            ls += val; lm = ph_fmax(lm, mval);\n";
+        o += "            const bool okc = " + ccheck("c") + ";\n            ls += okc ? val : 0.0; lm = okc ? ph_fmax(lm, mval) : lm;\n";
     }
     o += "          }\n        }\n      }\n";
+    if (P.do_rows && lane_cols) o += "      ls = lane_in ? ls : 0.0; lm = lane_in ? lm : (M)-__builtin_inf();\n";
     if (P.do_rows) o += "      rsv[k] = ls; rmv[k] = lm;\n";
     if (P.do_cols) {
       // The column sums and maxima are brought up to date row by row (sums
@@ -790,7 +817,7 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   s += fmt("  const i64 ct = blockIdx.x %% %s; const i64 rt = blockIdx.x / %s;\n", lit_i64(P.ntc).c_str(),
            lit_i64(P.ntc).c_str());
   s += "  const int lane = threadIdx.x & 63; const int wave = threadIdx.x >> 6;\n";
-  s += fmt("  const i64 col0 = ct * %d + wave * %d + lane * %d;\n", 256 * V, 64 * V, V);
+  s += fmt("  const i64 cb = ct * %d + wave * %d, col0 = cb + lane * %d;\n", 256 * V, 64 * V, V);
   s += fmt("  const i64 r0 = rt * %d;\n", P.tr);
   s += fmt("  const i64 r1 = (r0 + %d < %s) ? r0 + %d : %s;\n", P.tr, lit_i64(P.rows).c_str(), P.tr,
            lit_i64(P.rows).c_str());
@@ -811,15 +838,40 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   for (int j = 0; j < n_in; ++j)
     if (P.vec_ok[j])
       s += fmt("  const unsigned lane_b%d = threadIdx.x * %du;\n", j, V * (int)ph::dtype_size(in_dt[j]));
-  s += "  {\n" + loads("xa", "r0") + "  }\n";
-  s += "  for (i64 g0 = r0; g0 < r1; g0 += 16) {\n";
-  s += "    {\n" + loads("xb", "(g0 + 8)") + "    }\n";
-  s += compute("xa", "g0");
-  s += "    {\n" + loads("xa", "(g0 + 16)") + "    }\n";
-  s += compute("xb", "(g0 + 8)");
-  s += "  }\n";
+  auto main_loop = [&]() {
+    std::string o = "  {\n" + loads("xa", "r0") + "  }\n";
+    o += "  for (i64 g0 = r0; g0 < r1; g0 += 16) {\n";
+    o += "    {\n" + loads("xb", "(g0 + 8)") + "    }\n";
+    o += compute("xa", "g0");
+    o += "    {\n" + loads("xa", "(g0 + 16)") + "    }\n";
+    o += compute("xb", "(g0 + 8)");
+    o += "  }\n";
+    return o;
+  };
+  if (full_rows && full_cols) {
+    s += main_loop();
+  } else {
+    // whole tiles (every row and column present: all but the last row and
+    // column tiles) run a body without bounds checks
+    s += fmt("  if (r1 - r0 == %d && ct < %s) {\n", P.tr, full_cols ? lit_i64(P.ntc).c_str() : lit_i64(P.ntc - 1).c_str());
+    chk_rows = chk_cols = false;
+    s += main_loop();
+    s += "  } else {\n";
+    chk_rows = !full_rows;
+    chk_cols = !full_cols;
+    if (chk_cols && lane_ok) {
+      chk_cols = false;
+      lane_cols = true;
+      s += fmt("  const bool lane_in = col0 + %d <= %s;\n", V, C.c_str());
+      s += fmt("  const i64 col0c = lane_in ? col0 : %s;\n", lit_i64(P.cols - V).c_str());
+    }
+    s += main_loop();
+    lane_cols = false;
+    chk_cols = !full_cols;
+    s += "  }\n";
+  }
   if (P.do_cols) {
-    s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = col0 + v;\n", V);
+    s += fmt("#pragma unroll\n  for (int v = 0; v < %d; ++v) {\n    const i64 c = %s;\n", V, colv("v").c_str());
     s += "    if (" + ccheck("c") + fmt(") { S st; st_from(st, csum[v], (double)cmax[v]); ((S *)A.colpart)[rt * %s + c] = st; }\n  }\n", C.c_str());
   }
   if (P.do_rows) {

@@ -41,7 +41,8 @@ def _device(ph, a, storage):
 def test_reductions_ragged_shapes(ph, storage):
     rng = np.random.default_rng(23)
     np_dt = {"F64": np.float64, "F32": np.float32, "BF16": np.float32, "I32": np.int32}[storage]
-    for shape in [(1,), (5,), (1, 1), (1, 1027), (1027, 1), (513, 1029), (3, 5, 7), (2, 1, 1030), (17, 3, 2, 9)]:
+    for shape in [(1,), (5,), (1, 1), (1, 1027), (1027, 1), (513, 1029), (3, 5, 7), (2, 1, 1030), (17, 3, 2, 9),
+                  (100, 1000), (37, 2056), (3, 8), (1100, 3000)]:
         if storage == "I32":
             a = rng.integers(-1000, 1000, shape).astype(np.int32)
         else:
