@@ -1041,7 +1041,7 @@ void pool_release(void *ptr, size_t bytes) {
 }  // namespace
 
 Storage::~Storage() {
-  if (ptr) pool_release(ptr, round_bytes(bytes));
+  if (dptr) pool_release(dptr, round_bytes(bytes));
 }
 
 static ph_view_desc contiguous_desc(DType dt, const Shape &shape, int64_t extent) {
@@ -1059,15 +1059,99 @@ static ph_view_desc contiguous_desc(DType dt, const Shape &shape, int64_t extent
   return d;
 }
 
-void Storage::materialize() {
-  if (!pending_fill) return;
-  Scalar v = *pending_fill;
-  pending_fill.reset();
-  if (elements == 0) return;
-  ph_view_desc d = contiguous_desc(dtype, {elements}, elements);
-  unsigned char bits[8];
-  scalar_bits(v, dtype, bits);
-  check(ph_fill(ptr, &d, bits), "NArray.fill");
+static MirrorStats g_mirror;
+MirrorStats mirror_stats() { return g_mirror; }
+
+const void *Storage::dev_read() {
+  if (pending_fill) {
+    Scalar v = *pending_fill;
+    pending_fill.reset();
+    if (elements > 0) {
+      ph_view_desc d = contiguous_desc(dtype, {elements}, elements);
+      unsigned char bits[8];
+      scalar_bits(v, dtype, bits);
+      check(ph_fill(dptr, &d, bits), "NArray.fill");
+    }
+  }
+  if (dirty_) {
+    // write back each run of dirty pages with one copy
+    const size_t np = page_.size();
+    for (size_t p = 0; p < np;) {
+      if (page_[p] != 2) {
+        ++p;
+        continue;
+      }
+      size_t q = p;
+      while (q < np && page_[q] == 2) page_[q++] = 1;
+      const size_t off = p * kPage, len = std::min(bytes, q * kPage) - off;
+      check(ph_h2d((char *)dptr + off, host_.data() + off, len), "host mirror write-back");
+      ++g_mirror.runs_written;
+      g_mirror.bytes_written += (int64_t)len;
+      p = q;
+    }
+    dirty_ = 0;
+  }
+  return dptr;
+}
+
+void Storage::forget_host() {
+  if (dirty_) throw ArgumentError("host mirror: dirty pages would be lost");
+  if (held_) {
+    std::fill(page_.begin(), page_.end(), (uint8_t)0);
+    held_ = 0;
+  }
+}
+
+void *Storage::dev_write() {
+  dev_read();
+  if (held_) {
+    std::fill(page_.begin(), page_.end(), (uint8_t)0);
+    held_ = 0;
+  }
+  return dptr;
+}
+
+void Storage::ensure_pages(size_t off, size_t len) {
+  if (off + len > bytes) throw IndexError("host mirror access out of range");
+  if (host_.size() != bytes) {
+    host_.assign(bytes, 0);
+    page_.assign((bytes + kPage - 1) / kPage, (uint8_t)0);
+    held_ = dirty_ = 0;
+  }
+  if (len == 0) return;
+  for (size_t p = off / kPage; p <= (off + len - 1) / kPage; ++p) {
+    if (page_[p]) continue;
+    const size_t po = p * kPage, pl = std::min(bytes, po + kPage) - po;
+    if (pending_fill) {
+      // the device has not been filled yet: the page takes the fill value
+      unsigned char bits[8];
+      scalar_bits(*pending_fill, dtype, bits);
+      const size_t es = dtype_size(dtype);
+      for (size_t b = 0; b < pl; b += es) std::memcpy(host_.data() + po + b, bits, es);
+    } else {
+      check(ph_d2h(host_.data() + po, (const char *)dptr + po, pl), "host mirror fetch");
+      ++g_mirror.pages_fetched;
+      g_mirror.bytes_fetched += (int64_t)pl;
+    }
+    page_[p] = 1;
+    ++held_;
+  }
+}
+
+const unsigned char *Storage::host_read(size_t off, size_t len) {
+  ensure_pages(off, len);
+  return host_.data() + off;
+}
+
+unsigned char *Storage::host_write(size_t off, size_t len) {
+  ensure_pages(off, len);
+  if (len > 0)
+    for (size_t p = off / kPage; p <= (off + len - 1) / kPage; ++p)
+      if (page_[p] != 2) {
+        page_[p] = 2;
+        ++dirty_;
+      }
+  return host_.data() + off;
 }
 
 static std::shared_ptr<Storage> new_storage(DType dt, int64_t elements) {
@@ -1076,7 +1160,7 @@ static std::shared_ptr<Storage> new_storage(DType dt, int64_t elements) {
   s->dtype = dt;
   s->elements = elements;
   s->bytes = (size_t)elements * dtype_size(dt);
-  s->ptr = pool_alloc(round_bytes(s->bytes));
+  s->dptr = pool_alloc(round_bytes(s->bytes));
   return s;
 }
 
@@ -1104,11 +1188,15 @@ static int64_t desc_index(const ph_view_desc &d, const Coord &c) {
 }
 
 static Scalar fetch_one(const std::shared_ptr<Storage> &st, int64_t idx) {
-  st->materialize();
-  unsigned char bits[8];
   const size_t es = dtype_size(st->dtype);
-  check(ph_d2h(bits, (const char *)st->ptr + (size_t)idx * es, es), "element read");
-  return scalar_from_bits(st->dtype, bits);
+  return scalar_from_bits(st->dtype, st->host_read((size_t)idx * es, es));
+}
+
+static void store_one(const std::shared_ptr<Storage> &st, int64_t idx, const Scalar &v) {
+  const size_t es = dtype_size(st->dtype);
+  unsigned char bits[8];
+  scalar_bits(v, st->dtype, bits);
+  std::memcpy(st->host_write((size_t)idx * es, es), bits, es);
 }
 
 // --- descriptor composition (a new transform applied to view coordinates
@@ -1253,21 +1341,18 @@ NArray NArray::of_buffer(const Shape &shape, DType dtype, const void *host, size
   if ((size_t)n * dtype_size(dtype) != bytes)
     throw ShapeError("Cannot create NArray: Given shape does not match number of elements in buffer.");
   NArray a = empty(shape, dtype);
-  check(ph_h2d(a.storage->ptr, host, bytes), "NArray.of_buffer");
+  check(ph_h2d(a.storage->dev_write(), host, bytes), "NArray.of_buffer");
   return a;
 }
 
 NArray NArray::random(const Shape &shape, DType dtype, uint64_t seed, uint64_t offset, int dist, double a,
                       double b) {
   NArray r = empty(shape, dtype);
-  check(ph_fill_random(r.storage->ptr, (int32_t)dtype, r.size(), seed, offset, dist, a, b), "random fill");
+  check(ph_fill_random(r.storage->dev_write(), (int32_t)dtype, r.size(), seed, offset, dist, a, b), "random fill");
   return r;
 }
 
-void *NArray::device_ptr() const {
-  storage->materialize();
-  return storage->ptr;
-}
+void *NArray::device_ptr() const { return storage->dev_write(); }
 
 ph_view_desc NArray::desc() const {
   return contiguous_desc(dtype(), shape_, std::max<int64_t>(storage->elements, 1));
@@ -1285,8 +1370,7 @@ DType Operand::value_dtype() const { return body ? body->dtype() : compute_dtype
 
 void NArray::to_host(void *out, size_t bytes) const {
   if (bytes != (size_t)size() * dtype_size(dtype())) throw ArgumentError("to_host: wrong byte count");
-  storage->materialize();
-  check(ph_d2h(out, storage->ptr, bytes), "NArray#to_host");
+  check(ph_d2h(out, storage->dev_read(), bytes), "NArray#to_host");
 }
 
 Scalar NArray::get(const Coord &coord) const {
@@ -1296,11 +1380,7 @@ Scalar NArray::get(const Coord &coord) const {
 
 void NArray::set_element(const Coord &coord, const Scalar &v) {
   Coord c = CoordUtil::canonicalize_coord(coord, shape_);
-  storage->materialize();
-  unsigned char bits[8];
-  scalar_bits(v, dtype(), bits);
-  const size_t es = dtype_size(dtype());
-  check(ph_h2d((char *)storage->ptr + (size_t)desc_index(desc(), c) * es, bits, es), "set_element");
+  store_one(storage, desc_index(desc(), c), v);
 }
 
 static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
@@ -1314,9 +1394,9 @@ static NArray materialize_operand(const Operand &o) {
   }
   NArray out = NArray::empty(o.shape, (DType)o.desc.dtype);
   if (out.size() == 0) return out;
-  o.storage->materialize();
+  const void *src = o.storage->dev_read();
   ph_view_desc dd = out.desc();
-  check(ph_strided_copy(out.storage->ptr, &dd, o.storage->ptr, &o.desc, 0), "gather");
+  check(ph_strided_copy(out.storage->dev_write(), &dd, src, &o.desc, 0), "gather");
   return out;
 }
 
@@ -1627,27 +1707,28 @@ static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc
     // the two address ranges are disjoint
     ph_view_desc sfit = fit_src_desc(src.desc, dshape);
     if (ranges_overlap(sfit, dd)) {
-      dst->materialize();
-      check(ph_strided_copy_sequential(dst->ptr, &dd, src.storage->ptr, &sfit), "scatter (aliased)");
+      void *p = dst->dev_write();
+      check(ph_strided_copy_sequential(p, &dd, p, &sfit), "scatter (aliased)");
       return;
     }
   }
   ph_view_desc sd = fit_src_desc(src.desc, dshape);
-  src.storage->materialize();
-  if (dst->pending_fill && src.storage != dst) {
+  const void *sp = src.storage->dev_read();
+  if (dst->pending_fill && src.storage != dst && dst->host_clean()) {
     if (auto f = box_form(dd, buf_shape)) {
+      // the pending fill and the scatter in one pass: every element of dst
+      // is written, so the (clean) host mirror is dropped
       unsigned char bits[8];
       scalar_bits(*dst->pending_fill, dst->dtype, bits);
-      check(ph_fill_scatter(dst->ptr, (int32_t)dst->dtype, buf_shape.data(), (int32_t)buf_shape.size(),
-                            f->first.data(), f->step.data(), f->count.data(), f->axis.data(), src.storage->ptr,
-                            &sd, bits),
+      check(ph_fill_scatter(dst->dptr, (int32_t)dst->dtype, buf_shape.data(), (int32_t)buf_shape.size(),
+                            f->first.data(), f->step.data(), f->count.data(), f->axis.data(), sp, &sd, bits),
             "fill + scatter");
       dst->pending_fill.reset();
+      dst->forget_host();
       return;
     }
   }
-  dst->materialize();
-  check(ph_strided_copy(dst->ptr, &dd, src.storage->ptr, &sd, 0), "scatter");
+  check(ph_strided_copy(dst->dev_write(), &dd, sp, &sd, 0), "scatter");
 }
 
 void NArray::unsafe_set_chunk(const IndexRegion &region, const Operand &src) {
@@ -1660,10 +1741,9 @@ void NArray::unsafe_set_chunk(const IndexRegion &region, const Scalar &value) {
   View v = view();
   v.restrict_to(region);
   if (v.size() == 0) return;
-  storage->materialize();
   unsigned char bits[8];
   scalar_bits(value, dtype(), bits);
-  check(ph_fill(storage->ptr, &v.desc_, bits), "set_chunk (scalar)");
+  check(ph_fill(storage->dev_write(), &v.desc_, bits), "set_chunk (scalar)");
 }
 
 void NArray::set_chunk(const RegionLiteral &lit, const Operand &src) {
@@ -1896,19 +1976,14 @@ void MutableView::set_chunk(const RegionLiteral &lit, const Scalar &value) {
   View v = *this;
   v.restrict_to(IndexRegion::from_literal(lit, shape_));
   if (v.size() == 0) return;
-  storage->materialize();
   unsigned char bits[8];
   scalar_bits(value, dtype(), bits);
-  check(ph_fill(storage->ptr, &v.desc_, bits), "MutableView set_chunk (scalar)");
+  check(ph_fill(storage->dev_write(), &v.desc_, bits), "MutableView set_chunk (scalar)");
 }
 
 void MutableView::set_element(const Coord &coord, const Scalar &v) {
   Coord c = CoordUtil::canonicalize_coord(coord, shape_);
-  storage->materialize();
-  unsigned char bits[8];
-  scalar_bits(v, dtype(), bits);
-  const size_t es = dtype_size(dtype());
-  check(ph_h2d((char *)storage->ptr + (size_t)desc_index(desc_, c) * es, bits, es), "set_element");
+  store_one(storage, desc_index(desc_, c), v);
 }
 
 Operand ProcView::operand() const {
@@ -1982,14 +2057,13 @@ static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr
   std::vector<const void *> ptrs;
   std::vector<ph_view_desc> descs;
   for (const Operand *o : inputs) {
-    o->storage->materialize();
-    ptrs.push_back(o->storage->ptr);
+    ptrs.push_back(o->storage->dev_read());
     descs.push_back(o->desc);
   }
-  out.storage->materialize();
+  void *op = out.storage->dev_write();
   ph_view_desc od = out.desc();
   int32_t flags = 0;
-  check(ph_map(&prog, (int32_t)inputs.size(), ptrs.data(), descs.data(), out.storage->ptr, &od, &flags), "map");
+  check(ph_map(&prog, (int32_t)inputs.size(), ptrs.data(), descs.data(), op, &od, &flags), "map");
   check_flags(flags, "elementwise");
   return out;
 }
@@ -2038,12 +2112,12 @@ NArray &NArray::map_inplace(const std::function<Expr(const Expr &)> &fn) {
                         dtype_name(body.dtype()) + ")");
   if (dtype() == DType::BF16) throw ArgumentError("map! on BFloat16 storage is not supported");
   if (size() == 0) return *this;
-  storage->materialize();
+  void *sp = storage->dev_write();
   ph_prog prog = body.compile(1);
   ph_view_desc d = desc();
-  const void *ptrs[1] = {storage->ptr};
+  const void *ptrs[1] = {sp};
   int32_t flags = 0;
-  check(ph_map(&prog, 1, ptrs, &d, storage->ptr, &d, &flags), "map!");
+  check(ph_map(&prog, 1, ptrs, &d, sp, &d, &flags), "map!");
   check_flags(flags, "map!");
   return *this;
 }
@@ -2085,8 +2159,7 @@ static Prepared prepare(const Operand &a) {
   if (e.dtype() == DType::Bool) throw ArgumentError("undefined method '+' for Bool (sum of a Bool MultiIndexable)");
   p.vdt = e.dtype();
   p.prog = e.compile(1);
-  a.storage->materialize();
-  p.ptrs.push_back(a.storage->ptr);
+  p.ptrs.push_back(a.storage->dev_read());
   p.descs.push_back(a.desc);
   return p;
 }
@@ -2179,7 +2252,7 @@ NArray reduce2d_all_device(const Operand &a, Acc acc) {
   Prepared p = prepare(a);
   const int64_t R = a.shape[0], C = a.shape[1];
   NArray out = NArray::empty({2 + 2 * R + 2 * C}, DType::F64);
-  double *b = (double *)out.storage->ptr;
+  double *b = (double *)out.storage->dev_write();
   int32_t flags = 0;
   check(ph_reduce2d_all(&p.prog, 1, p.ptrs.data(), p.descs.data(), (int32_t)acc, b, b + 1, b + 2, b + 2 + R,
                         b + 2 + 2 * R, b + 2 + 2 * R + C, &flags),
@@ -2258,7 +2331,8 @@ Batch::~Batch() {
 NArray Batch::piece(const NArray &flat, int64_t off, const Shape &shape) const {
   NArray r = NArray::empty(shape, DType::F32);
   const int64_t n = r.size();
-  if (n) check(ph_d2d(r.device_ptr(), (const float *)flat.device_ptr() + off, (size_t)n * 4), "batch element copy");
+  if (n)
+    check(ph_d2d(r.device_ptr(), (const float *)flat.storage->dev_read() + off, (size_t)n * 4), "batch element copy");
   return r;
 }
 
@@ -2285,11 +2359,23 @@ void Batch::fill_uniform(uint64_t seed, double a, double b) {
   check(ph_fill_random(x.device_ptr(), PH_F32, x.size(), seed, 0, 0, a, b), "batch random fill");
 }
 
-void Batch::run(int mode) { check(ph_batch_run(handle_, mode), "batch run"); }
+// The kernels use the buffers bound at construction: bring x's device copy up
+// to date and drop y's and z's host mirrors before each run.
+void Batch::prepare_run() {
+  x.storage->dev_read();
+  y.storage->dev_write();
+  z.storage->dev_write();
+}
+
+void Batch::run(int mode) {
+  prepare_run();
+  check(ph_batch_run(handle_, mode), "batch run");
+}
 
 std::tuple<int64_t, int, int> Batch::tune() {
   int64_t chunk = 0;
   int32_t grid = 0, dynamic = 0;
+  prepare_run();
   check(ph_batch_tune(handle_, &chunk, &grid, &dynamic), "batch tune");
   return {chunk, (int)grid, (int)dynamic};
 }

@@ -206,15 +206,49 @@ class Expr {
 // left pending (NArray.fill): the first scatter into a pending-fill buffer
 // writes the fill and the scattered elements in one pass.
 // ---------------------------------------------------------------------------
+// Device storage of an NArray, with a lazily filled host mirror.  The mirror
+// is kept in pages: reading an element on the host fetches only its page (one
+// device-to-host copy, then host reads of that page cost nothing), element
+// writes land in the mirror and mark their page dirty, and dirty pages are
+// written back (one host-to-device copy per run of dirty pages) before the
+// device next uses the storage.  A device use that may write the storage
+// drops the mirror.  A pending fill is applied to host pages without any
+// copy, and issued on the device at its first use there.
 struct Storage {
-  void *ptr = nullptr;
+  static constexpr size_t kPage = size_t(1) << 16;
   size_t bytes = 0;
   int64_t elements = 0;
   DType dtype = DType::F64;
-  std::optional<Scalar> pending_fill;
+  std::optional<Scalar> pending_fill;  // fill not yet issued on the device
   ~Storage();
-  void materialize();  // issue a pending fill
+  void *dptr = nullptr;  // device memory: reach it through dev_read / dev_write
+  // The device copy, current: the pending fill issued, dirty pages written back.
+  const void *dev_read();
+  // The same, for a use that may write: the host mirror is dropped.
+  void *dev_write();
+  // Host mirror bytes [off, off + len), fetched page by page as needed; the
+  // write form marks the pages dirty.
+  const unsigned char *host_read(size_t off, size_t len);
+  unsigned char *host_write(size_t off, size_t len);
+  // No host page newer than the device.
+  bool host_clean() const { return dirty_ == 0; }
+  // Drops the (clean) host mirror after a device write made outside
+  // dev_write, which the storage's owner performs itself.
+  void forget_host();
+
+ private:
+  std::vector<unsigned char> host_;
+  std::vector<uint8_t> page_;  // per page: 0 not held, 1 held (equal to the device), 2 dirty
+  size_t held_ = 0, dirty_ = 0;
+  void ensure_pages(size_t off, size_t len);
 };
+
+// Host-mirror traffic since the process started (pages fetched from the
+// device, runs of dirty pages written back, and their bytes).
+struct MirrorStats {
+  int64_t pages_fetched = 0, bytes_fetched = 0, runs_written = 0, bytes_written = 0;
+};
+MirrorStats mirror_stats();
 
 // Coordinate transforms of a view chain (src/view_util/transforms.cr).
 struct Transform {
@@ -435,6 +469,7 @@ class Batch {
   std::unique_ptr<DeviceDoubles> sums_dev;
 
  private:
+  void prepare_run();
   void *handle_ = nullptr;
   NArray piece(const NArray &flat, int64_t off, const Shape &shape) const;
 };

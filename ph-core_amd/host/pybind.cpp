@@ -597,6 +597,15 @@ PYBIND11_MODULE(_phase, m) {
 
   m.def("float_to_s", &phase::float_to_s, py::arg("value"), py::arg("single") = false);
   m.def("heat2d", &phase::heat2d, py::arg("u"), py::arg("r"), py::arg("steps"), py::arg("every") = 0);
+  m.def("mirror_stats", []() {
+    const phase::MirrorStats st = phase::mirror_stats();
+    py::dict d;
+    d["pages_fetched"] = st.pages_fetched;
+    d["bytes_fetched"] = st.bytes_fetched;
+    d["runs_written"] = st.runs_written;
+    d["bytes_written"] = st.bytes_written;
+    return d;
+  }, "Host-mirror traffic since the process started (pages fetched, dirty runs written back, bytes).");
 
   py::class_<Batch>(m, "Batch")
       .def(py::init([](const std::vector<Shape> &shapes, const std::vector<float> &scalars) {
