@@ -199,14 +199,27 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
           }
         }
       }
-      // slice elements in z order: source (2a, 2b, 2c) of z (a, b, c)
+      // slice elements in z order: source (2a, 2b, 2c) of z (a, b, c); four
+      // per lane at a time, their loads issued together before the stores
       const uint32_t zhi = end == R.hi ? R.zhi : slice_before(R, end);
-      for (uint32_t zi = zlo + (uint32_t)lane; zi < zhi; zi += 64) {
-        const uint32_t t = fdiv(zi, R.mz_c, R.lz_c);
-        const uint32_t cc = zi - t * R.zshp[2];
-        const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
-        const uint32_t bb = t - aa * R.zshp[1];
-        za[zi] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
+      for (uint32_t z0 = zlo; z0 < zhi; z0 += 256) {
+        float g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t zi = z0 + 64u * u + (uint32_t)lane;
+          if (zi < zhi) {
+            const uint32_t t = fdiv(zi, R.mz_c, R.lz_c);
+            const uint32_t cc = zi - t * R.zshp[2];
+            const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
+            const uint32_t bb = t - aa * R.zshp[1];
+            g[u] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t zi = z0 + 64u * u + (uint32_t)lane;
+          if (zi < zhi) za[zi] = g[u];
+        }
       }
       zlo = zhi;
     }
