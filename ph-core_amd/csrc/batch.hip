@@ -6,9 +6,9 @@
 // in one pass over x: a persistent kernel whose wavefronts run work items
 // (an array, or a chunk of a large one) from a device-side table.  A
 // wavefront streams its item's elements once (16-byte loads of x, 16-byte
-// non-temporal stores of y, FP32 Neumaier sums), 1024 elements at a time,
-// gathering after each step the slice elements sourced from it out of the
-// lines it has just read (L2 hits) and writing them to z contiguously.  Per-item partial sums land in fixed slots and a second
+// non-temporal stores of y, FP32 Neumaier sums), 2048 elements at a time,
+// keeps each step's values in LDS, gathers from there the slice elements
+// sourced from the step and writes them to z contiguously.  Per-item partial sums land in fixed slots and a second
 // kernel folds each array's slots in item order, so the sums do not depend
 // on which wavefront ran which item.  The kernels are captured once in a HIP
 // graph and replayed.
@@ -136,19 +136,18 @@ __device__ __forceinline__ uint32_t slice_before(const BRec &R, uint32_t e) {
   return cnt;
 }
 
-// Elements a wavefront streams before gathering their slice elements (kSub,
-// a template parameter of 1024 or 2048): the lines just read are still in L2
-// when they are gathered.
-
 // Persistent: each wavefront runs work items until none are left.  dynamic:
 // items come from a queue (one atomic per item, lane 0); static: wavefront w
 // of W takes dispatch positions w, w + W, w + 2W, ... of the largest-first
 // order, which needs no atomics.  An item is processed kSub elements at a
-// time: lane l takes elements base + 4l + 256m (m = 0..3) as 16-byte
-// vectors (items start at multiples of 4 and every array is padded to 4
-// elements in the packed layout, so a vector never leaves x; a partial last
-// vector is stored element by element), then the wavefront gathers the
-// slice elements sourced from [base, base + kSub) and writes them to z.
+// time (a template parameter, 1024 or 2048): lane l takes elements
+// base + 4l + 256m (m = 0 .. kSub/256 - 1) as 16-byte vectors (items start at
+// multiples of 4 and every array is padded to 4 elements in the packed
+// layout, so a vector never leaves x; a partial last vector is stored
+// element by element) and also stores them to the wavefront's LDS copy of
+// the step; the wavefront then gathers the slice elements sourced from
+// [base, base + kSub) out of that copy and writes them to z, so x is read
+// from memory once.
 template <uint32_t kSub>
 __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, int32_t nitems, int32_t *counter,
                                                int32_t dynamic, const float *__restrict__ x,
@@ -157,6 +156,9 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
   const int lane = threadIdx.x & 63;
   const int32_t nwaves = (int32_t)gridDim.x * 4;
   int32_t q = (int32_t)blockIdx.x * 4 + (int32_t)(threadIdx.x >> 6);
+  // each wavefront's copy of its current step of x, for the slice gather
+  __shared__ f4 sx_all[4][kSub / 4];
+  f4 *sx = sx_all[threadIdx.x >> 6];
   if (dynamic) {
     int32_t t = 0;
     if (lane == 0) t = atomicAdd(counter, 1);
@@ -178,6 +180,9 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
         const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
         if (e < end) v[m] = *(const f4 *)(xa + e);
       }
+      // (positions past `end` hold stale values; the gather never reads them)
+#pragma unroll
+      for (int m = 0; m < kVecs; ++m) sx[lane + 64 * m] = v[m];
 #pragma unroll
       for (int m = 0; m < kVecs; ++m) {
         const uint32_t e = base + 4u * (uint32_t)lane + 256u * m;
@@ -199,8 +204,11 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
           }
         }
       }
-      // slice elements in z order: source (2a, 2b, 2c) of z (a, b, c); four
-      // per lane at a time, their loads issued together before the stores
+      // slice elements in z order: source (2a, 2b, 2c) of z (a, b, c), which
+      // lies in [base, end): read from the wavefront's LDS copy of the step;
+      // four per lane at a time, their reads issued before the stores
+      __builtin_amdgcn_wave_barrier();
+      const float *sxf = (const float *)sx;
       const uint32_t zhi = end == R.hi ? R.zhi : slice_before(R, end);
       for (uint32_t z0 = zlo; z0 < zhi; z0 += 256) {
         float g[4];
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
             const uint32_t cc = zi - t * R.zshp[2];
             const uint32_t aa = fdiv(t, R.mz_b, R.lz_b);
             const uint32_t bb = t - aa * R.zshp[1];
-            g[u] = xa[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc];
+            g[u] = sxf[((2 * aa) * R.shp[1] + 2 * bb) * R.shp[2] + 2 * cc - base];
           }
         }
 #pragma unroll
@@ -222,6 +230,7 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
         }
       }
       zlo = zhi;
+      __builtin_amdgcn_wave_barrier();  // the next step overwrites the LDS copy
     }
     wave_fold(s, c);
     if (lane == 0) partial[R.slot] = make_float2(s, c);
