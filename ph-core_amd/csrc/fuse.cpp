@@ -24,6 +24,7 @@
 // sequential checked sum would; every result is deterministic.
 #include <algorithm>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -174,9 +175,40 @@ int validate_prog(const ph_prog *p, int n_in, const int *in_dt) {
 // Emits the statements computing the program for element `e`; inputs are
 // the expressions `inexpr[j]` (already of their compute type).  Returns the
 // name of the result variable.
+// Exact multiply-add fusion.  A Float64 product x * 2^k, where x is a
+// Float32 or Int32 value widened to Float64 and 2^k a constant, is exact
+// (x has at most 31 significant bits and, for the k accepted below, the
+// product neither overflows nor loses bits below 2^-1074).  An add or
+// subtract of that product is then computed as one fused multiply-add with
+// the same single rounding as the two operations, so the fusion changes no
+// result.  Returns the exponent k of a constant 2^k in the accepted range for
+// the source type, or 0 with *ok false.
+static int exact_pow2(double c, int src_dt, bool *ok) {
+  *ok = false;
+  if (!(c > 0.0) || c != c) return 0;
+  int e = 0;
+  const double m = std::frexp(c, &e);
+  if (m != 0.5) return 0;  // not a power of two
+  const int k = e - 1;
+  const bool fits = src_dt == PH_F32 ? (k >= -900 && k <= 890) : src_dt == PH_I32 ? (k >= -1000 && k <= 990) : false;
+  *ok = fits;
+  return k;
+}
+
 std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, std::string *code,
                       const std::string &sfx) {
   std::string regv[PH_NREG];
+  // per register: the Float64 value widened from (dtype, or -1), a Float64
+  // constant, and an exact product x * 2^k (its x and the constant)
+  int widened[PH_NREG];
+  bool isconst[PH_NREG];
+  double cval[PH_NREG];
+  std::string px[PH_NREG], pk[PH_NREG];
+  for (int r = 0; r < PH_NREG; ++r) {
+    widened[r] = -1;
+    isconst[r] = false;
+    cval[r] = 0.0;
+  }
   for (int k = 0; k < p->n_insn; ++k) {
     const ph_insn &in = p->insn[k];
     const int dt = in.dt;
@@ -186,6 +218,44 @@ std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, 
     const std::string C = in.c < PH_NREG ? regv[in.c] : "";
     const char *T = ctype(dt);
     std::string s;
+    // register facts after this instruction
+    int nwid = -1;
+    bool ncon = false;
+    double nval = 0.0;
+    std::string npx, npk;
+    if (in.op == PH_OP_CAST && dt == PH_F64 && (in.c == PH_F32 || in.c == PH_I32)) nwid = in.c;
+    if (in.op == PH_OP_CONST && dt == PH_F64) {
+      ncon = true;
+      nval = in.imm.f;
+    }
+    if (in.op == PH_OP_MUL && dt == PH_F64 && in.a < PH_NREG && in.b < PH_NREG) {
+      bool ok = false;
+      if (widened[in.a] >= 0 && isconst[in.b]) {
+        exact_pow2(cval[in.b], widened[in.a], &ok);
+        if (ok) { npx = A; npk = B; }
+      } else if (widened[in.b] >= 0 && isconst[in.a]) {
+        exact_pow2(cval[in.a], widened[in.b], &ok);
+        if (ok) { npx = B; npk = A; }
+      }
+    }
+    if ((in.op == PH_OP_ADD || in.op == PH_OP_SUB) && dt == PH_F64 && in.a < PH_NREG && in.b < PH_NREG &&
+        (!px[in.a].empty() || !px[in.b].empty())) {
+      std::string f;
+      if (!px[in.a].empty())  // (x * 2^k) +/- B
+        f = fmt("__builtin_fma(%s, %s, %s%s)", px[in.a].c_str(), pk[in.a].c_str(), in.op == PH_OP_SUB ? "-" : "",
+                B.c_str());
+      else if (in.op == PH_OP_ADD)  // A + (x * 2^k)
+        f = fmt("__builtin_fma(%s, %s, %s)", px[in.b].c_str(), pk[in.b].c_str(), A.c_str());
+      else  // A - (x * 2^k)
+        f = fmt("__builtin_fma(-%s, %s, %s)", px[in.b].c_str(), pk[in.b].c_str(), A.c_str());
+      *code += "        const double " + v + " = " + f + ";  // exact fusion (see exact_pow2)\n";
+      regv[in.dst] = v;
+      widened[in.dst] = -1;
+      isconst[in.dst] = false;
+      px[in.dst].clear();
+      pk[in.dst].clear();
+      continue;
+    }
     switch (in.op) {
       case PH_OP_LOAD: s = fmt("const %s %s = %s;", T, v.c_str(), inexpr[in.a].c_str()); break;
       case PH_OP_CONST: s = fmt("const %s %s = %s;", T, v.c_str(), const_expr(dt, in).c_str()); break;
@@ -253,6 +323,11 @@ std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, 
     }
     *code += "        " + s + "\n";
     regv[in.dst] = v;
+    widened[in.dst] = nwid;
+    isconst[in.dst] = ncon;
+    cval[in.dst] = nval;
+    px[in.dst] = npx;
+    pk[in.dst] = npk;
   }
   return regv[p->out_reg];
 }

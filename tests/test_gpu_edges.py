@@ -146,3 +146,38 @@ def test_heat_edge_placed_sweeps(ph, dt):
             ref, eref = O.heat2d(u0, 0.2, steps, every=steps)
             assert np.array_equal(u.to_numpy(), ref), (ny, nx, steps)
             assert np.allclose(e, eref, rtol=1e-13, atol=1e-300), (ny, nx, steps)
+
+
+def test_exact_product_fusion_edge_values(ph):
+    """Float32 or Int32 values widened to Float64, times a power of two, then
+    added or subtracted: the device fuses these into one multiply-add, which
+    must equal the two roundings of the oracle at subnormals, the extremes,
+    zeros of both signs, infinities and NaN."""
+    from crystal import trace
+    f = np.float32
+    edge = np.array([0.0, -0.0, 1.4e-45, -1.4e-45, 1.1754942e-38, 1.17549435e-38, -1.17549435e-38,
+                     3.4028235e38, -3.4028235e38, np.inf, -np.inf, np.nan, 1.0, -2.5, 1e-30, 7e37], np.float32)
+    rng = np.random.default_rng(31)
+    x = np.concatenate([edge, rng.standard_normal(1000).astype(f), (rng.standard_normal(300) * 1e-40).astype(f)])
+    w = np.concatenate([edge[::-1], rng.standard_normal(1000 + 300).astype(f)])
+    assert x.size == w.size
+    cases = [
+        (lambda v: v * 0.5 - 1.0, 1),
+        (lambda v, u: u - v * 0.25, 2),
+        (lambda v, u: v * 1024.0 + u, 2),
+        (lambda v: v * (2.0 ** -800) + 1e-300, 1),
+        (lambda v: v * (2.0 ** 890) - 1e300, 1),
+        (lambda v: (v + 4) ** 2 * 0.5 - v, 1),
+    ]
+    for fn, nin in cases:
+        ins = [x, w][:nin]
+        exp, _ = O.eval_expr(trace(fn, nin), ins)
+        dev = [ph.NArray.from_numpy(a) for a in ins]
+        got = (dev[0].map_with(*dev[1:], block=fn) if nin > 1 else dev[0].process(fn).to_narr()).to_numpy()
+        assert got.dtype == exp.dtype == np.float64
+        assert np.array_equal(got, exp, equal_nan=True)
+        assert np.array_equal(np.signbit(got), np.signbit(exp))
+    i = np.array([0, 1, -1, 2 ** 31 - 1, -2 ** 31, 12345, -7], np.int32)
+    fn = lambda v: v * 0.5 + 0.1  # noqa: E731
+    exp, _ = O.eval_expr(trace(fn, 1), [i])
+    assert np.array_equal(ph.NArray.from_numpy(i).process(fn).to_narr().to_numpy(), exp)
