@@ -244,14 +244,21 @@ __global__ __launch_bounds__(256) void k_batch(const BRec *__restrict__ recs, in
   }
 }
 
-// Per array, fold its items' partials in item order.
+// Per array (one wavefront each), fold its items' partials with a fixed
+// tree: lane l folds a run of consecutive slots in order, then the lanes are
+// combined by the butterfly; the result depends only on the slot values.
 __global__ __launch_bounds__(256) void k_batch_final(const int32_t *__restrict__ first, int n,
                                                      const float2 *__restrict__ partial, double *sums) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
+  const int k = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= n) return;  // uniform per wavefront
+  const int32_t b = first[k], e = first[k + 1];
+  const int32_t per = (e - b + 63) / 64;
+  const int32_t lo = b + lane * per, hi = min(e, lo + per);
   float s = 0.f, c = 0.f;
-  for (int32_t i = first[k]; i < first[k + 1]; ++i) two_sum_fold(s, c, partial[i].x, partial[i].y);
-  sums[k] = (double)s + (double)c;
+  for (int32_t i = lo; i < hi; ++i) two_sum_fold(s, c, partial[i].x, partial[i].y);
+  wave_fold(s, c);
+  if (lane == 0) sums[k] = (double)s + (double)c;
 }
 
 __global__ void k_reset(int32_t *counter) {
@@ -376,7 +383,7 @@ int launch_batch(Batch *B) {
   else
     hipLaunchKernelGGL(k_batch<1024>, dim3((unsigned)B->grid), dim3(256), 0, st, (const BRec *)B->d_recs,
                        (int32_t)B->nitems, B->d_counter, (int32_t)B->dynamic, B->x, B->y, B->z, B->d_partial);
-  hipLaunchKernelGGL(k_batch_final, dim3((unsigned)((B->n + 255) / 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_batch_final, dim3((unsigned)((B->n + 3) / 4)), dim3(256), 0, st,
                      (const int32_t *)B->d_first, B->n, (const float2 *)B->d_partial, B->sums);
   return ph::check_launch("k_batch");
 }
