@@ -389,21 +389,26 @@ def other_workloads(phase, ev, args):
     u0 = heat_initial_field(n, seed=3)
     res = {}
     fields = {}
-    # the same run with 4 steps per pass for both precisions (both HBM-bound),
-    # next to the default schedule (8 steps per pass, where FP32 and FP64 are
-    # both limited by VALU: FP32 runs at FP64's rate without packed FP32 math)
-    os.environ["PH_HEAT_K"] = "4"
-    same = {}
-    for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
-        u = phase.NArray.from_numpy(u0.astype(npdt))
-        ev.sync()
-        aa, bb = ev.new(), ev.new()
-        ev.record(aa)
-        phase.heat2d(u, 0.2, steps, every=100)
-        ev.record(bb)
-        same[dt] = round(n * n * steps / (ev.ms(aa, bb) * 1e-3) / 1e9, 2)
-        del u
-    del os.environ["PH_HEAT_K"]
+    # the same run with 1, 2 and 4 steps per pass for both precisions (one
+    # step per pass is the reference's loop structure; 1 and 2 are
+    # HBM-bound in both precisions), next to
+    # the default schedule (8 steps per pass, where FP32 and FP64 are both
+    # limited by VALU: FP32 runs at FP64's rate without packed FP32 math)
+    per_pass = {}
+    for kk in ("1", "2", "4"):
+        os.environ["PH_HEAT_K"] = kk
+        same = {}
+        for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
+            u = phase.NArray.from_numpy(u0.astype(npdt))
+            ev.sync()
+            aa, bb = ev.new(), ev.new()
+            ev.record(aa)
+            phase.heat2d(u, 0.2, steps, every=100)
+            ev.record(bb)
+            same[dt] = round(n * n * steps / (ev.ms(aa, bb) * 1e-3) / 1e9, 2)
+            del u
+        per_pass[kk] = same
+        del os.environ["PH_HEAT_K"]
     for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
         u = phase.NArray.from_numpy(u0.astype(npdt))
         ev.sync()
@@ -421,8 +426,14 @@ def other_workloads(phase, ev, args):
     rel = float(np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)))
     out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
                            "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
-                           "four_steps_per_pass": {"f64_gcells": same["F64"], "f32_gcells": same["F32"],
-                                                   "f32_over_f64": round(same["F32"] / same["F64"], 3)},
+                           "four_steps_per_pass": {"f64_gcells": per_pass["4"]["F64"], "f32_gcells": per_pass["4"]["F32"],
+                                                   "f32_over_f64": round(per_pass["4"]["F32"] / per_pass["4"]["F64"], 3)},
+                           "two_steps_per_pass": {"f64_gcells": per_pass["2"]["F64"], "f32_gcells": per_pass["2"]["F32"],
+                                                  "f32_over_f64": round(per_pass["2"]["F32"] / per_pass["2"]["F64"], 3)},
+                           "one_step_per_pass": {"f64_gcells": per_pass["1"]["F64"], "f32_gcells": per_pass["1"]["F32"],
+                                                 "f32_over_f64": round(per_pass["1"]["F32"] / per_pass["1"]["F64"], 3),
+                                                 "frac_of_roofline_f64": round(per_pass["1"]["F64"] / 500.0, 3),
+                                                 "frac_of_roofline_f32": round(per_pass["1"]["F32"] / 1000.0, 3)},
                            "rel_l2_f32_vs_f64": rel,
                            "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0,
                            "traffic_per_launch_of_four_steps": {"f64": tr.get("k_heat_multi_f64"),

@@ -455,23 +455,32 @@ int64_t band_rows(int64_t ny, int64_t groups_per_band, int64_t resident, int k) 
 
 // `a` carries the grid and strip count; the band height is `ty` when
 // positive, else chosen by band_rows for this kernel's residency.
-template <typename T, int K>
-int launch_multi(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok, int ahead) {
+template <typename T, int K, int A>
+int launch_ahead(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok) {
   constexpr int H = 2;  // halo lanes on each side of a wavefront's strip
-  const void *f = ahead >= 3 ? (const void *)k_heat_multi<T, K, 3, H> : (const void *)k_heat_multi<T, K, 2, H>;
+  const void *f = (const void *)k_heat_multi<T, K, A, H>;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
   a.ty = ty > 0 ? ty : band_rows(a.ny, groups_per_band, resident_groups(f), K);
   a.nbands = (a.ny + a.ty - 1) / a.ty;
   a.nblocks = groups_per_band * a.nbands;
   a.per_xcd = (a.nblocks + 7) / 8;
   const unsigned grid = (unsigned)(a.per_xcd * 8);
-  if (ahead >= 3)
-    hipLaunchKernelGGL((k_heat_multi<T, K, 3, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a,
-                       r, vec_ok);
-  else
-    hipLaunchKernelGGL((k_heat_multi<T, K, 2, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a,
-                       r, vec_ok);
+  hipLaunchKernelGGL((k_heat_multi<T, K, A, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
+                     vec_ok);
   return ph::check_launch("k_heat_multi");
+}
+
+// Rows loaded ahead: 8 for passes of one or two steps (little arithmetic per
+// row, so more loads must be in flight to keep HBM busy), else 3; `ahead`
+// (PH_HEAT_AHEAD) picks 2, 3 or, for such short passes, 8.
+template <typename T, int K>
+int launch_multi(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok, int ahead) {
+  if (ahead <= 0) ahead = K <= 2 ? 8 : 3;
+  if constexpr (K <= 2) {
+    if (ahead >= 8) return launch_ahead<T, K, 8>(cur, nxt, a, ty, r, vec_ok);
+  }
+  if (ahead >= 3) return launch_ahead<T, K, 3>(cur, nxt, a, ty, r, vec_ok);
+  return launch_ahead<T, K, 2>(cur, nxt, a, ty, r, vec_ok);
 }
 
 template <typename T>
@@ -492,13 +501,13 @@ template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
              double *energy) {
   constexpr int V = Vec<T>::n;
-  // Steps per pass and row prefetch depth (measured on MI355X at 8192^2,
-  // both precisions: 8 steps, 3 rows loaded ahead); the band height follows
-  // from the grid and the kernel's residency (band_rows).  PH_HEAT_K,
-  // PH_HEAT_TY and PH_HEAT_AHEAD override them.
+  // Steps per pass (measured on MI355X at 8192^2, both precisions: 8); the
+  // rows loaded ahead follow from the steps per pass (launch_multi) and the
+  // band height from the grid and the kernel's residency (band_rows).
+  // PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD override them.
   int64_t kmax = 2 * V;
   int64_t ty = 0;
-  int64_t ahead = 3;
+  int64_t ahead = 0;  // by steps per pass (launch_multi)
   if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(2 * V, std::max<int64_t>(1, std::atol(e)));
   if (const char *e = std::getenv("PH_HEAT_TY")) ty = std::max<int64_t>(8, std::atol(e));
   if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead = std::atol(f);

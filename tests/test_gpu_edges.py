@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def test_heat_tiny_and_ragged_grids(ph, dt):
     rng = np.random.default_rng(17)
     for ny, nx in [(1, 1), (1, 7), (7, 1), (2, 3), (3, 1030), (130, 5), (65, 257), (200, 249)]:
-        for steps in (1, 3, 9, 17):
+        for steps in (1, 2, 3, 9, 17):
             u0 = rng.random((ny, nx)).astype(dt)
             u = ph.NArray.from_numpy(u0)
             e = ph.heat2d(u, 0.2, steps, every=4)
@@ -136,10 +136,10 @@ def test_beyond_2_31_elements(ph):
 def test_heat_edge_placed_sweeps(ph, dt):
     """Grids whose rows allow whole-vector lanes: strips flush with the west
     and east edges, the top band swept down and the bottom band up from the
-    grid's edge, eight steps per pass (17 = 8 + 8 + 1), bit-exact."""
+    grid's edge, two and eight steps per pass (17 = 8 + 8 + 1), bit-exact."""
     rng = np.random.default_rng(29)
     for ny, nx in [(40, 256), (300, 1000), (1000, 2048), (97, 260)]:
-        for steps in (8, 17):
+        for steps in (2, 8, 17):
             u0 = rng.random((ny, nx)).astype(dt)
             u = ph.NArray.from_numpy(u0)
             e = ph.heat2d(u, 0.2, steps, every=steps)
