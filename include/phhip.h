@@ -139,6 +139,8 @@ typedef enum {
   PH_OP_SQRT,      /* F sqrt (correctly rounded)                            */
   PH_OP_ABS,
   PH_OP_SELECT,    /* r[dst] = r[c] ? r[a] : r[b]                           */
+  PH_OP_INDEX,     /* r[dst] = the element's lexicographic index (Int32);   */
+                   /* ph_map only (map_with_index, src/n_array.cr:605-611) */
   PH_OP__COUNT
 } ph_opcode;
 
@@ -267,6 +269,10 @@ int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
  * n_array.cr:589-628), scalar-first patches (src/patches/number.cr:5-21),
  * and ProcView#to_narr (src/proc_view.cr:156-158 + src/view.cr:123-126).
  * In-place map (map!) passes out == in_0 with equal contiguous descriptors.
+ * A program with PH_OP_INDEX (NArray#map_with_index / map_with_index! /
+ * map_with_coord, src/n_array.cr:605-650) sees each element's lexicographic
+ * position in out_desc's shape as an Int32; a shape of more than 2^31 - 1
+ * elements is PH_E_OVERFLOW, since Crystal's Int32 index cannot reach it.
  * ------------------------------------------------------------------------- */
 int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins,
            const ph_view_desc *in_descs, void *out,

@@ -91,8 +91,9 @@ std::string const_expr(int dt, const ph_insn &in) {
 bool is_float(int dt) { return dt == PH_F32 || dt == PH_F64; }
 
 // Validates `p` against the input storage dtypes and checks every operand's
-// dtype; fills reg dtypes on the way.
-int validate_prog(const ph_prog *p, int n_in, const int *in_dt) {
+// dtype; fills reg dtypes on the way.  PH_OP_INDEX (an element's lexicographic
+// index) is accepted only where `allow_index` says the kernel defines it.
+int validate_prog(const ph_prog *p, int n_in, const int *in_dt, bool allow_index = false) {
   if (!p) return ph::set_error(PH_E_ARGUMENT, "null program");
   if (p->n_insn < 1 || p->n_insn > PH_MAX_INSN)
     return ph::set_error(PH_E_ARGUMENT, "program length %d", p->n_insn);
@@ -160,6 +161,10 @@ int validate_prog(const ph_prog *p, int n_in, const int *in_dt) {
         if (reg(in.c) != PH_BOOL || reg(in.a) != dt || reg(in.b) != dt)
           return ph::set_error(PH_E_ARGUMENT, "insn %d: select operands", k);
         break;
+      case PH_OP_INDEX:
+        if (!allow_index) return ph::set_error(PH_E_ARGUMENT, "insn %d: element index outside a map", k);
+        if (dt != PH_I32) return ph::set_error(PH_E_ARGUMENT, "insn %d: element index is Int32", k);
+        break;
       default:
         return ph::set_error(PH_E_ARGUMENT, "insn %d: opcode %d", k, in.op);
     }
@@ -170,6 +175,12 @@ int validate_prog(const ph_prog *p, int n_in, const int *in_dt) {
   if (rdt[p->out_reg] != p->out_dtype)
     return ph::set_error(PH_E_ARGUMENT, "program result dtype %d, declared %d", rdt[p->out_reg], p->out_dtype);
   return PH_OK;
+}
+
+bool prog_uses_index(const ph_prog *p) {
+  for (int k = 0; k < p->n_insn; ++k)
+    if (p->insn[k].op == PH_OP_INDEX) return true;
+  return false;
 }
 
 // Emits the statements computing the program for element `e`; inputs are
@@ -320,6 +331,7 @@ std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, 
           s = fmt("const %s %s = ph_fabs(%s);", T, v.c_str(), A.c_str());
         break;
       case PH_OP_SELECT: s = fmt("const %s %s = %s ? %s : %s;", T, v.c_str(), C.c_str(), A.c_str(), B.c_str()); break;
+      case PH_OP_INDEX: s = fmt("const int %s = (int)ph_lidx;", v.c_str()); break;
     }
     *code += "        " + s + "\n";
     regv[in.dst] = v;
@@ -428,6 +440,9 @@ std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::L
   for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d[e]", j));
   std::string body;
   std::string res = emit_prog(p, inexpr, &body, "");
+  // the element's lexicographic index: collapse() keeps the logical order
+  // (it drops extent-1 axes and merges neighbours), so it is row * cols + c
+  const bool uses_index = prog_uses_index(p);
   if (!flat) {
     const int64_t ntiles = (cols + tile - 1) / tile;
     *nblocks = rows * ntiles;
@@ -445,6 +460,7 @@ std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::L
     s += "      }\n    }\n";
     s += fmt("#pragma unroll\n    for (int e = 0; e < %d; ++e) {\n", ept);
     s += fmt("      const i64 c = c0 + (i64)e * 256;\n      if (c < %s) {\n", lit_i64(cols).c_str());
+    if (uses_index) s += fmt("        const i64 ph_lidx = row * %s + c;\n", lit_i64(cols).c_str());
     s += body;
     s += fmt("        ph_st((%s *)A.out, base%d + c * %s, %s);\n", stype(out_dt), n_in,
              lit_i64(inner_stride(n_in)).c_str(), res.c_str());
@@ -468,6 +484,7 @@ std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::L
     std::vector<std::string> inexpr0;
     for (int j = 0; j < n_in; ++j) inexpr0.push_back(fmt("x%d[0]", j));
     std::string res0 = emit_prog(p, inexpr0, &body0, "");
+    if (uses_index) s += "        const i64 ph_lidx = g;\n";
     s += body0;
     s += fmt("        ph_st((%s *)A.out, base%d + c * %s, %s);\n", stype(out_dt), n_in,
              lit_i64(inner_stride(n_in)).c_str(), res0.c_str());
@@ -1219,7 +1236,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
 }
 
 int check_inputs(const ph_prog *prog, int32_t n_in, const void *const *ins,
-                 const ph_view_desc *in_descs, int *in_dt) {
+                 const ph_view_desc *in_descs, int *in_dt, bool allow_index = false) {
   if (n_in < 1 || n_in > PH_MAX_INPUTS)
     return ph::set_error(PH_E_ARGUMENT, "%d inputs (1..%d)", n_in, PH_MAX_INPUTS);
   for (int j = 0; j < n_in; ++j) {
@@ -1233,7 +1250,7 @@ int check_inputs(const ph_prog *prog, int32_t n_in, const void *const *ins,
       if (in_descs[j].view.shape[i] != in_descs[0].view.shape[i])
         return ph::set_error(PH_E_SHAPE, "input %d shape differs on axis %d", j, i);
   }
-  return validate_prog(prog, n_in, in_dt);
+  return validate_prog(prog, n_in, in_dt, allow_index);
 }
 
 }  // namespace
@@ -1248,7 +1265,7 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   if (n_in < 1 || n_in > PH_MAX_INPUTS) return ph::set_error(PH_E_ARGUMENT, "%d inputs", n_in);
   int in_dt[PH_MAX_INPUTS];
   for (int j = 0; j < n_in; ++j) in_dt[j] = in_dtypes[j];
-  int rc = validate_prog(prog, n_in, in_dt);
+  int rc = validate_prog(prog, n_in, in_dt, kind == 0);
   if (rc) return rc;
   if (rows < 1 || cols < 1) return ph::set_error(PH_E_ARGUMENT, "empty shape");
   if (kind == 0) {
@@ -1306,7 +1323,7 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   PH_REQUIRE_INIT();
   if (flags_out) *flags_out = 0;
   int in_dt[PH_MAX_INPUTS];
-  int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
+  int rc = check_inputs(prog, n_in, ins, in_descs, in_dt, true);
   if (rc) return rc;
   rc = ph::validate_desc(out_desc, "output");
   if (rc) return rc;
@@ -1318,6 +1335,9 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   for (int i = 0; i < out_desc->view.rank; ++i)
     if (out_desc->view.shape[i] != in_descs[0].view.shape[i])
       return ph::set_error(PH_E_SHAPE, "output shape differs on axis %d", i);
+  if (prog_uses_index(prog) && ph::desc_count(out_desc) > INT32_MAX)
+    return ph::set_error(PH_E_OVERFLOW, "element index of %lld elements exceeds Int32",
+                         (long long)ph::desc_count(out_desc));
   if (ph::desc_count(out_desc) == 0) return PH_OK;
   // inputs with stages are gathered first
   ph_view_desc local[PH_MAX_INPUTS];

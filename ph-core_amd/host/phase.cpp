@@ -682,7 +682,7 @@ bool IndexRegion::operator==(const IndexRegion &o) const {
 // Expressions
 // ===========================================================================
 struct ExprNode {
-  enum Kind { Input, Const, Op } kind = Const;
+  enum Kind { Input, Const, Op, Index } kind = Const;
   int op = 0;          // ph_opcode
   DType dt = DType::I32;      // result dtype
   DType opdt = DType::I32;    // dtype the operation computes in
@@ -709,6 +709,13 @@ Expr Expr::constant(const Scalar &s) {
   n.kind = ExprNode::Const;
   n.value = s;
   n.dt = s.dt;
+  return Expr{mk(n)};
+}
+
+Expr Expr::index() {
+  ExprNode n;
+  n.kind = ExprNode::Index;
+  n.dt = DType::I32;
   return Expr{mk(n)};
 }
 
@@ -873,7 +880,7 @@ static std::shared_ptr<const ExprNode> subst(const std::shared_ptr<const ExprNod
     if (compute_dtype(n->storage) != r.dtype()) return cast_to(r, compute_dtype(n->storage)).node;
     return r.node;
   }
-  if (n->kind == ExprNode::Const) return n;
+  if (n->kind == ExprNode::Const || n->kind == ExprNode::Index) return n;
   ExprNode c = *n;
   c.a = subst(n->a, ex);
   c.b = subst(n->b, ex);
@@ -893,6 +900,7 @@ int Expr::arity() const { return arity_of(node); }
 static std::string to_s_node(const std::shared_ptr<const ExprNode> &n) {
   if (n->kind == ExprNode::Input) return "x" + std::to_string(n->input);
   if (n->kind == ExprNode::Const) return n->value.to_s();
+  if (n->kind == ExprNode::Index) return "idx";
   std::string s = "(op" + std::to_string(n->op) + " " + to_s_node(n->a);
   if (n->b) s += " " + to_s_node(n->b);
   if (n->c) s += " " + to_s_node(n->c);
@@ -933,6 +941,13 @@ struct Lowering {
       in.op = PH_OP_LOAD;
       in.dt = (uint8_t)compute_dtype(n->storage);
       in.a = (uint8_t)n->input;
+      in.dst = (uint8_t)alloc();
+      emit(in);
+      return in.dst;
+    }
+    if (n->kind == ExprNode::Index) {
+      in.op = PH_OP_INDEX;
+      in.dt = (uint8_t)DType::I32;
       in.dst = (uint8_t)alloc();
       emit(in);
       return in.dst;
@@ -2120,6 +2135,46 @@ NArray &NArray::map_inplace(const std::function<Expr(const Expr &)> &fn) {
   check(ph_map(&prog, 1, ptrs, &d, sp, &d, &flags), "map!");
   check_flags(flags, "map!");
   return *this;
+}
+
+NArray NArray::map_with_index(const std::function<Expr(const Expr &, const Expr &)> &fn) const {
+  return map(operand(), [&](const Expr &x) { return fn(x, Expr::index()); });
+}
+
+NArray &NArray::map_with_index_inplace(const std::function<Expr(const Expr &, const Expr &)> &fn) {
+  return map_inplace([&](const Expr &x) { return fn(x, Expr::index()); });
+}
+
+// Coordinates follow from the lexicographic index by floored division and
+// modulo with the axis strides of a contiguous array of `shape`: the same
+// coordinate LexIterator.cover (src/n_array.cr:643) reaches after idx steps.
+std::vector<Expr> coord_exprs(const Shape &shape, const Expr &idx) {
+  std::vector<Expr> c(shape.size(), idx);
+  int64_t stride = 1;
+  for (size_t k = shape.size(); k-- > 0;) {
+    Expr q = stride == 1 ? idx : Expr::binary("//", idx, Expr::constant(Scalar::i32(stride)));
+    c[k] = k == 0 ? q : Expr::binary("%", q, Expr::constant(Scalar::i32(shape[k])));
+    stride *= std::max<int64_t>(shape[k], 1);
+    if (stride > INT32_MAX) stride = INT32_MAX;  // only reached by shapes the Int32 index rejects
+  }
+  return c;
+}
+
+NArray map_with_coord(const Operand &a,
+                      const std::function<Expr(const Expr &, const std::vector<Expr> &, const Expr &)> &fn) {
+  return map(a, [&](const Expr &x) {
+    Expr i = Expr::index();
+    return fn(x, coord_exprs(a.shape, i), i);
+  });
+}
+
+NArray &NArray::map_with_coord_inplace(
+    const std::function<Expr(const Expr &, const std::vector<Expr> &, const Expr &)> &fn) {
+  Shape sh = shape();
+  return map_inplace([&](const Expr &x) {
+    Expr i = Expr::index();
+    return fn(x, coord_exprs(sh, i), i);
+  });
 }
 
 NArray eq(const Operand &a, const Operand &b) {

@@ -185,6 +185,9 @@ class Expr {
 
   static Expr input(int k, DType storage);  // element of operand k
   static Expr constant(const Scalar &s);
+  // The element's lexicographic index (Int32), as Array#map_with_index yields
+  // it; only elementwise maps define it.
+  static Expr index();
   DType dtype() const;
   // Crystal operator names: "+", "-", "*", "/", "//", "%", "**", "&+", "&-",
   // "&*", "&**", "&", "|", "^", ">", "<", ">=", "<=", "==", "!=", "<=>"
@@ -336,6 +339,13 @@ class NArray {
 
   // in-place map (map!, n_array.cr:622-628)
   NArray &map_inplace(const std::function<Expr(const Expr &)> &fn);
+  // map_with_index / map_with_index! (n_array.cr:605-611, :632-638): the block
+  // sees each element and its lexicographic index (Int32).
+  NArray map_with_index(const std::function<Expr(const Expr &, const Expr &)> &fn) const;
+  NArray &map_with_index_inplace(const std::function<Expr(const Expr &, const Expr &)> &fn);
+  // map_with_coord! (n_array.cr:642-650): element, coordinate, index.
+  NArray &map_with_coord_inplace(
+      const std::function<Expr(const Expr &, const std::vector<Expr> &, const Expr &)> &fn);
 };
 
 // Lazy view (src/view.cr): source storage + a transform chain, folded into a
@@ -398,6 +408,14 @@ NArray elementwise(const std::string &op, const Operand &a, const Scalar &b);
 NArray elementwise(const std::string &op, const Scalar &a, const Operand &b);  // patches/number.cr
 NArray elementwise_unary(const std::string &op, const Operand &a);
 NArray map(const Operand &a, const std::function<Expr(const Expr &)> &fn);
+// map_with_coord (src/multi_indexable.cr:708-712, NArray src/n_array.cr:614-
+// 618): the block sees each element, its coordinate (Int32 per axis) and its
+// lexicographic index.
+NArray map_with_coord(const Operand &a,
+                      const std::function<Expr(const Expr &, const std::vector<Expr> &, const Expr &)> &fn);
+// The coordinate of the element at lexicographic index `idx` in `shape`, one
+// Int32 expression per axis.
+std::vector<Expr> coord_exprs(const Shape &shape, const Expr &idx);
 // map_with over several operands of one shape (src/multi_indexable.cr:1026-1078)
 NArray map_with(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn);
 // eq (:899-913): elementwise equality, DimensionError on shape mismatch

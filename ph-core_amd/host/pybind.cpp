@@ -137,6 +137,21 @@ Expr as_expr(const py::handle &o) {
   return Expr::constant(to_scalar(o));
 }
 
+// Calls a map_with_coord block with (element, coordinate) or (element,
+// coordinate, index), as many as it declares: a Crystal block ignores yielded
+// values it has no parameter for.  The coordinate is a list of Int32
+// expressions, one per axis.
+Expr call_coord_block(const py::function &fn, const Expr &x, const std::vector<Expr> &c, const Expr &i) {
+  py::list coord;
+  for (const Expr &e : c) coord.append(py::cast(e));
+  size_t nparams = 2;
+  try {
+    nparams = py::len(py::module_::import("inspect").attr("signature")(fn).attr("parameters"));
+  } catch (const py::error_already_set &) {
+  }
+  return nparams >= 3 ? as_expr(fn(x, coord, i)) : as_expr(fn(x, coord));
+}
+
 template <typename C>
 void def_operators(C &cls) {
   struct Op {
@@ -181,6 +196,11 @@ void def_operators(C &cls) {
   cls.def("equals", [](py::object self, py::object other) { return equals(to_operand(self), to_operand(other)); });
   cls.def("map", [](py::object self, py::function fn) {
     return map(to_operand(self), [&](const Expr &x) { return as_expr(fn(x)); });
+  });
+  cls.def("map_with_coord", [](py::object self, py::function fn) {
+    return map_with_coord(to_operand(self), [&](const Expr &x, const std::vector<Expr> &c, const Expr &i) {
+      return call_coord_block(fn, x, c, i);
+    });
   });
   cls.def("map_with", [](py::object self, py::args others, py::kwargs kw) {
     py::function fn = kw["block"].cast<py::function>();
@@ -375,6 +395,7 @@ PYBIND11_MODULE(_phase, m) {
   });
   m.def("placeholder", [](DType d) { return Expr::input(0, d); }, "the element of a one-input block");
   m.def("input", [](int k, DType d) { return Expr::input(k, d); }, "element of operand k");
+  m.def("element_index", []() { return Expr::index(); }, "the element's lexicographic index (Int32)");
   m.def("compile_check", [](const Expr &e, const std::vector<DType> &in_dt, int64_t rows, int64_t cols, int kind) {
     ph_prog p = e.compile((int)in_dt.size());
     std::vector<int32_t> d;
@@ -506,6 +527,17 @@ PYBIND11_MODULE(_phase, m) {
       .def("slices", &NArray::slices, py::arg("axis") = 0)
       .def("map_", [](NArray &a, py::function fn) -> NArray & {
         return a.map_inplace([&](const Expr &x) { return as_expr(fn(x)); });
+      }, py::return_value_policy::reference_internal)
+      .def("map_with_index", [](const NArray &a, py::function fn) {
+        return a.map_with_index([&](const Expr &x, const Expr &i) { return as_expr(fn(x, i)); });
+      })
+      .def("map_with_index_", [](NArray &a, py::function fn) -> NArray & {
+        return a.map_with_index_inplace([&](const Expr &x, const Expr &i) { return as_expr(fn(x, i)); });
+      }, py::return_value_policy::reference_internal)
+      .def("map_with_coord_", [](NArray &a, py::function fn) -> NArray & {
+        return a.map_with_coord_inplace([&](const Expr &x, const std::vector<Expr> &c, const Expr &i) {
+          return call_coord_block(fn, x, c, i);
+        });
       }, py::return_value_policy::reference_internal)
       .def("__repr__", [](const NArray &a) {
         return std::string("<NArray ") + dtype_name(a.dtype()) + " " + shape_to_s(a.shape()) + ">";

@@ -235,6 +235,23 @@ def test_element_programs_compile_for_gfx950(host):
     host.compile_check(two * host.input(2, host.DType.F64), [host.DType.F64] * 3, 16, 16, 0)
 
 
+def test_element_index_programs(host):
+    """A block that reads the element's lexicographic index (map_with_index,
+    n_array.cr:605-611) compiles into the map kernel in its row-tiled and its
+    flat form, is Int32-typed, and is refused by the fused reduction, which
+    has no element order to give it."""
+    i = host.element_index()
+    assert str(i.dtype).endswith("I32")
+    x = host.input(0, host.DType.F64)
+    e = x * 2 + i
+    assert str(e.dtype).endswith("F64")
+    host.compile_check(e, [host.DType.F64], 64, 96, 0)     # flat: rows of fewer than 256
+    host.compile_check(e, [host.DType.F64], 8, 3000, 0)    # row-tiled
+    host.compile_check((i // 7) % 3, [host.DType.F64], 4, 4, 0)  # no input read at all
+    with pytest.raises(host.ArgumentError, match="element index outside a map"):
+        host.compile_check(e, [host.DType.F64], 64, 1024, 1)
+
+
 def test_float_text_layout(host):
     """Float element text for JSON/YAML: shortest round-trip digits in
     Crystal's layout (parity unpinned: no reference spec prints a float; the

@@ -287,6 +287,29 @@ def test_tile_and_iteration_order():
     assert colex == g["colex"]
 
 
+# ------------------------------------------------- map_with_index / coord
+def test_map_with_index_and_coord_known_answers():
+    """map_with_index is map_with over (element, Int32 lexicographic index)
+    and map_with_coord over (element, coordinate per axis): the oracle's zip
+    evaluator given np.arange / np.indices as the extra operands reproduces
+    the reference specs (n_array_spec.cr:451-518) and the map_with_coord doc
+    example (multi_indexable.cr:700-707)."""
+    g = GOLD["narray_maps"]
+    stock = np.arange(6, dtype=np.int32).reshape(2, 3)
+    ci, cj = np.indices((2, 3), dtype=np.int32)
+    r, fl = O.eval_expr(trace(lambda el, i, j: 10 * i + j + el, 3), [stock, ci, cj])
+    assert r.reshape(-1).tolist() == g["with_coord"] and fl == 0
+    idx = np.arange(6, dtype=np.int32).reshape(2, 3)
+    r, _ = O.eval_expr(trace(lambda el, k: k + el, 2), [stock * 2, idx])
+    assert r.reshape(-1).tolist() == g["times3"]
+    doc = np.array(g["doc_coord_values"], np.int32).reshape(g["doc_coord_shape"])
+    r, _ = O.eval_expr(trace(lambda el, i, j: el + (i + j), 3), [doc, ci, cj])
+    assert r.reshape(-1).tolist() == g["doc_coord_sum"]
+    # a transposed block order would give these instead
+    bad, _ = O.eval_expr(trace(lambda el, i, j: 10 * i + j + el, 3), [stock, cj, ci])
+    assert bad.reshape(-1).tolist() != g["with_coord"]
+
+
 # ---------------------------------------------------------------- arithmetic
 def test_int32_known_answers_and_overflow():
     g = GOLD["narray_maps"]
