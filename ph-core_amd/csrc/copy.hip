@@ -161,6 +161,113 @@ __global__ __launch_bounds__(kBlock) void k_copy_general(E *__restrict__ dst, ph
   }
 }
 
+// Transposing copy through LDS: the source is contiguous along axis A and the
+// destination along another axis B (a permuted view materialised, or a
+// contiguous array scattered into a permuted MutableView).  A workgroup owns
+// a 64 x 64 tile of (A, B) for one coordinate of the other axes: it reads the
+// tile with lanes along A (64 consecutive source elements per wavefront
+// load), stages it in LDS (rows padded by one element, so the column reads
+// below hit distinct banks), and writes it with lanes along B (64 consecutive
+// destination elements per store).  Every element is read once and written
+// once, each as part of whole cache lines.
+struct TileArgs {
+  int64_t na, nb;    // extents along A and B
+  int64_t ta, tb;    // 64-element tiles along A and B
+  int64_t nblocks;   // tiles in all
+  int64_t s_a, s_b;  // source strides along A (+-1) and B
+  int64_t d_a, d_b;  // destination strides along A and B (+-1)
+  int64_t d_off, s_off;
+  int32_t orank, pad_;
+  int64_t o_shape[PH_MAX_RANK], o_dst[PH_MAX_RANK], o_src[PH_MAX_RANK];
+};
+
+constexpr int kTileT = 64;
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_copy_tiled(E *__restrict__ dst, const E *__restrict__ src,
+                                                       TileArgs a) {
+  constexpr int kRows = kTileT / (kBlock / 64);  // tile rows per wavefront
+  __shared__ E lds[kTileT][kTileT + 1];          // [B][A]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t per = a.ta * a.tb;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    int64_t o = b / per;
+    const int64_t rem = b - o * per;
+    const int64_t ib = rem / a.ta, ia = rem - ib * a.ta;
+    int64_t db = a.d_off, sb = a.s_off;
+    for (int k = a.orank - 1; k >= 0; --k) {
+      const int64_t n = a.o_shape[k];
+      const int64_t q = o / n;
+      const int64_t c = o - q * n;
+      o = q;
+      db += c * a.o_dst[k];
+      sb += c * a.o_src[k];
+    }
+    const int64_t a0 = ia * kTileT, b0 = ib * kTileT;
+    // read: lane along A, wavefront w takes tile rows w, w + 4, ...
+    const int64_t ar = a0 + lane;
+    const E *sp = src + sb + ar * a.s_a + (b0 + wave) * a.s_b;
+    E v[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+      v[j] = E{};
+      if (ar < a.na && b0 + wave + 4 * j < a.nb) v[j] = sp[(int64_t)(4 * j) * a.s_b];
+    }
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) lds[wave + 4 * j][lane] = v[j];
+    __syncthreads();
+    // write: lane along B, wavefront w takes tile columns w, w + 4, ...
+    const int64_t bw = b0 + lane;
+    E *dp = dst + db + bw * a.d_b + (a0 + wave) * a.d_a;
+#pragma unroll
+    for (int j = 0; j < kRows; ++j)
+      if (bw < a.nb && a0 + wave + 4 * j < a.na)
+        __builtin_nontemporal_store(lds[lane][wave + 4 * j], dp + (int64_t)(4 * j) * a.d_a);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+// The (A, B) axes of a transposing copy over the collapsed layout (operand 0
+// the destination, 1 the source), or false when the innermost axis already
+// serves both operands or either operand has no unit-stride axis.  Both
+// extents must fill a good part of a tile.
+bool tiled_axes(const ph::Layout &L, int *A, int *B) {
+  if (L.rank < 2) return false;
+  auto unit = [](int64_t s) { return s == 1 || s == -1; };
+  const int q = L.rank - 1;
+  if (unit(L.stride[0][q]) && unit(L.stride[1][q])) return false;
+  *A = *B = -1;
+  for (int k = 0; k < L.rank; ++k) {
+    if (unit(L.stride[1][k]) && *A < 0) *A = k;
+    if (unit(L.stride[0][k]) && *B < 0) *B = k;
+  }
+  return *A >= 0 && *B >= 0 && *A != *B && L.shape[*A] >= 16 && L.shape[*B] >= 16;
+}
+
+void make_tile_args(const ph::Layout &L, int A, int B, TileArgs *t) {
+  std::memset(t, 0, sizeof(*t));
+  t->na = L.shape[A];
+  t->nb = L.shape[B];
+  t->ta = (t->na + kTileT - 1) / kTileT;
+  t->tb = (t->nb + kTileT - 1) / kTileT;
+  t->s_a = L.stride[1][A];
+  t->s_b = L.stride[1][B];
+  t->d_a = L.stride[0][A];
+  t->d_b = L.stride[0][B];
+  t->d_off = L.offset[0];
+  t->s_off = L.offset[1];
+  int64_t others = 1;
+  for (int k = 0; k < L.rank; ++k) {
+    if (k == A || k == B) continue;
+    t->o_shape[t->orank] = L.shape[k];
+    t->o_dst[t->orank] = L.stride[0][k];
+    t->o_src[t->orank] = L.stride[1][k];
+    ++t->orank;
+    others *= L.shape[k];
+  }
+  t->nblocks = others * t->ta * t->tb;
+}
+
 // One thread, lexicographic order: the aliasing-safe copy.
 template <typename E>
 __global__ void k_copy_sequential(E *dst, ph_view_desc dd, const E *src, ph_view_desc sd,
@@ -242,6 +349,14 @@ int launch_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_vie
   ph::Layout L;
   ph::collapse(descs, 2, &L);
   constexpr int EPT = sizeof(E) >= 8 ? 4 : 8;
+  int ax_a, ax_b;
+  if (src && tiled_axes(L, &ax_a, &ax_b)) {
+    TileArgs t;
+    make_tile_args(L, ax_a, ax_b, &t);
+    hipLaunchKernelGGL(k_copy_tiled<E>, dim3((unsigned)grid_for(t.nblocks)), dim3(kBlock), 0, st, (E *)dst,
+                       (const E *)src, t);
+    return ph::check_launch("k_copy_tiled");
+  }
   RowArgs a;
   make_row_args(L, 0, src ? 1 : -1, EPT, &a);
   if (a.cols * 4 < (int64_t)kBlock * EPT && a.rows > 1) {

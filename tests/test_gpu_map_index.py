@@ -46,6 +46,19 @@ def test_spec_known_answers(ph):
     assert _flat(stock.map_with_coord(lambda el, c, idx: idx * 100 + c[1])) == [0, 101, 202, 300, 401, 502]
 
 
+def test_coordinate_order_known_answers(ph):
+    """Element and coordinate pairs in lexicographic order, and in
+    colexicographic order as the lexicographic order of the transposed view
+    with each coordinate read back to front (elem_coord_iterator_spec.cr:
+    10-22)."""
+    g = GOLD["elem_coord_order"]
+    narr = ph.NArray.from_numpy(np.array(g["values"], np.int32))
+    enc = lambda el, c: el * 100 + c[0] * 10 + c[1]  # noqa: E731
+    assert _flat(narr.map_with_coord(enc)) == [e * 100 + c[0] * 10 + c[1] for e, c in g["lex"]]
+    got = _flat(narr.permute(1, 0).map_with_coord(lambda el, c: enc(el, c[::-1])))
+    assert got == [e * 100 + c[0] * 10 + c[1] for e, c in g["colex"]]
+
+
 def _oracle(fn, vals, bf16=False):
     """fn(el, coord, idx) over the host values, by the oracle's zip evaluator."""
     shape = vals.shape

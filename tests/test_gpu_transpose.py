@@ -1,0 +1,93 @@
+"""Permuted views at sizes that take the LDS-tiled transposing copy (the
+source contiguous along one axis, the destination along another): gathers
+of permuted, reversed and region views, and scatters through permuted
+MutableViews, for 1-, 2-, 4- and 8-byte elements, against the oracle's view
+gather and scatter (src/view_util/transforms.cr:118-305)."""
+import numpy as np
+import pytest
+
+import oracle_api as O
+from crystal import parse_literal
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ((64, 64), [1, 0]), ((100, 37), [1, 0]), ((16, 2000), [1, 0]), ((17, 16), [1, 0]),
+    ((3, 70, 130), [2, 0, 1]), ((3, 70, 130), [1, 2, 0]), ((70, 3, 130), [2, 1, 0]),
+    ((5, 40, 33), [0, 2, 1]), ((2, 1, 300, 17), [3, 2, 1, 0]), ((1000, 1000), [1, 0]),
+]
+KINDS = ["i32", "f64", "bf16", "bool", "f32"]
+
+
+def _host(rng, shape, kind):
+    if kind == "i32":
+        return rng.integers(-2 ** 31, 2 ** 31, shape, dtype=np.int64).astype(np.int32)
+    if kind == "f64":
+        return rng.standard_normal(shape)
+    if kind == "f32":
+        return rng.standard_normal(shape).astype(np.float32)
+    if kind == "bf16":
+        return rng.integers(0, 2 ** 16, shape, dtype=np.int64).astype(np.uint16)
+    return rng.random(shape) < 0.5
+
+
+def _dev(ph, host, kind):
+    if kind == "bf16":
+        return ph.NArray.from_numpy(host, ph.DType.BF16)
+    return ph.NArray.from_numpy(host)
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint8) if a.dtype == np.bool_ else a.view(f"u{a.itemsize}")
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_permuted_gathers(ph, kind):
+    rng = np.random.default_rng(51)
+    for shape, perm in CASES:
+        host = _host(rng, shape, kind)
+        dev = _dev(ph, host, kind)
+        vshape = [shape[p] for p in perm]
+        got = dev.permute(*perm).to_narr().to_numpy()
+        exp = O.view_gather(host, [("permute", perm)], vshape)
+        assert np.array_equal(_bits(got), _bits(exp)), (shape, perm, kind)
+        # reversed, then permuted: descending unit strides on both sides
+        chain = [("permute", perm), ("reverse", list(shape))]
+        got = dev.view().reverse().permute(perm).to_narr().to_numpy()
+        assert np.array_equal(_bits(got), _bits(O.view_gather(host, chain, vshape))), (shape, perm, kind)
+
+
+def test_region_then_permute(ph):
+    rng = np.random.default_rng(53)
+    host = rng.integers(-1000, 1000, (300, 257), dtype=np.int64).astype(np.int32)
+    dev = ph.NArray.from_numpy(host)
+    for lits in [("3..290", ".."), ("..", "5..200"), ("-1..0", "1..2..256"), ("10..3..299", "-2..-1..0")]:
+        r = O.region([parse_literal(x) for x in lits], list(host.shape))
+        chain = [("permute", [1, 0]), ("region", r)]
+        got = dev.view(*lits).permute(1, 0).to_narr().to_numpy()
+        exp = O.view_gather(host, chain, [r.shape[1], r.shape[0]])
+        assert np.array_equal(got, exp), lits
+
+
+@pytest.mark.parametrize("kind", ["i32", "f64", "bf16", "bool"])
+def test_scatter_through_permuted_mutable_view(ph, kind):
+    rng = np.random.default_rng(57)
+    for shape, perm in CASES[:9]:
+        host = _host(rng, shape, kind)
+        vshape = [shape[p] for p in perm]
+        src = _host(rng, vshape, kind)
+        target = _dev(ph, host, kind)
+        mv = target.mutable_view().permute(*perm)
+        mv[[]] = _dev(ph, src, kind)
+        ref = O.view_scatter(host.copy(), [("permute", perm)], O.region_cover(vshape), src)
+        assert np.array_equal(_bits(target.to_numpy()), _bits(ref)), (shape, perm, kind)
+
+
+def test_large_transpose(ph):
+    """8192 x 8192 Int32, whole and offset by a region: every element checked
+    against numpy's transpose (which the oracle's view gather is pinned to)."""
+    x = ph.NArray.random([8192, 8192], ph.DType.I32, seed=1, dist="int31")
+    h = x.to_numpy()
+    assert np.array_equal(x.permute(1, 0).to_narr().to_numpy(), h.T)
+    assert np.array_equal(x.view("1..8190", "7..").permute(1, 0).to_narr().to_numpy(), h[1:8191, 7:].T)

@@ -308,6 +308,15 @@ def test_map_with_index_and_coord_known_answers():
     # a transposed block order would give these instead
     bad, _ = O.eval_expr(trace(lambda el, i, j: 10 * i + j + el, 3), [stock, cj, ci])
     assert bad.reshape(-1).tolist() != g["with_coord"]
+    # np.indices walks coordinates in the order ElemAndCoordIterator does
+    # (elem_coord_iterator_spec.cr:10-22)
+    o = GOLD["elem_coord_order"]
+    v = np.array(o["values"], np.int32)
+    pairs = [[int(e), [int(a), int(b)]] for e, a, b in zip(v.flat, *[c.flat for c in np.indices(v.shape)])]
+    assert pairs == o["lex"]
+    t = np.ascontiguousarray(v.T)
+    pairs = [[int(e), [int(b), int(a)]] for e, a, b in zip(t.flat, *[c.flat for c in np.indices(t.shape)])]
+    assert pairs == o["colex"]
 
 
 # ---------------------------------------------------------------- arithmetic
