@@ -184,6 +184,40 @@ def cpu_c3(x):
                       f"slice reductions, {dt * 1e3:.0f} ms; about {dt * 16384 / rows:.1f} s for the full array"}
 
 
+def c3_errors(x, storage, res):
+    """Relative error of the device's six-result pass against the same block
+    evaluated by numpy in Crystal's types ((x+4)**2 in Float32, then * 0.5
+    and - x in Float64) and summed in 80-bit long double, 1024 rows at a
+    time; maxima are compared bit for bit."""
+    import numpy as np
+    if storage == "BF16":
+        x = (x.astype(np.uint32) << 16).view(np.float32)
+    n, m = x.shape
+    rows = np.zeros(n, np.longdouble)
+    cols = np.zeros(m, np.longdouble)
+    rmax = np.empty(n)
+    cmax = np.full(m, -np.inf)
+    for r0 in range(0, n, 1024):
+        xs = x[r0:r0 + 1024]
+        t = xs + np.float32(4)
+        f = (t * t).astype(np.float64) * 0.5 - xs.astype(np.float64)
+        fl = f.astype(np.longdouble)
+        rows[r0:r0 + 1024] = fl.sum(axis=1)
+        cols += fl.sum(axis=0)
+        rmax[r0:r0 + 1024] = f.max(axis=1)
+        cmax = np.maximum(cmax, f.max(axis=0))
+    full = rows.sum()
+
+    def rel(got, want):
+        got = np.asarray(got, np.longdouble)
+        return float(np.max(np.abs(got - want) / np.abs(want)))
+
+    return {"max_rel_err_vs_fp64": {"full_sum": rel(res["full_sum"], full), "row_sum": rel(res["row_sum"], rows),
+                                    "col_sum": rel(res["col_sum"], cols)},
+            "maxima_bit_exact": bool(res["full_max"] == rmax.max() and np.array_equal(res["row_max"], rmax)
+                                     and np.array_equal(res["col_max"], cmax))}
+
+
 def cpu_c4(u0):
     """The heat step restated by the oracle on the full 8192^2 FP64 grid for
     2 steps, 1 core; Gcell/s."""
@@ -373,13 +407,16 @@ def other_workloads(phase, ev, args):
         pv.reduce2d_all_device()
         ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10, phase)
         by = esz * n * n
+        xh = x.to_numpy()
         if not args.no_cpu and st == "F32":
-            out["c3_cpu"] = cpu_c3(x.to_numpy())
+            out["c3_cpu"] = cpu_c3(xh)
         out[f"c3_procview_reduce_{st.lower()}_16384"] = {
             "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
             "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
             "traffic": tr.get(f"ph_red_tile_{st.lower()}"),
+            **c3_errors(xh, st, pv.reduce2d_all()),
             "note": "one pass: sum and max in full, along slices(0) and slices(1); FP64 sums inside each tile, compensated FP32 pairs across tiles"}
+        del xh
         del x, pv
 
     # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64
@@ -464,7 +501,90 @@ def other_workloads(phase, ev, args):
     t2 = time.perf_counter()
     out["json_f64_1024x1024"] = {"to_json_ms": round((t1 - t0) * 1e3, 1), "from_json_ms": round((t2 - t1) * 1e3, 1),
                                  "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing, 1 thread"}
+    out["path_ops_i32_8192"] = path_ops(phase, ev, args)
     return out
+
+
+def _cpu_leg(fn, by, what, size="full 8192^2"):
+    """One timed call of the oracle's restatement (1 core); `by` are the
+    algorithmic bytes of the sample."""
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    return {"value": round(by / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{what}, {size}, {dt * 1e3:.0f} ms"}
+
+
+def path_ops(phase, ev, args):
+    """The other operations of the path on Int32 8192 x 8192 arrays:
+    transposed gather and scatter (permuted View / MutableView), mask
+    assignment with a scalar and with a source array, elementwise equality
+    and map_with_coord.  Bytes are algorithmic: each element read or written
+    once, Bool masks and results 1 byte."""
+    import numpy as np
+    D = phase.DType
+    n = 8192
+    nn = n * n
+    x = phase.NArray.random([n, n], D.I32, seed=1, dist="int31")
+    y = phase.NArray.random([n, n], D.I32, seed=5, dist="int31")
+    z = phase.NArray.random([n, n], D.I32, seed=6, dist="int31")
+    mask = x.process(lambda v: (v & 1) == 0).to_narr()
+    res = {}
+
+    def entry(name, kernel, fn, by, cpu=None):
+        fn()
+        ms = time_loop(ev, fn, 10, phase)
+        e = {"kernel": kernel, "ms": round(ms, 4), "bytes": by, "achieved_gbs": round(by / ms / 1e6, 1),
+             "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
+        if cpu is not None and not args.no_cpu:
+            e["cpu"] = cpu()
+        res[name] = e
+
+    O = _oracle() if not args.no_cpu else None
+    xh = x.to_numpy() if O else None
+    # the oracle's transposes walk the view element by element: its legs
+    # take the first 1024 of the view's 8192 rows
+    rows = 1024
+    if O:
+        from crystal import parse_literal
+        r = O.region([parse_literal(f"0...{rows}"), parse_literal("..")], [n, n])
+    entry("transpose_gather", "k_copy_tiled (LDS 64x64 tiles)", lambda: x.view().permute(1, 0).to_narr(), 8 * nn,
+          lambda: _cpu_leg(lambda: O.view_gather(xh, [("region", r), ("permute", [1, 0])], [rows, n]),
+                           8 * rows * n, "view_gather of x.permute(1, 0)", f"its first {rows} rows"))
+    zt = z.mutable_view().permute(1, 0)
+
+    def scat():
+        zt[[]] = y
+    yh = y.to_numpy() if O else None
+    entry("transpose_scatter", "k_copy_tiled (LDS 64x64 tiles)", scat, 8 * nn,
+          lambda: _cpu_leg(lambda: O.view_scatter(xh.copy(), [("permute", [1, 0])], r, yh[:rows]),
+                           8 * rows * n, "view_scatter through z.mutable_view.permute(1, 0)",
+                           f"its first {rows} rows"))
+    mh = mask.to_numpy() if O else None
+
+    def ms_scalar():
+        z[mask] = 7
+    entry("set_mask_scalar", "ph_map_kernel (select)", ms_scalar, 9 * nn,
+          lambda: _cpu_leg(lambda: O.set_mask(xh.copy(), mh, 7), 9 * nn, "NArray#[]=(mask, 7)"))
+
+    def ms_src():
+        z[mask] = y
+    entry("set_mask_source", "ph_map_kernel (select)", ms_src, 13 * nn,
+          lambda: _cpu_leg(lambda: O.set_mask(xh.copy(), mh, yh), 13 * nn, "NArray#[]=(mask, y)"))
+    entry("eq", "ph_map_kernel", lambda: x.eq(y), 9 * nn,
+          lambda: _cpu_leg(lambda: O.eval_expr("(eq x0 x1)", [xh, yh]), 9 * nn, "x.eq(y)"))
+
+    def cpu_coord():
+        from crystal import trace
+        ci, cj = np.indices((n, n), dtype=np.int32)
+        t0 = time.perf_counter()
+        O.eval_expr(trace(lambda el, i, j: el.wadd(i).wsub(j), 3), [xh, ci, cj])
+        dt = time.perf_counter() - t0
+        return {"value": round(8 * nn / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                "sample": f"map_with_coord (element and coordinate zipped), full 8192^2, {dt * 1e3:.0f} ms"}
+    entry("map_with_coord", "ph_map_kernel (index instruction)",
+          lambda: x.map_with_coord(lambda el, c: el.wadd(c[0]).wsub(c[1])), 8 * nn, cpu_coord)
+    return res
 
 
 def batched_workload(phase, ev, args):

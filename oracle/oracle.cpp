@@ -720,6 +720,18 @@ int64_t or_view_scatter(void *dst, const int64_t *dst_shape, int dst_rank,
   return k;
 }
 
+// NArray#[]=(mask, value) (src/n_array.cr:510-519): walk the mask's buffer
+// in index order and store `value` where it is true; and the form with a
+// source array (:523-536), whose lexicographic walk over a row-major mask,
+// source and destination of one shape is the same buffer index order.
+void or_set_mask(void *dst, int64_t n, int esize, const unsigned char *mask, const void *src,
+                 int src_is_array) {
+  unsigned char *d = (unsigned char *)dst;
+  const unsigned char *s = (const unsigned char *)src;
+  for (int64_t idx = 0; idx < n; ++idx)
+    if (mask[idx]) std::memcpy(d + idx * esize, s + (src_is_array ? idx * esize : 0), (size_t)esize);
+}
+
 // MultiIndexable#tile (src/multi_indexable.cr:818-827) with TilingLexIterator
 // (src/multi_indexable/tiling_lex_iterator.cr:27-45): out[c] = src[c mod shape].
 int64_t or_tile(const void *src, const int64_t *shape, int rank, int esize,

@@ -525,14 +525,27 @@ int read_flags(int *dflags, bool may_flag, int32_t *flags_out) {
 }
 
 // Whether any element of `p` can raise a PH_FLAG_* condition.
+// Whether any element can raise a flag.  Int32 // and % by a constant other
+// than 0 and -1 cannot (the divisor is known when the program is built, so
+// neither division by zero nor the MIN / -1 overflow can occur): a
+// coordinate computed from the element index then costs no read-back.
 bool prog_may_flag(const ph_prog *p) {
+  bool isc[PH_NREG] = {false};
+  int64_t cv[PH_NREG] = {0};
   for (int k = 0; k < p->n_insn; ++k) {
     const ph_insn &in = p->insn[k];
     if (in.op == PH_OP_IPOW || in.op == PH_OP_WIPOW) return true;
+    if (in.dt == PH_I32 && (in.op == PH_OP_FLOORDIV || in.op == PH_OP_MOD)) {
+      const bool safe = in.b < PH_NREG && isc[in.b] && cv[in.b] != 0 && cv[in.b] != -1;
+      if (!safe) return true;
+    }
     if (in.dt == PH_I32 && (in.op == PH_OP_ADD || in.op == PH_OP_SUB || in.op == PH_OP_MUL ||
-                            in.op == PH_OP_FLOORDIV || in.op == PH_OP_MOD || in.op == PH_OP_NEG ||
-                            in.op == PH_OP_ABS))
+                            in.op == PH_OP_NEG || in.op == PH_OP_ABS))
       return true;
+    if (in.dst < PH_NREG) {
+      isc[in.dst] = in.op == PH_OP_CONST && in.dt == PH_I32;
+      cv[in.dst] = in.imm.i;
+    }
   }
   return false;
 }

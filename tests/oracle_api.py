@@ -94,6 +94,8 @@ def lib():
         L.or_view_scatter.argtypes = [C.c_void_p, i64p, C.c_int, C.c_int, C.POINTER(Transform), C.c_int, C.POINTER(Region), C.c_void_p, C.c_int64]
         L.or_view_scatter.restype = C.c_int64
         L.or_tile.argtypes = [C.c_void_p, i64p, C.c_int, C.c_int, i64p, C.c_void_p]
+        L.or_set_mask.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.or_set_mask.restype = None
         L.or_tile.restype = C.c_int64
         L.or_binop.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
         L.or_unop.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
@@ -312,6 +314,17 @@ def view_scatter(dst, chain, r, src):
     src = np.ascontiguousarray(src, dtype=dst.dtype).reshape(-1)
     lib().or_view_scatter(_ptr(dst), _i64(dst.shape), dst.ndim, dst.itemsize, _chain(chain), len(chain),
                           C.byref(r), _ptr(src), src.size)
+    return dst
+
+
+def set_mask(dst, mask, value):
+    """NArray#[]=(mask, value) in place on a contiguous numpy array; `value`
+    is a scalar or an array of dst's shape."""
+    assert dst.flags.c_contiguous and mask.shape == dst.shape
+    m = np.ascontiguousarray(mask, dtype=np.bool_).view(np.uint8)
+    is_arr = isinstance(value, np.ndarray)
+    v = np.ascontiguousarray(value, dtype=dst.dtype) if is_arr else np.array([value], dtype=dst.dtype)
+    lib().or_set_mask(_ptr(dst), dst.size, dst.itemsize, _ptr(m), _ptr(v), int(is_arr))
     return dst
 
 

@@ -181,3 +181,23 @@ def test_exact_product_fusion_edge_values(ph):
     fn = lambda v: v * 0.5 + 0.1  # noqa: E731
     exp, _ = O.eval_expr(trace(fn, 1), [i])
     assert np.array_equal(ph.NArray.from_numpy(i).process(fn).to_narr().to_numpy(), exp)
+
+
+def test_int32_division_by_constants(ph):
+    """Int32 // and % by a constant: 0 raises DivisionByZeroError, -1 raises
+    OverflowError on Int32::MIN // -1, any other constant cannot raise (the
+    device then skips the flag read-back); values against the oracle."""
+    from crystal import trace
+    h = np.array([5, -7, -2 ** 31, 2 ** 31 - 1, 0, 13, -13, 1], np.int32)
+    d = ph.NArray.from_numpy(h)
+    with pytest.raises(ph.DivisionByZeroError):
+        d.map(lambda v: v // 0)
+    with pytest.raises(ph.DivisionByZeroError):
+        d.map(lambda v: v % 0)
+    with pytest.raises(ph.OverflowError):
+        d.map(lambda v: v // -1)
+    for fn in [lambda v: v % -1, lambda v: v // 3, lambda v: v % 3, lambda v: v // -5, lambda v: v % -5,
+               lambda v: (v // 7) % 4, lambda v: v // (2 ** 31 - 1), lambda v: v % -2 ** 31]:
+        exp, fl = O.eval_expr(trace(fn), [h])
+        assert fl == 0
+        assert np.array_equal(d.map(fn).to_numpy(), exp)

@@ -287,6 +287,26 @@ def test_tile_and_iteration_order():
     assert colex == g["colex"]
 
 
+# ------------------------------------------------------------------ masks
+def test_set_mask_known_answers():
+    """NArray#[]=(mask, value) and (mask, source) against the reference's
+    spec (n_array_spec.cr:297-333), a mask of all false, and numpy.where on
+    random masks of every element size."""
+    m = GOLD["narray_masks"]
+    mask = np.array(m["mask"]).reshape(2, 3)
+    stock = np.arange(6, dtype=np.int32).reshape(2, 3)
+    assert O.set_mask(stock.copy(), mask, m["scalar"]).reshape(-1).tolist() == m["expected_scalar"]
+    assert O.set_mask(stock.copy(), mask, stock + m["src_plus"]).reshape(-1).tolist() == m["expected_src"]
+    assert O.set_mask(stock.copy(), np.zeros((2, 3), bool), 99).tolist() == stock.tolist()
+    rng = np.random.default_rng(5)
+    for dt in (np.uint8, np.uint16, np.int32, np.float64):
+        d = rng.integers(0, 100, (37, 41)).astype(dt)
+        mk = rng.random((37, 41)) < 0.3
+        src = rng.integers(100, 200, (37, 41)).astype(dt)
+        assert np.array_equal(O.set_mask(d.copy(), mk, 7), np.where(mk, dt(7), d))
+        assert np.array_equal(O.set_mask(d.copy(), mk, src), np.where(mk, src, d))
+
+
 # ------------------------------------------------- map_with_index / coord
 def test_map_with_index_and_coord_known_answers():
     """map_with_index is map_with over (element, Int32 lexicographic index)
