@@ -11,9 +11,14 @@
 // round-robin), which keeps the overlapping columns in that XCD's L2.
 //
 // The cell update keeps the operation order of examples/heat_equation.cr:47
-// and :50 extended to two axes, with no contraction into FMAs:
+// and :50 extended to two axes:
 //   d = (((uN - 4u) + uS) + uW) + uE;  u' = u + d * r
+// with one contraction that changes no result: 4u is exact unless it
+// overflows, so uN - 4u rounds once whether it is computed as a multiply and
+// a subtract or as one fused multiply-add.  The fused form runs only when the
+// host has shown that 4u cannot overflow in the passes ahead (fused_safe).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -48,6 +53,10 @@ __device__ __forceinline__ typename Vec<T>::type pack(const T *o) {
   }
   return v;
 }
+
+// Fused multiply-add in the cell's own precision.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Whole-wavefront lane shifts by one (DPP wave_shr:1 / wave_shl:1): lane i
 // receives lane i-1's (shr) or lane i+1's (shl) value; the lane at the end
@@ -114,7 +123,7 @@ __device__ __forceinline__ void static_for(F &&f) {
 // grid.  Each value is computed with the single-step formula, so K steps per
 // pass are bit-identical to K single steps.
 // ---------------------------------------------------------------------------
-template <typename T, int K, int kAhead, int kHalo>
+template <typename T, int K, int kAhead, int kHalo, bool kFused>
 __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict__ u, T *__restrict__ out,
                                                             HeatArgs a, T r, int vec_ok) {
   constexpr int V = Vec<T>::n;
@@ -145,7 +154,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict_
     for (int i = 0; i < V; ++i) {
       const T w = i == 0 ? wl : c[i - 1];
       const T e = i == V - 1 ? er : c[i + 1];
-      const T d = (((n[i] - four * c[i]) + s[i]) + w) + e;
+      T nc;
+      if constexpr (kFused) nc = fma_t(-four, c[i], n[i]);  // exact 4u: one rounding either way
+      else nc = n[i] - four * c[i];
+      const T d = ((nc + s[i]) + w) + e;
       o[i] = c[i] + d * r;
     }
   };
@@ -455,17 +467,17 @@ int64_t band_rows(int64_t ny, int64_t groups_per_band, int64_t resident, int k) 
 
 // `a` carries the grid and strip count; the band height is `ty` when
 // positive, else chosen by band_rows for this kernel's residency.
-template <typename T, int K, int A>
+template <typename T, int K, int A, bool F>
 int launch_ahead(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok) {
   constexpr int H = 2;  // halo lanes on each side of a wavefront's strip
-  const void *f = (const void *)k_heat_multi<T, K, A, H>;
+  const void *f = (const void *)k_heat_multi<T, K, A, H, F>;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
   a.ty = ty > 0 ? ty : band_rows(a.ny, groups_per_band, resident_groups(f), K);
   a.nbands = (a.ny + a.ty - 1) / a.ty;
   a.nblocks = groups_per_band * a.nbands;
   a.per_xcd = (a.nblocks + 7) / 8;
   const unsigned grid = (unsigned)(a.per_xcd * 8);
-  hipLaunchKernelGGL((k_heat_multi<T, K, A, H>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
+  hipLaunchKernelGGL((k_heat_multi<T, K, A, H, F>), dim3(grid), dim3(64 * kWaves), 0, ph::stream(), cur, nxt, a, r,
                      vec_ok);
   return ph::check_launch("k_heat_multi");
 }
@@ -473,28 +485,104 @@ int launch_ahead(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok) 
 // Rows loaded ahead: 8 for passes of one or two steps (little arithmetic per
 // row, so more loads must be in flight to keep HBM busy), else 3; `ahead`
 // (PH_HEAT_AHEAD) picks 2, 3 or, for such short passes, 8.
-template <typename T, int K>
+template <typename T, int K, bool F>
 int launch_multi(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok, int ahead) {
   if (ahead <= 0) ahead = K <= 2 ? 8 : 3;
   if constexpr (K <= 2) {
-    if (ahead >= 8) return launch_ahead<T, K, 8>(cur, nxt, a, ty, r, vec_ok);
+    if (ahead >= 8) return launch_ahead<T, K, 8, F>(cur, nxt, a, ty, r, vec_ok);
   }
-  if (ahead >= 3) return launch_ahead<T, K, 3>(cur, nxt, a, ty, r, vec_ok);
-  return launch_ahead<T, K, 2>(cur, nxt, a, ty, r, vec_ok);
+  if (ahead >= 3) return launch_ahead<T, K, 3, F>(cur, nxt, a, ty, r, vec_ok);
+  return launch_ahead<T, K, 2, F>(cur, nxt, a, ty, r, vec_ok);
+}
+
+template <typename T, bool F>
+int launch_kf(int k, const T *cur, T *nxt, const HeatArgs &a, int64_t ty, T r, int vec_ok, int ahead) {
+  switch (k) {
+    case 8: return launch_multi<T, 8, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 7: return launch_multi<T, 7, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 6: return launch_multi<T, 6, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 5: return launch_multi<T, 5, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 4: return launch_multi<T, 4, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 3: return launch_multi<T, 3, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    case 2: return launch_multi<T, 2, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+    default: return launch_multi<T, 1, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+  }
 }
 
 template <typename T>
-int launch_k(int k, const T *cur, T *nxt, const HeatArgs &a, int64_t ty, T r, int vec_ok, int ahead) {
-  switch (k) {
-    case 8: return launch_multi<T, 8>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 7: return launch_multi<T, 7>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 6: return launch_multi<T, 6>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 5: return launch_multi<T, 5>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 4: return launch_multi<T, 4>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 3: return launch_multi<T, 3>(cur, nxt, a, ty, r, vec_ok, ahead);
-    case 2: return launch_multi<T, 2>(cur, nxt, a, ty, r, vec_ok, ahead);
-    default: return launch_multi<T, 1>(cur, nxt, a, ty, r, vec_ok, ahead);
+int launch_k(int k, bool fused, const T *cur, T *nxt, const HeatArgs &a, int64_t ty, T r, int vec_ok, int ahead) {
+  return fused ? launch_kf<T, true>(k, cur, nxt, a, ty, r, vec_ok, ahead)
+               : launch_kf<T, false>(k, cur, nxt, a, ty, r, vec_ok, ahead);
+}
+
+// Largest |u| of the grid as an ordered bit pattern: with the sign bit
+// cleared, finite values order as their magnitudes, and infinities and NaN
+// order above every finite value.
+template <typename T> struct Bits;
+template <> struct Bits<float> {
+  typedef unsigned int U;
+  static constexpr U kMag = 0x7fffffffu;
+  static constexpr U kLimit = 251u << 23;  // 2^124
+  static constexpr double kEps = 0x1p-24;
+};
+template <> struct Bits<double> {
+  typedef unsigned long long U;
+  static constexpr U kMag = 0x7fffffffffffffffull;
+  static constexpr U kLimit = 2043ull << 52;  // 2^1020
+  static constexpr double kEps = 0x1p-53;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_absmax_bits(const T *__restrict__ u, int64_t n, typename Bits<T>::U *out) {
+  using U = typename Bits<T>::U;
+  const U *b = reinterpret_cast<const U *>(u);
+  U m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const U v = b[i] & Bits<T>::kMag;
+    m = v > m ? v : m;
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    const U o = __shfl_xor(m, off);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Whether the next passes may compute uN - 4u as one fused multiply-add:
+// 4u must not overflow in any of them.  For 0 <= r <= 1/4 the update is a
+// convex combination of a cell and its neighbours (weights 1 - 4r and r),
+// so the largest |u| cannot grow beyond rounding: each step's roundings add
+// at most 11 eps of it (d carries four roundings of terms summing to at most
+// 8 max|u|, and the product and the final sum one each).  Starting from
+// max|u| <= 2^124 (Float32) or 2^1020 (Float64), 4u stays finite for
+// ln 4 / (11 eps) steps (about 2.1 million for Float32); *steps_ok returns
+// a bound within that, after which the host checks again.  A grid holding
+// an infinity, a NaN or a larger value, or r outside [0, 1/4], keeps the
+// separate multiply and subtract.
+template <typename T>
+int fused_safe(const T *u, int64_t n, T r, bool *safe, int64_t *steps_ok) {
+  using U = typename Bits<T>::U;
+  *safe = false;
+  *steps_ok = 0;
+  if (!(r >= T(0) && r <= T(0.25))) return PH_OK;
+  void *p;
+  int rc = ph::scratch(64, &p, 2);
+  if (rc) return rc;
+  rc = ph::clear_i32((int *)p, 2);
+  if (rc) return rc;
+  rc = ph::launch_gate("heat max|u|");
+  if (rc) return rc;
+  const int64_t blocks = std::min<int64_t>(2048, (n + 255) / 256);
+  hipLaunchKernelGGL(k_absmax_bits<T>, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(256), 0, ph::stream(), u,
+                     n, (U *)p);
+  rc = ph::check_launch("k_absmax_bits");
+  if (rc) return rc;
+  U m = 0;
+  PH_HIP(hipMemcpyAsync(&m, p, sizeof(U), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  *safe = m <= Bits<T>::kLimit;
+  *steps_ok = (int64_t)(0.5 * std::log(4.0) / (11.0 * Bits<T>::kEps));
+  return PH_OK;
 }
 
 template <typename T>
@@ -523,6 +611,11 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   T *cur = u, *nxt = tmp;
   int64_t ei = 0;
   int rc;
+  // The fused form needs one read-back of max|u| (a synchronisation), so
+  // short runs keep the separate multiply and subtract.
+  const bool try_fused = steps >= 32;
+  bool fused = false;
+  int64_t fused_until = 0;
   if (energy && every > 0) {
     rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
     if (rc) return rc;
@@ -534,9 +627,15 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     int64_t until = steps;
     if (energy && every > 0) until = std::min<int64_t>(steps, (t / every + 1) * every);
     const int k = (int)std::min<int64_t>(kmax, until - t);
+    if (try_fused && t >= fused_until) {
+      int64_t span = 0;
+      rc = fused_safe<T>(cur, ny * nx, rr, &fused, &span);
+      if (rc) return rc;
+      fused_until = fused ? t + span : steps;
+    }
     rc = ph::launch_gate("heat step");
     if (rc) return rc;
-    rc = launch_k<T>(k, cur, nxt, a, ty, rr, vec_ok, (int)ahead);
+    rc = launch_k<T>(k, fused, cur, nxt, a, ty, rr, vec_ok, (int)ahead);
     if (rc) return rc;
     T *sw = cur;
     cur = nxt;

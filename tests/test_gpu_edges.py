@@ -201,3 +201,46 @@ def test_int32_division_by_constants(ph):
         exp, fl = O.eval_expr(trace(fn), [h])
         assert fl == 0
         assert np.array_equal(d.map(fn).to_numpy(), exp)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_heat_fused_first_operation(ph, dt):
+    """Runs of 32 steps or more compute uN - 4u as one fused multiply-add
+    once max|u| shows that 4u cannot overflow; the result must stay
+    bit-identical to the oracle's separate multiply and subtract, on grids
+    that take the clean, edge and masked sweeps."""
+    rng = np.random.default_rng(71)
+    for ny, nx, steps in [(200, 249, 40), (300, 1000, 33), (65, 257, 35), (1000, 2048, 40), (7, 1, 32)]:
+        u0 = (rng.standard_normal((ny, nx)) * 1e3).astype(dt)
+        u = ph.NArray.from_numpy(u0)
+        e = ph.heat2d(u, 0.2, steps, every=8)
+        ref, eref = O.heat2d(u0, 0.2, steps, every=8)
+        assert np.array_equal(u.to_numpy(), ref), (ny, nx, steps)
+        assert np.allclose(e, eref, rtol=1e-13, atol=1e-300), (ny, nx, steps)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_heat_unfused_when_4u_may_overflow(ph, dt):
+    """Grids whose 4u can overflow (values near the largest finite value),
+    that hold an infinity or a NaN, or that step with r > 1/4 (values may
+    grow) keep the separate multiply and subtract: bit-identical to the
+    oracle, infinities and NaN included."""
+    big = np.finfo(dt).max
+    rng = np.random.default_rng(73)
+    cases = []
+    u0 = rng.random((64, 300)).astype(dt)
+    u0[10, 20] = big * 0.9
+    u0[40, 100:104] = -big * 0.6
+    cases.append((u0, 0.2))
+    u1 = rng.random((64, 300)).astype(dt)
+    u1[5, 5] = np.inf
+    u1[50, 250] = np.nan
+    cases.append((u1, 0.2))
+    cases.append((rng.random((64, 300)).astype(dt), 0.3))
+    for u0, r in cases:
+        u = ph.NArray.from_numpy(u0)
+        ph.heat2d(u, r, 34, every=0)
+        ref, _ = O.heat2d(u0, r, 34, every=0)
+        got = u.to_numpy()
+        assert np.array_equal(got, ref, equal_nan=True), r
+        assert np.array_equal(np.signbit(got), np.signbit(ref)), r
