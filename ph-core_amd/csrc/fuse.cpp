@@ -577,6 +577,35 @@ std::string prog_key(const ph_prog *p) {
 
 // Materialises staged (reshape / tile) inputs into contiguous scratch so the
 // fused kernels only see affine operands.
+// A transposed affine view: its innermost axis (of extent 16 or more) steps
+// across cache lines while another axis of 16 or more elements is
+// contiguous.  A kernel whose lanes run along the innermost axis would touch
+// one line per element; one pass of the LDS-tiled transposing copy into
+// contiguous scratch (the path ph_strided_copy takes for this layout) costs
+// less than that.
+bool transposed_layout(const ph_view_desc &d) {
+  if (d.nstage != 0 || d.view.rank < 2) return false;
+  int q = -1;
+  for (int i = d.view.rank - 1; i >= 0; --i)
+    if (d.view.shape[i] != 1) {
+      q = i;
+      break;
+    }
+  if (q < 0 || d.view.shape[q] < 16) return false;
+  const int64_t sq = d.view.stride[q];
+  if (sq == 0 || sq == 1 || sq == -1) return false;
+  for (int i = 0; i < q; ++i)
+    if (d.view.shape[i] >= 16 && (d.view.stride[i] == 1 || d.view.stride[i] == -1)) return true;
+  return false;
+}
+
+// Whether the innermost axis (of extent above 1) of a view is contiguous.
+bool inner_contiguous(const ph_view_desc &d) {
+  for (int i = d.view.rank - 1; i >= 0; --i)
+    if (d.view.shape[i] != 1) return d.view.stride[i] == 1 || d.view.stride[i] == -1;
+  return true;
+}
+
 int materialize(const ph_view_desc *d, const void *src, int slot, ph_view_desc *nd, const void **np) {
   const int64_t n = ph::desc_count(d);
   void *buf;
@@ -1352,17 +1381,25 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
     return ph::set_error(PH_E_OVERFLOW, "element index of %lld elements exceeds Int32",
                          (long long)ph::desc_count(out_desc));
   if (ph::desc_count(out_desc) == 0) return PH_OK;
-  // inputs with stages are gathered first
+  // inputs with stages are gathered first, and transposed inputs (while
+  // the two scratch slots last) are copied through the tiled transpose
+  // when the output is contiguous along its innermost axis
   ph_view_desc local[PH_MAX_INPUTS];
   const void *ptrs[PH_MAX_INPUTS];
   int nstaged = 0;
   for (int j = 0; j < n_in; ++j) nstaged += in_descs[j].nstage > 0 ? 1 : 0;
   if (nstaged > 2) return ph::set_error(PH_E_UNSUPPORTED, "at most two reshaped/tiled inputs per map");
-  int slot = 0;
+  const bool out_inner = inner_contiguous(*out_desc);
+  int slot = 0, spare = 2 - nstaged;
   for (int j = 0; j < n_in; ++j) {
     local[j] = in_descs[j];
     ptrs[j] = ins[j];
-    if (in_descs[j].nstage > 0) {
+    bool copy = in_descs[j].nstage > 0;
+    if (!copy && out_inner && spare > 0 && transposed_layout(in_descs[j])) {
+      copy = true;
+      --spare;
+    }
+    if (copy) {
       rc = materialize(&in_descs[j], ins[j], slot++, &local[j], &ptrs[j]);
       if (rc) return rc;
     }
@@ -1439,7 +1476,7 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
   for (int j = 0; j < n_in; ++j) {
     local[j] = in_descs[j];
     ptrs[j] = ins[j];
-    if (in_descs[j].nstage > 0) {
+    if (in_descs[j].nstage > 0 || (j <= 1 && transposed_layout(in_descs[j]))) {
       if (j > 1) return ph::set_error(PH_E_UNSUPPORTED, "at most two reshaped/tiled inputs");
       rc = materialize(&in_descs[j], ins[j], j, &local[j], &ptrs[j]);
       if (rc) return rc;
@@ -1492,8 +1529,9 @@ int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const p
   for (int j = 0; j < n_in; ++j) {
     ph_view_desc d = local[j];
     bool need_copy = d.nstage > 0;
-    if (middle && !need_copy) {
-      // permute axis k to the front (a pure view change)
+    if (middle) {
+      // permute axis k to the front (a pure view change; for a staged view
+      // too, whose stages act on the linear index after the view's map)
       ph_view_desc p = d;
       p.view.shape[0] = d.view.shape[keep_axis];
       p.view.stride[0] = d.view.stride[keep_axis];
@@ -1516,6 +1554,7 @@ int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const p
         if (d.view.shape[i + 1] != 1 && d.view.stride[i] != d.view.stride[i + 1] * d.view.shape[i + 1])
           need_copy = true;
     }
+    if (!need_copy && j <= 1 && transposed_layout(d)) need_copy = true;
     if (need_copy) {
       if (j > 1) return ph::set_error(PH_E_UNSUPPORTED, "at most two non-collapsible inputs");
       rc = materialize(&d, ptrs[j], j, &local[j], &ptrs[j]);
