@@ -548,7 +548,7 @@ def path_ops(phase, ev, args):
     if O:
         from crystal import parse_literal
         r = O.region([parse_literal(f"0...{rows}"), parse_literal("..")], [n, n])
-    entry("transpose_gather", "k_copy_tiled (LDS 64x64 tiles)", lambda: x.view().permute(1, 0).to_narr(), 8 * nn,
+    entry("transpose_gather", "k_copy_tiled_v (LDS 128x128 tiles, 16-byte lanes)", lambda: x.view().permute(1, 0).to_narr(), 8 * nn,
           lambda: _cpu_leg(lambda: O.view_gather(xh, [("region", r), ("permute", [1, 0])], [rows, n]),
                            8 * rows * n, "view_gather of x.permute(1, 0)", f"its first {rows} rows"))
     zt = z.mutable_view().permute(1, 0)
@@ -556,7 +556,7 @@ def path_ops(phase, ev, args):
     def scat():
         zt[[]] = y
     yh = y.to_numpy() if O else None
-    entry("transpose_scatter", "k_copy_tiled (LDS 64x64 tiles)", scat, 8 * nn,
+    entry("transpose_scatter", "k_copy_tiled_v (LDS 128x128 tiles, 16-byte lanes)", scat, 8 * nn,
           lambda: _cpu_leg(lambda: O.view_scatter(xh.copy(), [("permute", [1, 0])], r, yh[:rows]),
                            8 * rows * n, "view_scatter through z.mutable_view.permute(1, 0)",
                            f"its first {rows} rows"))

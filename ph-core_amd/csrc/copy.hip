@@ -227,6 +227,85 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiled(E *__restrict__ dst, cons
   }
 }
 
+// The same with 16-byte lanes on both sides and a T x T tile (T = 128, or 64
+// for 8-byte elements): a wavefront reads and writes whole 512-byte (or
+// longer) runs of a row, which suits DRAM pages better than the 256-byte
+// runs of the tile above (measured cold, 8192^2 Int32: 110 us against 143
+// us).  The tile goes to LDS as [A][B] with rows padded by one vector, by
+// element from the loaded vectors, and leaves it by vector.  Needs unit
+// strides (+1) along A in the source and B in the destination, both extents
+// multiples of the vector, and every row start 16-byte aligned.
+template <typename E, int T>
+__global__ __launch_bounds__(kBlock) void k_copy_tiled_v(E *__restrict__ dst, const E *__restrict__ src,
+                                                         TileArgs a) {
+  typedef unsigned int V4 __attribute__((ext_vector_type(4)));
+  constexpr int VW = 16 / (int)sizeof(E);  // elements per 16-byte lane
+  constexpr int P = T + VW;               // LDS row (16-byte multiple)
+  constexpr int LPR = T / VW;             // lanes per tile row
+  constexpr int RPP = kBlock / LPR;       // tile rows per pass
+  constexpr int NP = T / RPP;             // passes
+  static_assert(kBlock % LPR == 0 && T % RPP == 0, "tile shape");
+  __shared__ __align__(16) E lds[T * P];
+  const int t = threadIdx.x;
+  const int q = t % LPR, r0 = t / LPR;
+  const int64_t per = a.ta * a.tb;
+  for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    int64_t o = b / per;
+    const int64_t rem = b - o * per;
+    const int64_t ib = rem / a.ta, ia = rem - ib * a.ta;
+    int64_t db = a.d_off, sb = a.s_off;
+    for (int k = a.orank - 1; k >= 0; --k) {
+      const int64_t n = a.o_shape[k];
+      const int64_t qq = o / n;
+      const int64_t c = o - qq * n;
+      o = qq;
+      db += c * a.o_dst[k];
+      sb += c * a.o_src[k];
+    }
+    const int64_t a0 = ia * T, b0 = ib * T;
+    // read: VW consecutive A elements per lane, tile rows r0, r0 + RPP, ...
+    const int64_t ar = a0 + (int64_t)q * VW;
+    V4 v[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      v[j] = V4{0u, 0u, 0u, 0u};
+      const int64_t bv = b0 + r0 + RPP * j;
+      if (ar < a.na && bv < a.nb) v[j] = *(const V4 *)(src + sb + ar + bv * a.s_b);
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      E e[VW];
+      __builtin_memcpy(e, &v[j], 16);
+      const int bl = r0 + RPP * j;
+#pragma unroll
+      for (int i = 0; i < VW; ++i) lds[(q * VW + i) * P + bl] = e[i];
+    }
+    __syncthreads();
+    // write: VW consecutive B elements per lane, tile rows (A) r0, r0 + RPP, ...
+    const int64_t bw = b0 + (int64_t)q * VW;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int al = r0 + RPP * j;
+      if (bw < a.nb && a0 + al < a.na)
+        __builtin_nontemporal_store(*(const V4 *)&lds[al * P + q * VW], (V4 *)(dst + db + bw + (a0 + al) * a.d_a));
+    }
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+// Whether k_copy_tiled_v applies: unit strides, extents in whole vectors and
+// 16-byte row starts for every tile row of both operands.
+bool tiled_vec_ok(const TileArgs &t, const void *dst, const void *src, int esize) {
+  const int64_t vw = 16 / esize;
+  if (t.s_a != 1 || t.d_b != 1 || t.na % vw != 0 || t.nb % vw != 0) return false;
+  if ((uintptr_t)dst % 16 != 0 || (uintptr_t)src % 16 != 0) return false;
+  auto al = [&](int64_t elems) { return (elems * esize) % 16 == 0; };
+  if (!al(t.s_off) || !al(t.d_off) || !al(t.s_b) || !al(t.d_a)) return false;
+  for (int k = 0; k < t.orank; ++k)
+    if (!al(t.o_src[k]) || !al(t.o_dst[k])) return false;
+  return true;
+}
+
 // The (A, B) axes of a transposing copy over the collapsed layout (operand 0
 // the destination, 1 the source), or false when the innermost axis already
 // serves both operands or either operand has no unit-stride axis.  Both
@@ -244,12 +323,12 @@ bool tiled_axes(const ph::Layout &L, int *A, int *B) {
   return *A >= 0 && *B >= 0 && *A != *B && L.shape[*A] >= 16 && L.shape[*B] >= 16;
 }
 
-void make_tile_args(const ph::Layout &L, int A, int B, TileArgs *t) {
+void make_tile_args(const ph::Layout &L, int A, int B, int tile, TileArgs *t) {
   std::memset(t, 0, sizeof(*t));
   t->na = L.shape[A];
   t->nb = L.shape[B];
-  t->ta = (t->na + kTileT - 1) / kTileT;
-  t->tb = (t->nb + kTileT - 1) / kTileT;
+  t->ta = (t->na + tile - 1) / tile;
+  t->tb = (t->nb + tile - 1) / tile;
   t->s_a = L.stride[1][A];
   t->s_b = L.stride[1][B];
   t->d_a = L.stride[0][A];
@@ -352,7 +431,14 @@ int launch_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_vie
   int ax_a, ax_b;
   if (src && tiled_axes(L, &ax_a, &ax_b)) {
     TileArgs t;
-    make_tile_args(L, ax_a, ax_b, &t);
+    constexpr int TV = sizeof(E) >= 8 ? 64 : 128;
+    make_tile_args(L, ax_a, ax_b, TV, &t);
+    if (tiled_vec_ok(t, dst, src, (int)sizeof(E))) {
+      hipLaunchKernelGGL((k_copy_tiled_v<E, TV>), dim3((unsigned)grid_for(t.nblocks)), dim3(kBlock), 0, st,
+                         (E *)dst, (const E *)src, t);
+      return ph::check_launch("k_copy_tiled_v");
+    }
+    make_tile_args(L, ax_a, ax_b, kTileT, &t);
     hipLaunchKernelGGL(k_copy_tiled<E>, dim3((unsigned)grid_for(t.nblocks)), dim3(kBlock), 0, st, (E *)dst,
                        (const E *)src, t);
     return ph::check_launch("k_copy_tiled");
