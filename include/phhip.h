@@ -242,6 +242,16 @@ int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src,
 int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd,
                                const void *src, const ph_view_desc *sd);
 
+/* Masked copy strictly in lexicographic order (one device thread): for every
+ * coordinate k in that order, if mask[md(k)] (a Bool view of the same
+ * shape) then dst[dd(k)] = src[sd(k)], each source element read after the
+ * writes that precede it.  Replaces NArray#[]=(mask, value : MultiIndexable)
+ * (src/n_array.cr:523-536) when `value` is a view of the array itself, so
+ * that the loop's reads see its own earlier writes. */
+int ph_masked_copy_sequential(void *dst, const ph_view_desc *dd,
+                              const void *src, const ph_view_desc *sd,
+                              const void *mask, const ph_view_desc *md);
+
 /* ---------------------------------------------------------------------------
  * Fused fill + scatter over a whole row-major destination buffer of shape
  * dst_shape[rank]: element b gets src[sd(c)] if, on every axis a,

@@ -374,6 +374,35 @@ __global__ void k_copy_sequential(E *dst, ph_view_desc dd, const E *src, ph_view
   }
 }
 
+// One thread, lexicographic order, element k copied only where the Bool
+// mask holds: the aliasing-safe masked copy.
+template <typename E>
+__global__ void k_mask_sequential(E *dst, ph_view_desc dd, const E *src, ph_view_desc sd,
+                                  const unsigned char *mask, ph_view_desc md, int64_t total) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t coord[PH_MAX_RANK];
+  for (int i = 0; i < dd.view.rank; ++i) coord[i] = 0;
+  for (int64_t g = 0; g < total; ++g) {
+    if (mask[map_index(md, coord)]) dst[map_index(dd, coord)] = src[map_index(sd, coord)];
+    for (int i = dd.view.rank - 1; i >= 0; --i) {
+      if (++coord[i] < dd.view.shape[i]) break;
+      coord[i] = 0;
+    }
+  }
+}
+
+template <typename E>
+int launch_mask_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
+                           const void *mask, const ph_view_desc *md) {
+  const int64_t total = ph::desc_count(dd);
+  if (total == 0) return PH_OK;
+  int rc = ph::launch_gate("sequential masked copy");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mask_sequential<E>, dim3(1), dim3(64), 0, ph::stream(), (E *)dst, *dd, (const E *)src, *sd,
+                     (const unsigned char *)mask, *md, total);
+  return ph::check_launch("k_mask_sequential");
+}
+
 template <typename E>
 int launch_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd) {
   const int64_t total = ph::desc_count(dd);
@@ -854,6 +883,35 @@ int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd, const void *sr
     case 2: return launch_sequential<uint16_t>(dst, dd, src, sd);
     case 4: return launch_sequential<uint32_t>(dst, dd, src, sd);
     case 8: return launch_sequential<uint64_t>(dst, dd, src, sd);
+  }
+  return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
+}
+
+int ph_masked_copy_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
+                              const void *mask, const ph_view_desc *md) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  int rc = ph::validate_desc(dd, "copy destination");
+  if (rc) return rc;
+  rc = ph::validate_desc(sd, "copy source");
+  if (rc) return rc;
+  rc = ph::validate_desc(md, "mask");
+  if (rc) return rc;
+  if (!dst || !src || !mask) return ph::set_error(PH_E_ARGUMENT, "masked copy: null pointer");
+  if (md->dtype != PH_BOOL) return ph::set_error(PH_E_ARGUMENT, "mask dtype %d, not Bool", md->dtype);
+  if (ph::dtype_size(dd->dtype) != ph::dtype_size(sd->dtype))
+    return ph::set_error(PH_E_ARGUMENT, "copy between dtypes %d and %d", sd->dtype, dd->dtype);
+  if (dd->view.rank != sd->view.rank || dd->view.rank != md->view.rank)
+    return ph::set_error(PH_E_SHAPE, "masked copy between ranks %d, %d and %d", sd->view.rank, dd->view.rank,
+                         md->view.rank);
+  for (int i = 0; i < dd->view.rank; ++i)
+    if (dd->view.shape[i] != sd->view.shape[i] || dd->view.shape[i] != md->view.shape[i])
+      return ph::set_error(PH_E_SHAPE, "masked copy shapes differ on axis %d", i);
+  switch (ph::dtype_size(dd->dtype)) {
+    case 1: return launch_mask_sequential<uint8_t>(dst, dd, src, sd, mask, md);
+    case 2: return launch_mask_sequential<uint16_t>(dst, dd, src, sd, mask, md);
+    case 4: return launch_mask_sequential<uint32_t>(dst, dd, src, sd, mask, md);
+    case 8: return launch_mask_sequential<uint64_t>(dst, dd, src, sd, mask, md);
   }
   return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
 }

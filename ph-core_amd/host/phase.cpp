@@ -1789,6 +1789,21 @@ void NArray::set_mask(const Operand &mask, const Operand &src) {
                                             " does not match array shape " + shape_to_s(shape_) + ".");
   if (mask.value_dtype() != DType::Bool) throw ArgumentError("mask must be Bool");
   Operand self = operand();
+  const ph_view_desc own = desc();
+  if (src.storage == storage && !src.body && std::memcmp(&src.desc, &own, sizeof(own)) != 0 && size() > 0) {
+    // the source is another view of this array: the reference's loop
+    // (n_array.cr:523-536) reads each source element after the writes that
+    // precede it, which a parallel map does not; run it in order
+    NArray m;
+    Operand mo = mask;
+    if (mask.body || (DType)mask.desc.dtype != DType::Bool) {
+      m = materialize_operand(mask);
+      mo = m.operand();
+    }
+    void *p = storage->dev_write();
+    check(ph_masked_copy_sequential(p, &own, p, &src.desc, mo.storage->dev_read(), &mo.desc), "mask (aliased)");
+    return;
+  }
   map_core({mask, src, self}, [&](const std::vector<Expr> &x) { return Expr::select(x[0], x[1], x[2]); }, this);
 }
 

@@ -62,3 +62,35 @@ def test_mask_errors(ph):
         d[ph.NArray.fill([5, 4], True)] = 3
     with pytest.raises(ph.ShapeError):
         d[ph.NArray.fill([4, 5], True)] = ph.NArray.fill([5, 4], 2)
+
+
+def test_mask_source_aliasing_the_destination(ph):
+    """z[mask] = (a view of z): the reference's loop (n_array.cr:523-536)
+    reads each source element after the writes that precede it in
+    lexicographic order, so the result differs from reading a snapshot.
+    The expected values replay that loop element by element on small
+    arrays."""
+    rng = np.random.default_rng(71)
+    differs = 0
+    for shape, kind in [((6, 6), "T"), ((5, 7), "rev"), ((40, 40), "T"), ((3, 4, 5), "rev"), ((9, 9), "T")]:
+        h = rng.integers(-1000, 1000, shape).astype(np.int32)
+        mh = rng.random(shape) < 0.5
+        z = ph.NArray.from_numpy(h)
+        if kind == "T":
+            src = z.view().permute(1, 0)
+            idx = lambda c: (c[1], c[0])  # noqa: E731
+        else:
+            src = z.view().reverse()
+            idx = lambda c, s=shape: tuple(n - 1 - ci for ci, n in zip(c, s))  # noqa: E731
+        z[ph.NArray.from_numpy(mh)] = src
+        exp = h.copy()
+        for c in np.ndindex(*shape):
+            if mh[c]:
+                exp[c] = exp[idx(c)]
+        assert np.array_equal(z.to_numpy(), exp), (shape, kind)
+        snap = h.copy()
+        for c in np.ndindex(*shape):
+            if mh[c]:
+                snap[c] = h[idx(c)]
+        differs += int(not np.array_equal(snap, exp))
+    assert differs >= 3  # the cases do tell the two readings apart
