@@ -490,6 +490,12 @@ def other_workloads(phase, ev, args):
     out["concatenate_f32_4x4096x4096_axis1"] = {"ms": round(ms, 4), "bytes": by,
                                                 "achieved_gbs": round(by / ms / 1e6, 1),
                                                 "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
+    if not args.no_cpu:
+        O = _oracle()
+        hp = [p.to_numpy() for p in parts]
+        out["concatenate_f32_4x4096x4096_axis1"]["cpu"] = _cpu_leg(
+            lambda: O.concatenate(hp, axis=1), by, "NArray.concatenate restated by the oracle", "full size")
+        del hp
     del parts
     # JSON / YAML text of a 1024 x 1024 Float64 array (device -> host -> text
     # and back; host-bound)
