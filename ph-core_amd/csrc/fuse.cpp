@@ -23,6 +23,7 @@
 // lexicographic order, so Int32 running-sum overflow is detected exactly as a
 // sequential checked sum would; every result is deterministic.
 #include <algorithm>
+#include <utility>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
@@ -604,6 +605,32 @@ bool inner_contiguous(const ph_view_desc &d) {
   for (int i = d.view.rank - 1; i >= 0; --i)
     if (d.view.shape[i] != 1) return d.view.stride[i] == 1 || d.view.stride[i] == -1;
   return true;
+}
+
+// A 2-D reduction whose every input is a transposed view (a unit stride
+// along axis 0, a wider one along axis 1, both extents of 16 or more) and
+// whose values are floating point can run in the storage's orientation:
+// the view's row results are the storage's column results and the other
+// way round.  Only the order of the additions changes, which the float
+// sums' stated accuracy does not depend on.  Int32 sums keep the view's
+// lexicographic order, which their running-total overflow check follows.
+bool swap_transposed_2d(const ph_prog *prog, int n_in, const ph_view_desc *descs) {
+  if (prog->out_dtype != PH_F32 && prog->out_dtype != PH_F64) return false;
+  for (int j = 0; j < n_in; ++j) {
+    const ph_view_desc &d = descs[j];
+    if (d.nstage != 0 || d.view.rank != 2 || d.view.shape[0] < 16 || d.view.shape[1] < 16) return false;
+    const int64_t r = d.view.stride[0], c = d.view.stride[1];
+    if (!(r == 1 || r == -1) || c == 0 || c == 1 || c == -1) return false;
+  }
+  return true;
+}
+
+// The same view with its two axes exchanged.
+ph_view_desc swapped_2d(const ph_view_desc &d) {
+  ph_view_desc t = d;
+  std::swap(t.view.shape[0], t.view.shape[1]);
+  std::swap(t.view.stride[0], t.view.stride[1]);
+  return t;
 }
 
 int materialize(const ph_view_desc *d, const void *src, int slot, ph_view_desc *nd, const void **np) {
@@ -1468,6 +1495,12 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
   if (in_descs[0].view.rank != 2) return ph::set_error(PH_E_DIMENSION, "ph_reduce2d_all needs a 2-D view");
   if (ph::desc_count(&in_descs[0]) == 0)
     return ph::set_error(PH_E_ARGUMENT, "reduction over an empty view");
+  if (swap_transposed_2d(prog, n_in, in_descs)) {
+    ph_view_desc sw[PH_MAX_INPUTS];
+    for (int j = 0; j < n_in; ++j) sw[j] = swapped_2d(in_descs[j]);
+    return ph_reduce2d_all(prog, n_in, ins, sw, acc, full_sum, full_max, col_sum, col_max, row_sum, row_max,
+                           flags_out);
+  }
   View2D V;
   const void *ptrs[PH_MAX_INPUTS];
   ph_view_desc local[PH_MAX_INPUTS];
@@ -1507,6 +1540,12 @@ int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const p
   if (ph::desc_count(&in_descs[0]) == 0) {
     if (keep_axis >= 0 && in_descs[0].view.shape[keep_axis] == 0) return PH_OK;  // no results
     return ph::set_error(PH_E_ARGUMENT, "reduction over an empty view");
+  }
+  if (rank == 2 && swap_transposed_2d(prog, n_in, in_descs)) {
+    ph_view_desc sw[PH_MAX_INPUTS];
+    for (int j = 0; j < n_in; ++j) sw[j] = swapped_2d(in_descs[j]);
+    return ph_reduce(prog, n_in, ins, sw, keep_axis < 0 ? keep_axis : 1 - keep_axis, ops, acc, out_sum, out_max,
+                     flags_out);
   }
   // Arrange as 2-D: rows x cols in lexicographic order of the reduced data.
   //   keep -1         : [prod(shape[:-1]), shape[-1]]      -> full

@@ -118,27 +118,55 @@ def test_maps_over_transposed_operands(ph):
     assert np.array_equal(pv.to_narr().to_numpy(), exp)
 
 
-@pytest.mark.parametrize("kind", ["i32", "f32"])
-def test_reductions_over_transposed_views(ph, kind):
-    """sum / max in full and along each axis, and the six-result pass, over
-    a transposed view equal the same reductions of its materialised copy
-    (bit for bit: the view is copied to the same layout first), and Int32
-    sums equal numpy's exact sums."""
+def test_reductions_over_transposed_views_int32(ph):
+    """Int32 sum / max in full and along each axis, and the six-result
+    pass, over a transposed view: the view is copied through the tiled
+    transpose first (the overflow check follows its lexicographic order),
+    so the results equal those of its materialised copy bit for bit, and
+    numpy's exact sums."""
     rng = np.random.default_rng(61)
-    shape = (300, 1000)
-    h = (rng.integers(-1000, 1000, shape).astype(np.int32) if kind == "i32"
-         else rng.standard_normal(shape).astype(np.float32))
-    x = ph.NArray.from_numpy(h)
-    v = x.view().permute(1, 0)
+    h = rng.integers(-1000, 1000, (300, 1000)).astype(np.int32)
+    v = ph.NArray.from_numpy(h).view().permute(1, 0)
     m = v.to_narr()
     assert v.sum() == m.sum() and v.max() == m.max()
     for ax in (0, 1):
         assert list(v.slices_sum(ax)) == list(m.slices_sum(ax))
         assert list(v.slices_max(ax)) == list(m.slices_max(ax))
     assert v.reduce2d_all() == m.reduce2d_all()
-    if kind == "i32":
-        assert v.sum() == int(h.astype(np.int64).sum())
-        assert np.array_equal(np.array(v.slices_sum(0)), h.T.astype(np.int64).sum(axis=1))
+    assert v.sum() == int(h.astype(np.int64).sum())
+    assert np.array_equal(np.array(v.slices_sum(0)), h.T.astype(np.int64).sum(axis=1))
+    # the view's lexicographic running total leaves Int32 (MAX, then +5),
+    # the storage's order would not (MAX, -10, ..., +5)
+    big = np.zeros((40, 30), np.int32)
+    big[0, 0] = 2 ** 31 - 1
+    big[0, 1] = 5
+    big[1, 0] = -10
+    with pytest.raises(ph.OverflowError):
+        ph.NArray.from_numpy(np.ascontiguousarray(big.T)).view().permute(1, 0).sum()
+
+
+def test_reductions_over_transposed_views_float(ph):
+    """Float sums over a transposed view run in the storage's orientation
+    (rows and columns trade places): maxima equal the materialised copy's
+    bit for bit, sums agree with the oracle's exact sums to 1e-12."""
+    rng = np.random.default_rng(63)
+    h = rng.standard_normal((300, 1000)).astype(np.float32)
+    v = ph.NArray.from_numpy(h).view().permute(1, 0)
+    vt = np.ascontiguousarray(h.T).astype(np.float64)
+    m = v.to_narr()
+    assert v.max() == m.max()
+    assert abs(v.sum() - O.sum_exact(vt)) <= 1e-12 * np.abs(vt).sum()
+    for ax in (0, 1):
+        assert list(v.slices_max(ax)) == list(m.slices_max(ax))
+        want = O.slices_reduce(vt if ax == 0 else np.ascontiguousarray(vt.T), 0, 2)
+        scale = np.abs(vt).sum(axis=1 - ax)
+        assert np.all(np.abs(np.array(v.slices_sum(ax)) - want) <= 1e-12 * scale), ax
+    r, rm = v.reduce2d_all(), m.reduce2d_all()
+    for k in ("full_max", "row_max", "col_max"):
+        assert r[k] == rm[k], k
+    for k in ("row_sum", "col_sum"):
+        assert np.allclose(r[k], rm[k], rtol=1e-12, atol=0), k
+    assert abs(r["full_sum"] - rm["full_sum"]) <= 1e-12 * np.abs(vt).sum()
 
 
 def test_middle_axis_reduction_of_reshaped_view(ph):
