@@ -495,6 +495,101 @@ std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::L
   return s;
 }
 
+// Map whose transposed inputs (bit j of `tmask`) are contiguous along axis
+// `ax` of the collapsed layout while the output is contiguous along the
+// last axis.  A workgroup owns a 64 x 64 tile of (ax, last) for one
+// coordinate of the other axes: it reads the tile of each transposed input
+// with lanes along `ax` into LDS ([last][ax], rows padded to 65 values),
+// then computes with lanes along the last axis, reading the other inputs
+// directly and the transposed ones from LDS, and writes the output.  So
+// every operand moves whole cache lines and the transposed inputs need no
+// pass of their own.
+std::string map_tiled_source(const ph_prog *p, int n_in, const int *in_dt, const ph::Layout &L, int out_dt, int ax,
+                             unsigned tmask, int64_t *nblocks) {
+  const int bx = L.rank - 1;
+  const int64_t na = L.shape[ax], nb = L.shape[bx];
+  const int64_t ta = (na + 63) / 64, tb = (nb + 63) / 64;
+  int64_t others = 1;
+  for (int k = 0; k < L.rank; ++k)
+    if (k != ax && k != bx) others *= L.shape[k];
+  *nblocks = others * ta * tb;
+  std::string s = kPrelude;
+  s += fmt("struct MapArgs { const void *in[%d]; i64 in_off[%d]; void *out; i64 out_off; int *flags; };\n",
+           PH_MAX_INPUTS, PH_MAX_INPUTS);
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_map_tiled(MapArgs A) {\n";
+  s += "  int fl = 0;\n  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
+  for (int j = 0; j < n_in; ++j)
+    if (tmask >> j & 1u) s += fmt("  __shared__ %s lds%d[64][65];\n", ctype(compute_dt(in_dt[j])), j);
+  s += fmt("  for (i64 b = blockIdx.x; b < %s; b += gridDim.x) {\n", lit_i64(*nblocks).c_str());
+  s += fmt("    i64 o = b / %s; const i64 rem = b - o * %s;\n", lit_i64(ta * tb).c_str(), lit_i64(ta * tb).c_str());
+  s += fmt("    const i64 ib = rem / %s, ia = rem - ib * %s;\n", lit_i64(ta).c_str(), lit_i64(ta).c_str());
+  for (int o = 0; o <= n_in; ++o)
+    s += fmt("    i64 base%d = %s;\n", o, o < n_in ? fmt("A.in_off[%d]", o).c_str() : "A.out_off");
+  for (int k = L.rank - 1; k >= 0; --k) {
+    if (k == ax || k == bx) continue;
+    s += fmt("    { const i64 q = o / %s; const i64 c = o - q * %s; o = q;\n", lit_i64(L.shape[k]).c_str(),
+             lit_i64(L.shape[k]).c_str());
+    for (int o = 0; o <= n_in; ++o)
+      if (L.stride[o][k] != 0) s += fmt("      base%d += c * %s;\n", o, lit_i64(L.stride[o][k]).c_str());
+    s += "    }\n";
+  }
+  s += "    const i64 a0 = ia * 64, b0 = ib * 64;\n";
+  // phase 1: the transposed inputs into LDS, lanes along ax
+  s += "    {\n      const i64 ar = a0 + lane;\n#pragma unroll\n      for (int r = 0; r < 16; ++r) {\n";
+  s += fmt("        const i64 bb = b0 + wave + 4 * r;\n        if (ar < %s && bb < %s) {\n", lit_i64(na).c_str(),
+           lit_i64(nb).c_str());
+  for (int j = 0; j < n_in; ++j)
+    if (tmask >> j & 1u)
+      s += fmt("          lds%d[wave + 4 * r][lane] = ph_ld((const %s *)A.in[%d], base%d + ar * %s + bb * %s);\n", j,
+               stype(in_dt[j]), j, j, lit_i64(L.stride[j][ax]).c_str(), lit_i64(L.stride[j][bx]).c_str());
+  s += "        }\n      }\n    }\n    __syncthreads();\n";
+  // phase 2: compute with lanes along the last axis
+  std::vector<std::string> inexpr;
+  for (int j = 0; j < n_in; ++j) inexpr.push_back(fmt("x%d", j));
+  std::string body;
+  const std::string res = emit_prog(p, inexpr, &body, "");
+  s += "    {\n      const i64 bc = b0 + lane;\n#pragma unroll 4\n      for (int r = 0; r < 16; ++r) {\n";
+  s += fmt("        const i64 aa = a0 + wave + 4 * r;\n        if (aa < %s && bc < %s) {\n", lit_i64(na).c_str(),
+           lit_i64(nb).c_str());
+  for (int j = 0; j < n_in; ++j) {
+    if (tmask >> j & 1u)
+      s += fmt("          const %s x%d = lds%d[lane][wave + 4 * r];\n", ctype(compute_dt(in_dt[j])), j, j);
+    else
+      s += fmt("          const %s x%d = ph_ld((const %s *)A.in[%d], base%d + aa * %s + bc * %s);\n",
+               ctype(compute_dt(in_dt[j])), j, stype(in_dt[j]), j, j, lit_i64(L.stride[j][ax]).c_str(),
+               lit_i64(L.stride[j][bx]).c_str());
+  }
+  s += body;
+  s += fmt("          ph_st((%s *)A.out, base%d + aa * %s + bc * %s, %s);\n", stype(out_dt), n_in,
+           lit_i64(L.stride[n_in][ax]).c_str(), lit_i64(L.stride[n_in][bx]).c_str(), res.c_str());
+  s += "        }\n      }\n    }\n    __syncthreads();  // the next tile reuses the LDS\n  }\n";
+  s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  return s;
+}
+
+// The axis along which every input of `tmask` is contiguous in the collapsed
+// layout, if the tiled map applies: it is not the last axis, the output is
+// contiguous along the last axis, and both extents are 16 or more.
+int tiled_map_axis(const ph::Layout &L, int n_in, unsigned tmask) {
+  if (L.rank < 2) return -1;
+  const int bx = L.rank - 1;
+  const int64_t so = L.stride[n_in][bx];
+  if (!(so == 1 || so == -1) || L.shape[bx] < 16) return -1;
+  int ax = -1;
+  for (int j = 0; j < n_in; ++j) {
+    if (!(tmask >> j & 1u)) continue;
+    int aj = -1;
+    for (int k = 0; k < bx; ++k)
+      if (L.stride[j][k] == 1 || L.stride[j][k] == -1) {
+        aj = k;
+        break;
+      }
+    if (aj < 0 || (ax >= 0 && aj != ax)) return -1;
+    ax = aj;
+  }
+  return (ax >= 0 && L.shape[ax] >= 16) ? ax : -1;
+}
+
 int64_t grid_for(int64_t blocks) {
   const int64_t cap = 1 << 20;
   return blocks < 1 ? 1 : (blocks > cap ? cap : blocks);
@@ -1322,6 +1417,49 @@ int check_inputs(const ph_prog *prog, int32_t n_in, const void *const *ins,
   return validate_prog(prog, n_in, in_dt, allow_index);
 }
 
+// Runs map_tiled_source's kernel (compiled once per program and layout).
+int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *in_dt, const ph::Layout &L, int ax,
+              unsigned tmask, void *out, const ph_view_desc *out_desc, int32_t *flags_out) {
+  std::string key = "MT" + prog_key(prog);
+  key_add(&key, in_dt, sizeof(int) * (size_t)n_in);
+  key_add(&key, &L.rank, sizeof(L.rank));
+  key_add(&key, L.shape, sizeof(int64_t) * (size_t)L.rank);
+  for (int o = 0; o <= n_in; ++o) key_add(&key, L.stride[o], sizeof(int64_t) * (size_t)L.rank);
+  const int32_t mk[3] = {out_desc->dtype, ax, (int32_t)tmask};
+  key_add(&key, mk, sizeof(mk));
+  hipFunction_t fn;
+  int64_t nblocks = 0;
+  auto hit = fused_cache().find(key);
+  if (hit != fused_cache().end()) {
+    fn = hit->second.k[0];
+    nblocks = hit->second.nblocks;
+  } else {
+    std::string src = map_tiled_source(prog, n_in, in_dt, L, out_desc->dtype, ax, tmask, &nblocks);
+    int rc = ph::jit_kernel(src, "ph_map_tiled", &fn);
+    if (rc) return rc;
+    fused_cache()[key] = FusedKernels{{fn, nullptr, nullptr}, nblocks};
+  }
+  const bool may_flag = prog_may_flag(prog);
+  int *dflags;
+  int rc = get_flags_buffer(&dflags, may_flag);
+  if (rc) return rc;
+  MapArgs A;
+  std::memset(&A, 0, sizeof(A));
+  for (int j = 0; j < n_in; ++j) {
+    A.in[j] = ins[j];
+    A.in_off[j] = L.offset[j];
+  }
+  A.out = out;
+  A.out_off = L.offset[n_in];
+  A.flags = dflags;
+  size_t asz = sizeof(A);
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &asz, HIP_LAUNCH_PARAM_END};
+  rc = ph::launch_gate("map (tiled)");
+  if (rc) return rc;
+  PH_HIP(hipModuleLaunchKernel(fn, (unsigned)grid_for(nblocks), 1, 1, kBlock, 1, 1, 0, ph::stream(), nullptr, cfg));
+  return read_flags(dflags, may_flag, flags_out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1417,6 +1555,25 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   for (int j = 0; j < n_in; ++j) nstaged += in_descs[j].nstage > 0 ? 1 : 0;
   if (nstaged > 2) return ph::set_error(PH_E_UNSUPPORTED, "at most two reshaped/tiled inputs per map");
   const bool out_inner = inner_contiguous(*out_desc);
+  if (out_inner && nstaged == 0 && !prog_uses_index(prog)) {
+    // transposed inputs staged through LDS inside the map (at most two)
+    unsigned tmask = 0;
+    int nt = 0;
+    for (int j = 0; j < n_in; ++j)
+      if (transposed_layout(in_descs[j])) {
+        tmask |= 1u << j;
+        ++nt;
+      }
+    if (nt >= 1 && nt <= 2) {
+      const ph_view_desc *td[PH_MAX_INPUTS + 1];
+      for (int j = 0; j < n_in; ++j) td[j] = &in_descs[j];
+      td[n_in] = out_desc;
+      ph::Layout TL;
+      ph::collapse(td, n_in + 1, &TL);
+      const int ax = tiled_map_axis(TL, n_in, tmask);
+      if (ax >= 0) return map_tiled(prog, n_in, ins, in_dt, TL, ax, tmask, out, out_desc, flags_out);
+    }
+  }
   int slot = 0, spare = 2 - nstaged;
   for (int j = 0; j < n_in; ++j) {
     local[j] = in_descs[j];
