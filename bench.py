@@ -590,6 +590,18 @@ def path_ops(phase, ev, args):
                 "sample": f"map_with_coord (element and coordinate zipped), full 8192^2, {dt * 1e3:.0f} ms"}
     entry("map_with_coord", "ph_map_kernel (index instruction)",
           lambda: x.map_with_coord(lambda el, c: el.wadd(c[0]).wsub(c[1])), 8 * nn, cpu_coord)
+    xt = x.view().permute(1, 0)
+
+    def cpu_tmap():
+        from crystal import trace
+        t0 = time.perf_counter()
+        g = O.view_gather(xh, [("region", r), ("permute", [1, 0])], [rows, n])
+        O.eval_expr(trace(lambda a, b: a.wadd(b), 2), [g, yh[:rows]])
+        dt = time.perf_counter() - t0
+        return {"value": round(12 * rows * n / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                "sample": f"map_with over x.permute(1, 0) and y, its first {rows} rows, {dt * 1e3:.0f} ms"}
+    entry("map_transposed", "ph_map_tiled (transposed operand staged through LDS)",
+          lambda: xt.map_with(y, block=lambda a, b: a.wadd(b)), 12 * nn, cpu_tmap)
     return res
 
 
