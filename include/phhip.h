@@ -402,8 +402,10 @@ int ph_batch_destroy(void *handle);
  * Diagnostics (no reference counterpart).
  * ph_compile_check generates and compiles, without launching, the fused
  * kernels ph_map (kind 0) and ph_reduce2d_all (kind 1) would use for `prog`
- * over contiguous inputs of the given dtypes and a rows x cols shape.  It
- * needs no GPU, so program lowering can be checked on a build host.
+ * over contiguous inputs of the given dtypes and a rows x cols shape, and
+ * (kind 2) the tiled map ph_map uses when input 0 is the transposed view of
+ * a cols x rows array (rows and cols of 16 or more).  It needs no GPU, so
+ * program lowering can be checked on a build host.
  * ------------------------------------------------------------------------- */
 int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes,
                      int64_t rows, int64_t cols, int32_t kind);

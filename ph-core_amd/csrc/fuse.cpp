@@ -1475,6 +1475,29 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   int rc = validate_prog(prog, n_in, in_dt, kind == 0);
   if (rc) return rc;
   if (rows < 1 || cols < 1) return ph::set_error(PH_E_ARGUMENT, "empty shape");
+  if (kind == 2) {
+    if (prog_uses_index(prog)) return ph::set_error(PH_E_ARGUMENT, "the tiled map has no element index");
+    ph_view_desc d[PH_MAX_INPUTS + 1];
+    const ph_view_desc *dp[PH_MAX_INPUTS + 1];
+    for (int j = 0; j <= n_in; ++j) {
+      std::memset(&d[j], 0, sizeof(d[j]));
+      d[j].dtype = j < n_in ? in_dt[j] : prog->out_dtype;
+      d[j].extent = rows * cols;
+      d[j].view.rank = 2;
+      d[j].view.shape[0] = rows;
+      d[j].view.shape[1] = cols;
+      d[j].view.stride[0] = j == 0 ? 1 : cols;    // input 0: the transposed view
+      d[j].view.stride[1] = j == 0 ? rows : 1;
+      dp[j] = &d[j];
+    }
+    ph::Layout L;
+    ph::collapse(dp, n_in + 1, &L);
+    const int ax = tiled_map_axis(L, n_in, 1u);
+    if (ax < 0) return ph::set_error(PH_E_ARGUMENT, "the tiled map does not apply to %lld x %lld", (long long)rows,
+                                     (long long)cols);
+    int64_t nb;
+    return ph::jit_compile_only(map_tiled_source(prog, n_in, in_dt, L, prog->out_dtype, ax, 1u, &nb));
+  }
   if (kind == 0) {
     ph_view_desc d[PH_MAX_INPUTS + 1];
     const ph_view_desc *dp[PH_MAX_INPUTS + 1];

@@ -235,6 +235,19 @@ def test_element_programs_compile_for_gfx950(host):
     host.compile_check(two * host.input(2, host.DType.F64), [host.DType.F64] * 3, 16, 16, 0)
 
 
+def test_tiled_map_programs_compile_for_gfx950(host):
+    """The map that stages a transposed operand through LDS is generated
+    and compiled for a few blocks and element types (no launch, no device);
+    the element index is refused there."""
+    for dt, fn in BLOCKS[:3] + BLOCKS[8:10]:
+        d = getattr(host.DType, DT[dt][0])
+        host.compile_check(fn(host.placeholder(d)), [d], 64, 96, 2)
+    two = host.input(0, host.DType.BF16) * host.input(1, host.DType.F32)
+    host.compile_check(two, [host.DType.BF16, host.DType.F32], 300, 257, 2)
+    with pytest.raises(host.ArgumentError):
+        host.compile_check(host.input(0, host.DType.I32) + host.element_index(), [host.DType.I32], 64, 64, 2)
+
+
 def test_element_index_programs(host):
     """A block that reads the element's lexicographic index (map_with_index,
     n_array.cr:605-611) compiles into the map kernel in its row-tiled and its
