@@ -429,8 +429,9 @@ def other_workloads(phase, ev, args):
     # the same run with 1, 2 and 4 steps per pass for both precisions (one
     # step per pass is the reference's loop structure; 1 and 2 are
     # HBM-bound in both precisions), next to
-    # the default schedule (8 steps per pass, where FP32 and FP64 are both
-    # limited by VALU: FP32 runs at FP64's rate without packed FP32 math)
+    # the default schedule (12 steps per pass for FP32, 8 for FP64, where
+    # both are limited by VALU: FP32 runs at FP64's rate without packed FP32
+    # math)
     per_pass = {}
     for kk in ("1", "2", "4"):
         os.environ["PH_HEAT_K"] = kk

@@ -4,8 +4,8 @@
 // Step kernel (k_heat_multi): K time steps per pass over the grid.  A
 // wavefront owns a strip of columns and sweeps a band of rows top to bottom,
 // keeping two rows per time level in registers, so each pass reads the grid
-// once (plus 2K halo rows per band and two overlapping lanes per strip edge)
-// and writes it once.  Loads and stores are VEC-wide (16 or 32 bytes per
+// once (plus 2K halo rows per band and two or three overlapping lanes per
+// strip edge) and writes it once.  Loads and stores are VEC-wide (16 or 32 bytes per
 // lane) and coalesced.  Workgroups are numbered so that strips adjacent in
 // the grid run on the same XCD (workgroups are dealt to the 8 XCDs
 // round-robin), which keeps the overlapping columns in that XCD's L2.
@@ -469,8 +469,10 @@ int64_t band_rows(int64_t ny, int64_t groups_per_band, int64_t resident, int k) 
 // positive, else chosen by band_rows for this kernel's residency.
 template <typename T, int K, int A, bool F>
 int launch_ahead(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok) {
-  constexpr int H = 2;  // halo lanes on each side of a wavefront's strip
+  constexpr int H = K > 8 ? 3 : 2;  // halo lanes on each side of a wavefront's strip
   const void *f = (const void *)k_heat_multi<T, K, A, H, F>;
+  constexpr int64_t wcols = (64 - 2 * H) * Vec<T>::n;  // columns a wavefront writes
+  a.nstrips = (a.nx + wcols - 1) / wcols;
   const int64_t groups_per_band = (a.nstrips + kWaves - 1) / kWaves;
   a.ty = ty > 0 ? ty : band_rows(a.ny, groups_per_band, resident_groups(f), K);
   a.nbands = (a.ny + a.ty - 1) / a.ty;
@@ -497,6 +499,9 @@ int launch_multi(const T *cur, T *nxt, HeatArgs a, int64_t ty, T r, int vec_ok, 
 
 template <typename T, bool F>
 int launch_kf(int k, const T *cur, T *nxt, const HeatArgs &a, int64_t ty, T r, int vec_ok, int ahead) {
+  if constexpr (sizeof(T) == 4) {  // Float32 also runs passes of twelve steps (three halo lanes)
+    if (k == 12) return launch_multi<T, 12, F>(cur, nxt, a, ty, r, vec_ok, ahead);
+  }
   switch (k) {
     case 8: return launch_multi<T, 8, F>(cur, nxt, a, ty, r, vec_ok, ahead);
     case 7: return launch_multi<T, 7, F>(cur, nxt, a, ty, r, vec_ok, ahead);
@@ -589,22 +594,23 @@ template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
              double *energy) {
   constexpr int V = Vec<T>::n;
-  // Steps per pass (measured on MI355X at 8192^2, both precisions: 8); the
-  // rows loaded ahead follow from the steps per pass (launch_multi) and the
-  // band height from the grid and the kernel's residency (band_rows).
-  // PH_HEAT_K, PH_HEAT_TY and PH_HEAT_AHEAD override them.
-  int64_t kmax = 2 * V;
+  // Steps per pass, measured on MI355X at 8192^2: 12 for Float32 (three
+  // halo lanes; 8, 10, 14 and 16 are slower) and 8 for Float64 (its 252
+  // registers leave no room for more).  The rows loaded ahead follow from
+  // the steps per pass (launch_multi) and the band height from the grid
+  // and the kernel's residency (band_rows).  PH_HEAT_K, PH_HEAT_TY and
+  // PH_HEAT_AHEAD override them.
+  constexpr int64_t kcap = sizeof(T) == 4 ? 3 * V : 2 * V;
+  int64_t kmax = kcap;
   int64_t ty = 0;
   int64_t ahead = 0;  // by steps per pass (launch_multi)
-  if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(2 * V, std::max<int64_t>(1, std::atol(e)));
+  if (const char *e = std::getenv("PH_HEAT_K")) kmax = std::min<int64_t>(kcap, std::max<int64_t>(1, std::atol(e)));
   if (const char *e = std::getenv("PH_HEAT_TY")) ty = std::max<int64_t>(8, std::atol(e));
   if (const char *f = std::getenv("PH_HEAT_AHEAD")) ahead = std::atol(f);
   HeatArgs a;
   a.ny = ny;
   a.nx = nx;
-  const int64_t wcols = (64 - 2 * 2) * V;  // columns a wavefront writes (2 halo lanes per side)
-  a.nstrips = (nx + wcols - 1) / wcols;
-  a.ty = a.nbands = a.nblocks = a.per_xcd = 0;  // set per pass by launch_multi
+  a.nstrips = a.ty = a.nbands = a.nblocks = a.per_xcd = 0;  // set per pass by launch_ahead
   constexpr uintptr_t kAlign = sizeof(typename Vec<T>::type);
   const int vec_ok = (nx % V == 0) && ((uintptr_t)u % kAlign == 0) && ((uintptr_t)tmp % kAlign == 0);
   const T rr = (T)r;
@@ -626,7 +632,8 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   while (t < steps) {
     int64_t until = steps;
     if (energy && every > 0) until = std::min<int64_t>(steps, (t / every + 1) * every);
-    const int k = (int)std::min<int64_t>(kmax, until - t);
+    int k = (int)std::min<int64_t>(kmax, until - t);
+    if (k > 8 && k < 12) k = 8;  // passes of more than 8 steps come only in twelves
     if (try_fused && t >= fused_until) {
       int64_t span = 0;
       rc = fused_safe<T>(cur, ny * nx, rr, &fused, &span);
