@@ -76,9 +76,10 @@ enum {
  *         z   = delinearize(idx, s.shape[0..s.rank-1])   (row-major)
  *         idx = s.offset + sum_j z_j * s.stride[j]
  * Strides are signed element strides; 0 repeats an element (broadcast).
- * Region, Permute and Reverse transforms (src/view_util/transforms.cr:
- * 392-505) fold into the affine map; Reshape (:319-390) and tile
- * (src/multi_indexable.cr:818-827) need a stage.  `extent` is the element
+ * Region, Permute and Reverse transforms (their apply methods at
+ * src/view_util/transforms.cr:215-221, :267-270 and :298-304) fold into the
+ * affine map; Reshape (:184-188) and tile (src/multi_indexable.cr:818-827)
+ * need a stage.  `extent` is the element
  * count of the underlying buffer; the library rejects a descriptor that can
  * address outside [0, extent).
  * ------------------------------------------------------------------------- */
@@ -102,7 +103,7 @@ typedef struct {
  * Element program: the compiled form of an elementwise expression (the
  * operator macros of src/multi_indexable.cr:931-985, scalar operands
  * src/multi_indexable.cr:947-951 and src/patches/number.cr:5-15, and a
- * ProcView block src/proc_view.cr:156-158).  Registers hold one value each
+ * ProcView block src/proc_view.cr:28-30).  Registers hold one value each
  * (Int32, Float32, Float64 or Bool, tagged by the instruction's dtype);
  * results round to the instruction's dtype exactly as Crystal does, with no
  * contraction into fused multiply-adds.
@@ -277,7 +278,7 @@ int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
  * Replaces MultiIndexable#map_with / each_with (src/multi_indexable.cr:
  * 1026-1102), the operator macros (:931-985), NArray#map / map! (src/
  * n_array.cr:589-628), scalar-first patches (src/patches/number.cr:5-21),
- * and ProcView#to_narr (src/proc_view.cr:156-158 + src/view.cr:123-126).
+ * and ProcView#to_narr (src/proc_view.cr:28-30 + src/view.cr:123-126).
  * In-place map (map!) passes out == in_0 with equal contiguous descriptors.
  * A program with PH_OP_INDEX (NArray#map_with_index / map_with_index! /
  * map_with_coord, src/n_array.cr:605-650) sees each element's lexicographic

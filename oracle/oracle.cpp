@@ -489,9 +489,10 @@ static int64_t region_size(const or_region *r) {
 }
 
 // NArray#unsafe_fetch_chunk (src/n_array.cr:450-453): the buffered lex
-// odometer (src/buffered/indexed/stride_iterator.cr:89-92, lex_iterator.cr:
-// 7-21) walks the region's proper coordinates and keeps the flat buffer
-// index up to date incrementally.
+// odometer walks the region's proper coordinates and keeps the flat buffer
+// index up to date incrementally: its start index is
+// src/buffered/indexed/stride_iterator.cr:10-13, its step
+// src/buffered/indexed/lex_iterator.cr:7-21.
 int64_t or_gather(const void *src, const int64_t *shape, int rank, int esize,
                   const or_region *r, void *out) {
   int64_t n = region_size(r);
@@ -589,7 +590,8 @@ typedef struct {
   or_region region;    // kind 0
 } or_transform;
 
-// One transform's apply (transforms.cr:385-389, 416-422, 468-471, 499-505).
+// One transform's apply (src/view_util/transforms.cr:215-221 region,
+// :184-188 reshape, :267-270 permute, :298-304 reverse).
 static int apply_transform(const or_transform *t, const int64_t *in, int nin,
                            int64_t *out, int *nout) {
   switch (t->kind) {
@@ -1162,7 +1164,7 @@ static Val eval_node(const std::vector<Node> &nodes, int id, const Val *in, int 
 
 // Evaluates `expr` elementwise over n_in inputs of n elements each (the
 // map_with zip of src/multi_indexable.cr:1080-1102, or a ProcView's proc over
-// its source, src/proc_view.cr:156-158).  Returns the result dtype, or -1 on
+// its source, src/proc_view.cr:28-30).  Returns the result dtype, or -1 on
 // a parse error.
 int or_eval(const char *expr, int n_in, const int *in_dtypes,
             const void *const *ins, int64_t n, void *out, int *flags) {
@@ -1184,6 +1186,132 @@ int or_eval(const char *expr, int n_in, const int *in_dtypes,
     rdt = eval_node(p.nodes, root, vals, &f).dt;
   }
   return rdt;
+}
+
+// ===========================================================================
+// Lazy sources: the reference reads a source or mask element only when its
+// loop reaches it, so a source that is a view (or a ProcView) of the array
+// being written sees the writes of the elements before it.
+// ===========================================================================
+
+// One operand of a lazily evaluated expression: a buffer (read at every use,
+// so writes made since are seen), its row-major shape, and the transform
+// chain (newest first) taking an expression coordinate to a buffer
+// coordinate.  An empty chain means the coordinate is the buffer's own.
+typedef struct {
+  const void *buf;
+  int32_t dtype;
+  int32_t rank;
+  int64_t shape[OR_MAXR];
+  const or_transform *chain;
+  int32_t nchain;
+  int32_t pad_;
+} or_input;
+
+static Val read_input(const or_input &in, const int64_t *coord, int rank) {
+  int64_t sc[OR_MAXR];
+  int ns = 0;
+  chain_apply(in.chain, in.nchain, coord, rank, sc, &ns);
+  int64_t bs[OR_MAXR];
+  axis_strides(in.shape, in.rank, bs);
+  int64_t idx = 0;
+  for (int i = 0; i < ns; ++i) idx += sc[i] * bs[i];
+  return load_val(in.dtype, in.buf, idx);
+}
+
+static Val eval_lazy(const Parser &p, int root, int n_in, const or_input *ins, const int64_t *coord,
+                     int rank, int *flags) {
+  Val vals[8];
+  for (int j = 0; j < n_in && j < 8; ++j) vals[j] = read_input(ins[j], coord, rank);
+  return eval_node(p.nodes, root, vals, flags);
+}
+
+// NArray#unsafe_set_chunk(region, src) (src/n_array.cr:484-492) and, with a
+// non-empty `dchain`, MultiWritable#unsafe_set_chunk through a MutableView
+// (src/multi_writable.cr:29-36, src/mutable_view.cr:16-18), where `src` is
+// the lazily evaluated `expr` over `ins` with shape src_shape (a View is the
+// expression "x0", a ProcView its block): element k of src (lexicographic
+// over src_shape, `src.each`) is evaluated when the loop reaches it and
+// written to the k-th coordinate of the region.  Returns the count written,
+// or -1 if `expr` does not parse.
+int64_t or_set_chunk_lazy(void *dst, const int64_t *dst_shape, int dst_rank, const or_transform *dchain,
+                          int ndchain, const or_region *r, const char *expr, int n_in, const or_input *ins,
+                          const int64_t *src_shape, int src_rank, int *flags) {
+  Parser p;
+  p.s = expr;
+  const int root = p.parse();
+  if (!p.ok) return -1;
+  int64_t n = region_size(r);
+  if (n == 0) return 0;
+  for (int i = 0; i < r->rank; ++i)
+    if (r->step[i] == 0) return 0;
+  int64_t bs[OR_MAXR];
+  axis_strides(dst_shape, dst_rank, bs);
+  int64_t coord[OR_MAXR], sc[OR_MAXR] = {0};
+  for (int i = 0; i < r->rank; ++i) coord[i] = r->first[i];
+  int64_t k = 0;
+  for (;;) {
+    const Val v = eval_lazy(p, root, n_in, ins, sc, src_rank, flags);
+    int64_t dc[OR_MAXR];
+    int nd = 0;
+    chain_apply(dchain, ndchain, coord, r->rank, dc, &nd);
+    int64_t idx = 0;
+    for (int i = 0; i < nd; ++i) idx += dc[i] * bs[i];
+    store_val(v, dst, idx);
+    ++k;
+    for (int i = src_rank - 1; i >= 0; --i) {
+      if (++sc[i] < src_shape[i]) break;
+      sc[i] = 0;
+    }
+    int i = r->rank - 1;
+    for (; i >= 0; --i) {
+      if (coord[i] == r->last[i]) {
+        coord[i] = r->first[i];
+        if (i == 0) return k;
+      } else {
+        coord[i] += r->step[i];
+        break;
+      }
+    }
+  }
+}
+
+// NArray#[]=(mask, value) (src/n_array.cr:523-551): for every coordinate of
+// the array in lexicographic order, `mask.get(coord)` is evaluated, and where
+// it holds, `value.get(coord)` (or the scalar) is written at that
+// coordinate.  `mask_expr` and `value_expr` are lazily evaluated expressions
+// over `ins` (value_expr NULL: the scalar `*scalar` of the array's dtype).
+// Returns 0, or -1 if an expression does not parse.
+int or_set_mask_lazy(void *dst, int dtype, const int64_t *shape, int rank, const char *mask_expr,
+                     const char *value_expr, const void *scalar, int n_in, const or_input *ins, int *flags) {
+  Parser pm;
+  pm.s = mask_expr;
+  const int mroot = pm.parse();
+  if (!pm.ok) return -1;
+  Parser pv;
+  int vroot = -1;
+  if (value_expr) {
+    pv.s = value_expr;
+    vroot = pv.parse();
+    if (!pv.ok) return -1;
+  }
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) n *= shape[i];
+  if (rank == 0 || n == 0) return 0;
+  const Val sv = load_val(dtype, scalar ? scalar : dst, 0);
+  int64_t coord[OR_MAXR] = {0};
+  for (int64_t idx = 0; idx < n; ++idx) {
+    const Val m = eval_lazy(pm, mroot, n_in, ins, coord, rank, flags);
+    if (m.i) {
+      const Val v = value_expr ? eval_lazy(pv, vroot, n_in, ins, coord, rank, flags) : sv;
+      store_val(v, dst, idx);
+    }
+    for (int i = rank - 1; i >= 0; --i) {
+      if (++coord[i] < shape[i]) break;
+      coord[i] = 0;
+    }
+  }
+  return 0;
 }
 
 // ===========================================================================

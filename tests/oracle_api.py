@@ -56,6 +56,11 @@ class Transform(C.Structure):
                 ("region", Region)]
 
 
+class Input(C.Structure):
+    _fields_ = [("buf", C.c_void_p), ("dtype", C.c_int32), ("rank", C.c_int32), ("shape", C.c_int64 * MAXR),
+                ("chain", C.POINTER(Transform)), ("nchain", C.c_int32), ("pad_", C.c_int32)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
@@ -100,6 +105,12 @@ def lib():
         L.or_binop.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
         L.or_unop.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
         L.or_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_int64, C.c_void_p, C.POINTER(C.c_int)]
+        L.or_set_chunk_lazy.argtypes = [C.c_void_p, i64p, C.c_int, C.POINTER(Transform), C.c_int,
+                                        C.POINTER(Region), C.c_char_p, C.c_int, C.POINTER(Input), i64p, C.c_int,
+                                        C.POINTER(C.c_int)]
+        L.or_set_chunk_lazy.restype = C.c_int64
+        L.or_set_mask_lazy.argtypes = [C.c_void_p, C.c_int, i64p, C.c_int, C.c_char_p, C.c_char_p, C.c_void_p,
+                                       C.c_int, C.POINTER(Input), C.POINTER(C.c_int)]
         L.or_sum_seq_f64.argtypes = [C.c_void_p, C.c_int64]
         L.or_sum_seq_f64.restype = C.c_double
         L.or_concatenate.argtypes = [C.c_int, C.POINTER(C.c_void_p), i64p, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -315,6 +326,62 @@ def view_scatter(dst, chain, r, src):
     lib().or_view_scatter(_ptr(dst), _i64(dst.shape), dst.ndim, dst.itemsize, _chain(chain), len(chain),
                           C.byref(r), _ptr(src), src.size)
     return dst
+
+
+def _inputs(inputs, keep):
+    """Lazily read operands: each is (array, chain newest first); the array
+    is read where it lies, so passing the destination array itself makes the
+    expression see the writes made so far.  `keep` holds the ctypes buffers
+    alive for the call."""
+    arr = (Input * max(1, len(inputs)))()
+    for k, (a, chain) in enumerate(inputs):
+        assert a.flags.c_contiguous
+        ch = _chain(chain)
+        keep.append(ch)
+        I = arr[k]
+        I.buf = a.ctypes.data
+        I.dtype = _dt_of(a)
+        I.rank = a.ndim
+        for i, v in enumerate(a.shape):
+            I.shape[i] = v
+        I.chain = C.cast(ch, C.POINTER(Transform))
+        I.nchain = len(chain)
+    return arr
+
+
+def set_chunk_lazy(dst, dchain, r, expr, inputs, src_shape):
+    """MultiWritable#set_chunk with a lazily evaluated source (a View is the
+    expression "x0", a ProcView its block): in place on `dst`; returns the
+    flags raised."""
+    assert dst.flags.c_contiguous
+    keep = []
+    flags = C.c_int(0)
+    n = lib().or_set_chunk_lazy(_ptr(dst), _i64(dst.shape), dst.ndim, _chain(dchain), len(dchain), C.byref(r),
+                                expr.encode(), len(inputs), _inputs(inputs, keep), _i64(src_shape), len(src_shape),
+                                C.byref(flags))
+    if n < 0:
+        raise ValueError("oracle could not parse " + expr)
+    return flags.value
+
+
+def set_mask_lazy(dst, mask_expr, value, inputs):
+    """NArray#[]=(mask, value) with the mask and the value (an expression
+    string, or a scalar) evaluated lazily in the reference's loop order; in
+    place on `dst`; returns the flags raised."""
+    assert dst.flags.c_contiguous
+    keep = []
+    flags = C.c_int(0)
+    if isinstance(value, str):
+        vexpr, sc = value.encode(), None
+    else:
+        vexpr, sc = None, np.array([value], dtype=dst.dtype)
+        keep.append(sc)
+    rc = lib().or_set_mask_lazy(_ptr(dst), _dt_of(dst), _i64(dst.shape), dst.ndim, mask_expr.encode(), vexpr,
+                                None if sc is None else sc.ctypes.data, len(inputs), _inputs(inputs, keep),
+                                C.byref(flags))
+    if rc < 0:
+        raise ValueError("oracle could not parse an expression")
+    return flags.value
 
 
 def set_mask(dst, mask, value):

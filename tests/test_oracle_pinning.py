@@ -229,6 +229,47 @@ def test_transform_known_answers():
     assert O.chain_apply([("reverse", rv["shape"])], rv["coord"]) == rv["expected"]
 
 
+def _random_chain(rng, shape, max_len=4):
+    """A random Region/Permute/Reverse/Reshape chain over `shape`, built
+    alongside the same composition in numpy.  Returns (chain newest first,
+    the view's shape, the view of np.arange(size) through numpy), so the last
+    item holds, per view coordinate, the flat source index it maps to."""
+    cur = np.arange(int(np.prod(shape)), dtype=np.int64).reshape(shape)
+    chain = []  # newest first
+    vshape = list(shape)
+    for _ in range(rng.randint(1, max_len)):
+        op = rng.choice(["region", "permute", "reverse", "reshape"])
+        if op == "region":
+            lits = [_random_literal(rng, b) for b in vshape]
+            try:
+                r = O.region(P(lits), vshape)
+            except O.OracleError:
+                continue
+            if r.size == 0:
+                continue
+            cur = cur[_np_select(r)].reshape(r.shape)
+            chain.insert(0, ("region", r))
+            vshape = r.shape
+        elif op == "permute":
+            pat = list(range(len(vshape)))
+            rng.shuffle(pat)
+            cur = np.transpose(cur, pat)
+            chain.insert(0, ("permute", pat))
+            vshape = [vshape[p] for p in pat]
+        elif op == "reverse":
+            cur = cur[tuple(slice(None, None, -1) for _ in vshape)]
+            chain.insert(0, ("reverse", list(vshape)))
+        else:
+            n = int(np.prod(vshape))
+            divs = [d for d in range(1, n + 1) if n % d == 0]
+            d = rng.choice(divs)
+            ns = [d, n // d] if rng.random() < 0.5 else [n]
+            cur = cur.reshape(ns)
+            chain.insert(0, ("reshape", list(vshape), ns))
+            vshape = ns
+    return chain, list(vshape), cur
+
+
 def test_view_chains_vs_numpy():
     """View#to_narr through region/permute/reverse/reshape chains equals the
     same composition done with numpy indexing."""
@@ -237,41 +278,67 @@ def test_view_chains_vs_numpy():
         rank = rng.randint(1, 3)
         shape = [rng.randint(1, 5) for _ in range(rank)]
         src = np.arange(int(np.prod(shape)), dtype=np.int64).reshape(shape)
-        cur = src
-        chain = []  # newest first
-        vshape = list(shape)
-        for _ in range(rng.randint(1, 4)):
-            op = rng.choice(["region", "permute", "reverse", "reshape"])
-            if op == "region":
-                lits = [_random_literal(rng, b) for b in vshape]
-                try:
-                    r = O.region(P(lits), vshape)
-                except O.OracleError:
-                    continue
-                if r.size == 0:
-                    continue
-                cur = cur[_np_select(r)].reshape(r.shape)
-                chain.insert(0, ("region", r))
-                vshape = r.shape
-            elif op == "permute":
-                pat = list(range(len(vshape)))
-                rng.shuffle(pat)
-                cur = np.transpose(cur, pat)
-                chain.insert(0, ("permute", pat))
-                vshape = [vshape[p] for p in pat]
-            elif op == "reverse":
-                cur = cur[tuple(slice(None, None, -1) for _ in vshape)]
-                chain.insert(0, ("reverse", list(vshape)))
-            else:
-                n = int(np.prod(vshape))
-                divs = [d for d in range(1, n + 1) if n % d == 0]
-                d = rng.choice(divs)
-                ns = [d, n // d] if rng.random() < 0.5 else [n]
-                cur = cur.reshape(ns)
-                chain.insert(0, ("reshape", list(vshape), ns))
-                vshape = ns
+        chain, vshape, cur = _random_chain(rng, shape)
         got = O.view_gather(src, chain, vshape)
         assert np.array_equal(got, cur), chain
+
+
+def test_view_scatter_vs_numpy():
+    """MutableView write-through (MultiWritable#unsafe_set_chunk over a view,
+    src/multi_writable.cr:29-36, src/mutable_view.cr:16-18): the chain maps
+    each view coordinate of the region to a flat destination index, found
+    here by pushing np.arange(size) through the same composition in numpy;
+    the source's elements go there in lexicographic order.  A chain applied
+    oldest first, a colexicographic walk of the region or a region step
+    dropped fails this."""
+    rng = random.Random(11)
+    checked = 0
+    for _ in range(300):
+        rank = rng.randint(1, 3)
+        shape = [rng.randint(1, 6) for _ in range(rank)]
+        chain, vshape, idx = _random_chain(rng, shape)
+        lits = [_random_literal(rng, b) for b in vshape[: rng.randint(0, len(vshape))]]
+        try:
+            r = O.region(P(lits), vshape, drop=rng.random() < 0.5)
+        except O.OracleError:
+            continue
+        dst = np.full(shape, -1, np.int32)
+        if r.size == 0:
+            assert np.array_equal(O.view_scatter(dst, chain, r, np.zeros(0, np.int32)), np.full(shape, -1))
+            continue
+        vals = (np.arange(r.size, dtype=np.int32) * 5 + 7)
+        got = O.view_scatter(dst, chain, r, vals)
+        targets = idx[_np_select(r)].reshape(-1)  # lexicographic over the region
+        assert len(set(targets.tolist())) == targets.size  # a view chain is injective
+        exp = np.full(int(np.prod(shape)), -1, np.int32)
+        exp[targets] = vals
+        assert np.array_equal(got.reshape(-1), exp), (shape, chain, lits)
+        checked += 1
+    assert checked > 150
+
+
+def test_view_scatter_known_answers():
+    """MultiWritable#set_chunk(region, chunk) (spec/multi_writable_spec.cr:
+    24-33) written through MutableViews: the identity view with the spec's
+    region, and a view restricted to that region then written whole.  Both
+    must give the spec's buffer; the transposed spelling of the same write
+    (a permuted view of the transposed array) must too."""
+    g = GOLD["multi_writable"]
+    shape, buf = g["shape"], g["buffer"]
+    chunk = np.array(g["chunk"], np.int32).reshape(g["chunk_shape"])
+    exp = g["expected_chunk"]
+    r = O.region(P(g["region"]), shape)
+    base = np.array(buf, np.int32).reshape(shape)
+    assert O.view_scatter(base.copy(), [], r, chunk).reshape(-1).tolist() == exp
+    whole = O.region_cover(r.shape)
+    assert O.view_scatter(base.copy(), [("region", r)], whole, chunk).reshape(-1).tolist() == exp
+    # the transposed array viewed through permute(1, 0) is the original layout
+    t = np.ascontiguousarray(base.T)
+    got = O.view_scatter(t, [("permute", [1, 0])], r, chunk)
+    assert got.T.reshape(-1).tolist() == exp
+    # and the scalar form (spec/multi_writable_spec.cr:14-22)
+    full = np.full(r.size, g["scalar"], np.int32)
+    assert O.view_scatter(base.copy(), [], r, full).reshape(-1).tolist() == g["expected_scalar"]
 
 
 def test_tile_and_iteration_order():
