@@ -188,13 +188,20 @@ def c3_errors(x, storage, res):
     """Relative error of the device's six-result pass against the same block
     evaluated by numpy in Crystal's types ((x+4)**2 in Float32, then * 0.5
     and - x in Float64) and summed in 80-bit long double, 1024 rows at a
-    time; maxima are compared bit for bit."""
+    time; beside it, the distance to ph-core's own sums (Enumerable#sum:
+    sequential Float64 additions in lexicographic order, by the oracle for
+    the full and row sums, row after row for the column sums) and those
+    sequential sums' own error; maxima are compared bit for bit."""
     import numpy as np
+    O = _oracle()
     if storage == "BF16":
         x = (x.astype(np.uint32) << 16).view(np.float32)
     n, m = x.shape
     rows = np.zeros(n, np.longdouble)
     cols = np.zeros(m, np.longdouble)
+    seq_rows = np.zeros(n)
+    seq_cols = np.zeros(m)
+    seq_full = 0.0
     rmax = np.empty(n)
     cmax = np.full(m, -np.inf)
     for r0 in range(0, n, 1024):
@@ -206,16 +213,57 @@ def c3_errors(x, storage, res):
         cols += fl.sum(axis=0)
         rmax[r0:r0 + 1024] = f.max(axis=1)
         cmax = np.maximum(cmax, f.max(axis=0))
+        seq_full = O.sum_seq_from(f, seq_full)
+        seq_rows[r0:r0 + 1024] = O.slices_reduce(f, 0, 0)
+        for row in f:
+            seq_cols += row
     full = rows.sum()
 
     def rel(got, want):
         got = np.asarray(got, np.longdouble)
+        want = np.asarray(want, np.longdouble)
         return float(np.max(np.abs(got - want) / np.abs(want)))
 
     return {"max_rel_err_vs_fp64": {"full_sum": rel(res["full_sum"], full), "row_sum": rel(res["row_sum"], rows),
                                     "col_sum": rel(res["col_sum"], cols)},
+            "max_rel_dist_to_sequential_sum": {"full_sum": rel(res["full_sum"], seq_full),
+                                               "row_sum": rel(res["row_sum"], seq_rows),
+                                               "col_sum": rel(res["col_sum"], seq_cols)},
+            "sequential_sum_rel_err_vs_fp64": {"full_sum": rel(seq_full, full), "row_sum": rel(seq_rows, rows),
+                                               "col_sum": rel(seq_cols, cols)},
             "maxima_bit_exact": bool(res["full_max"] == rmax.max() and np.array_equal(res["row_max"], rmax)
                                      and np.array_equal(res["col_max"], cmax))}
+
+
+def cpu_info():
+    """The host the CPU legs ran on: CPU model and the cores this process may
+    use (the legs themselves use one)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc}
+
+
+def stamp_cpu_legs(obj, info):
+    """Adds the host's CPU model and nproc to every CPU leg (a dict with
+    "cores" and "kind") inside the bench line."""
+    if isinstance(obj, dict):
+        if "cores" in obj and "kind" in obj:
+            obj.update(info)
+        for v in obj.values():
+            stamp_cpu_legs(v, info)
+    elif isinstance(obj, list):
+        for v in obj:
+            stamp_cpu_legs(v, info)
 
 
 def cpu_c4(u0):
@@ -341,6 +389,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": extra,
     }
+    stamp_cpu_legs(line, cpu_info())
     print(json.dumps(line))
 
 
@@ -355,17 +404,29 @@ def other_workloads(phase, ev, args):
     c = phase.NArray.random([n, n], D.F64, seed=0, offset=n * n + n, a=-1.0, b=1.0)
     bt = b.tile(n, 1)
     by = 24 * n * n + 8 * n
-    # eager, as ph-core evaluates it: two operator kernels and a temporary
-    fn = lambda: (a + bt) * c  # noqa: E731
-    fn()
-    ms = time_loop(ev, fn, 10, phase)
-    # fused: one map_with kernel, same per-element operations and roundings
-    fz = lambda: a.map_with(bt, c, block=lambda x, y, z: (x + y) * z)  # noqa: E731
+    # the operator form as ph-core code writes it: a + b.tile is left
+    # pending and fused into the `* c` kernel when the result is used
+    fo = lambda: ((a + bt) * c).materialize()  # noqa: E731
+    fo()
+    mso = time_loop(ev, fo, 10, phase)
+    # the same with the intermediate forced (two kernels and a temporary, as
+    # ph-core's eager operators evaluate it)
+    def f2():
+        t = (a + bt).materialize()
+        return (t * c).materialize()
+    f2()
+    ms2 = time_loop(ev, f2, 10, phase)
+    # map_with: one kernel, same per-element operations and roundings
+    fz = lambda: a.map_with(bt, c, block=lambda x, y, z: (x + y) * z).materialize()  # noqa: E731
     fz()
     msf = time_loop(ev, fz, 10, phase)
     out["c1_broadcast_f64_16384"] = {
-        "eager": {"ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
-                  "note": "(a + b.tile) * c as two elementwise kernels (the reference's eager operators)"},
+        "operators": {"ms": round(mso, 4), "achieved_gbs": round(by / mso / 1e6, 1),
+                      "frac": round(by / mso / 1e6 / PEAK_HBM_GBS, 4),
+                      "note": "(a + b.tile) * c with ph-core's operators: the pending a + b.tile fuses into the "
+                              "product's kernel (one kernel)"},
+        "two_kernels": {"ms": round(ms2, 4), "achieved_gbs": round(by / ms2 / 1e6, 1),
+                        "note": "the same with the intermediate forced: two kernels and a temporary"},
         "fused": {"ms": round(msf, 4), "achieved_gbs": round(by / msf / 1e6, 1),
                   "frac": round(by / msf / 1e6 / PEAK_HBM_GBS, 4),
                   "note": "a.map_with(b.tile, c) { (x + y) * z }: one kernel, stride-0 broadcast"},
@@ -380,12 +441,13 @@ def other_workloads(phase, ev, args):
     b = phase.NArray.random([1, m], D.F64, seed=0, offset=m * m, a=-1.0, b=1.0)
     c = phase.NArray.random([m, m], D.F64, seed=0, offset=m * m + m, a=-1.0, b=1.0)
     bt = b.tile(m, 1)
-    f1 = lambda: (a + bt) * c  # noqa: E731
+    f1 = lambda: ((a + bt) * c).materialize()  # noqa: E731
     f1()
     ms1 = time_loop(ev, f1, 50, phase)
     out["c1_broadcast_f64_1024"] = {"ms": round(ms1, 4), "bytes": 24 * m * m + 8 * m,
                                     "achieved_gbs": round((24 * m * m + 8 * m) / ms1 / 1e6, 1),
-                                    "note": "eager (a + b.tile) * c, two kernels; inputs fit the 256 MB MALL"}
+                                    "note": "(a + b.tile) * c with the operators (one fused kernel); inputs fit "
+                                            "the 256 MB MALL"}
     if not args.no_cpu:
         out["c1_broadcast_f64_1024"]["cpu"] = cpu_c1(a.to_numpy(), b.to_numpy(), c.to_numpy())
     del a, b, c, bt
@@ -578,7 +640,7 @@ def path_ops(phase, ev, args):
         z[mask] = y
     entry("set_mask_source", "ph_map_kernel (select)", ms_src, 13 * nn,
           lambda: _cpu_leg(lambda: O.set_mask(xh.copy(), mh, yh), 13 * nn, "NArray#[]=(mask, y)"))
-    entry("eq", "ph_map_kernel", lambda: x.eq(y), 9 * nn,
+    entry("eq", "ph_map_kernel", lambda: x.eq(y).materialize(), 9 * nn,
           lambda: _cpu_leg(lambda: O.eval_expr("(eq x0 x1)", [xh, yh]), 9 * nn, "x.eq(y)"))
 
     def cpu_coord():
@@ -590,7 +652,7 @@ def path_ops(phase, ev, args):
         return {"value": round(8 * nn / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                 "sample": f"map_with_coord (element and coordinate zipped), full 8192^2, {dt * 1e3:.0f} ms"}
     entry("map_with_coord", "ph_map_kernel (index instruction)",
-          lambda: x.map_with_coord(lambda el, c: el.wadd(c[0]).wsub(c[1])), 8 * nn, cpu_coord)
+          lambda: x.map_with_coord(lambda el, c: el.wadd(c[0]).wsub(c[1])).materialize(), 8 * nn, cpu_coord)
     xt = x.view().permute(1, 0)
 
     def cpu_tmap():
@@ -602,7 +664,7 @@ def path_ops(phase, ev, args):
         return {"value": round(12 * rows * n / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                 "sample": f"map_with over x.permute(1, 0) and y, its first {rows} rows, {dt * 1e3:.0f} ms"}
     entry("map_transposed", "ph_map_tiled (transposed operand staged through LDS)",
-          lambda: xt.map_with(y, block=lambda a, b: a.wadd(b)), 12 * nn, cpu_tmap)
+          lambda: xt.map_with(y, block=lambda a, b: a.wadd(b)).materialize(), 12 * nn, cpu_tmap)
     return res
 
 

@@ -226,32 +226,18 @@ int ph_fill(void *buf, const ph_view_desc *dst, const void *value);
  *   src) (src/n_array.cr:484-492), MultiWritable#unsafe_set_chunk
  *   (src/multi_writable.cr:29-36) through MutableView#unsafe_set_element
  *   (src/mutable_view.cr:16-18).
- * flags: must be 0.  PH_COPY_ZERO_REST (a fill of the rest of dst fused
- *   into the scatter) is reserved and returns PH_E_UNSUPPORTED: the fused
- *   fill + scatter takes the region form, ph_fill_scatter below.
+ * A source that shares the destination's buffer (src == dst) is read as
+ * the reference's element loop reads it: element k (lexicographic over the
+ * shape) sees the writes of the elements before it, since the loop pulls
+ * `src.each` one element at a time (src/n_array.cr:484-492).  The library
+ * finds whether any element reads another's destination and runs the
+ * parallel copy when none does; otherwise it resolves the order exactly
+ * (pointer jumping along the chains of reads after writes).  A source that
+ * overlaps the destination through another base pointer is PH_E_ARGUMENT.
+ * flags: must be 0.
  * ------------------------------------------------------------------------- */
-#define PH_COPY_ZERO_REST 1
 int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src,
                     const ph_view_desc *sd, int32_t flags);
-
-/* The same copy performed strictly in lexicographic order of the logical
- * shape (one device thread), for a source that aliases the destination: each
- * source element is read after every write that precedes it, which is what
- * the reference's element-by-element loop does when a view of an array is
- * written into the same array (src/n_array.cr:484-492 iterating `src.each`
- * lazily, src/multi_writable.cr:29-36). */
-int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd,
-                               const void *src, const ph_view_desc *sd);
-
-/* Masked copy strictly in lexicographic order (one device thread): for every
- * coordinate k in that order, if mask[md(k)] (a Bool view of the same
- * shape) then dst[dd(k)] = src[sd(k)], each source element read after the
- * writes that precede it.  Replaces NArray#[]=(mask, value : MultiIndexable)
- * (src/n_array.cr:523-536) when `value` is a view of the array itself, so
- * that the loop's reads see its own earlier writes. */
-int ph_masked_copy_sequential(void *dst, const ph_view_desc *dd,
-                              const void *src, const ph_view_desc *sd,
-                              const void *mask, const ph_view_desc *md);
 
 /* ---------------------------------------------------------------------------
  * Fused fill + scatter over a whole row-major destination buffer of shape
@@ -280,6 +266,15 @@ int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
  * n_array.cr:589-628), scalar-first patches (src/patches/number.cr:5-21),
  * and ProcView#to_narr (src/proc_view.cr:28-30 + src/view.cr:123-126).
  * In-place map (map!) passes out == in_0 with equal contiguous descriptors.
+ * Inputs may read the output's buffer (the same base pointer, the output's
+ * element type) through any descriptor: element k then sees the writes of
+ * the elements before it in lexicographic order, as the reference's loops
+ * do when a source or mask is a view of the array being written
+ * (src/n_array.cr:484-492, :523-536).  When no element reads another's
+ * output the map runs in one parallel pass; otherwise the order is resolved
+ * by repeated passes to a fixed point, and chains of more than 12 dependent
+ * elements by a sequential loop on one device thread.  An input that
+ * overlaps the output through another base pointer is PH_E_ARGUMENT.
  * A program with PH_OP_INDEX (NArray#map_with_index / map_with_index! /
  * map_with_coord, src/n_array.cr:605-650) sees each element's lexicographic
  * position in out_desc's shape as an Int32; a shape of more than 2^31 - 1
@@ -330,9 +325,13 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
  *   u'[i,j] = u[i,j] + ((((uN - 4u) + uS) + uW) + uE) * r
  * the 2-D generalisation of examples/heat_equation.cr:38-51 (same operation
  * order as :47, `(s[i-1] - 2*s[i] + s[i+1]) * COEFF`, then `el + diff` :50).
- * dtype PH_F32 or PH_F64; u holds the initial field and receives the final
- * one; tmp is scratch of the same size.  Up to eight steps run per pass
- * over the grid (temporal blocking), bit-identical to single steps.  If
+ * dtype PH_F64 rounds every operation as written (the reference's example
+ * is Float64).  dtype PH_F32, the build's own Float32 variant, computes the
+ * last multiply and add with one rounding, u' = fma(d, r, u); its accuracy
+ * is stated against the Float64 run.  u holds the initial field and
+ * receives the final one; tmp is scratch of the same size.  Up to eight
+ * (Float64) or twelve (Float32) steps run per pass over the grid (temporal
+ * blocking), bit-identical to single steps.  If
  * energy_every > 0, energy_out
  * (device, steps/energy_every + 1 doubles) receives the FP64 sum of u at
  * steps 0, energy_every, 2*energy_every, ...
@@ -405,13 +404,21 @@ int ph_batch_destroy(void *handle);
  * kernels ph_map (kind 0) and ph_reduce2d_all (kind 1) would use for `prog`
  * over contiguous inputs of the given dtypes and a rows x cols shape, and
  * (kind 2) the tiled map ph_map uses when input 0 is the transposed view of
- * a cols x rows array (rows and cols of 16 or more).  It needs no GPU, so
- * program lowering can be checked on a build host.
+ * a cols x rows array (rows and cols of 16 or more).  Kind 3 compiles the
+ * kernels of a map whose input 0 reads the output's own buffer transposed
+ * (rows == cols; input 0 has the output's dtype), kind 4 those of a copy
+ * of in_dtypes[0] elements that reverses a buffer onto itself.  It needs no
+ * GPU, so program lowering can be checked on a build host.
  * ------------------------------------------------------------------------- */
 int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes,
                      int64_t rows, int64_t cols, int32_t kind);
 /* Number of fused-kernel modules compiled (not loaded from cache) so far. */
 long ph_jit_compiles(void);
+/* Number of device operations issued so far: each kernel launch, or each
+ * fixed sequence of launches one entry point issues as a unit (a reduction's
+ * tile and finish kernels, a batch pass), counts once.  PH_HIP_FAIL_AFTER=n
+ * (read at the first launch) refuses every operation after the first n. */
+long ph_launch_count(void);
 
 #ifdef __cplusplus
 }

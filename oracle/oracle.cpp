@@ -1320,6 +1320,14 @@ int or_set_mask_lazy(void *dst, int dtype, const int64_t *shape, int rank, const
 // (src/multi_indexable.cr:742-786).
 // ===========================================================================
 
+// Enumerable#sum for Float64 elements continued from a running total `start`
+// (the sequential sum of a long sequence, fed in pieces).
+double or_sum_seq_f64_from(const double *x, int64_t n, double start) {
+  double s = start;
+  for (int64_t i = 0; i < n; ++i) s += x[i];
+  return s;
+}
+
 // Enumerable#sum for Float64 elements: sequential lex-order sum from 0.0.
 double or_sum_seq_f64(const double *x, int64_t n) {
   double s = 0.0;
@@ -1525,7 +1533,10 @@ int or_heat1d_reference(int64_t steps, double *out) {
 // the ny x nx grid read 0 (Dirichlet zero), the per-cell operation order of
 // heat_equation.cr:47/:50 extended to two axes:
 //   d = (((uN - 4u) + uS) + uW) + uE;   u' = u + d * r
-// dtype 1 (F32, all arithmetic in float, r rounded to float) or 2 (F64).
+// dtype 2 (F64): exactly that, every operation rounded.  dtype 1 (F32, all
+// arithmetic in float, r rounded to float): the build's Float32 variant of
+// the workload (the reference's example is Float64 only), whose last two
+// operations round once, u' = fma(d, r, u), as the device computes them.
 // energy (may be NULL): FP64 sum of the field (correctly rounded) at steps
 // 0, every, 2*every, ...
 extern "C++" {
@@ -1550,7 +1561,8 @@ static void heat2d_t(T *u, int64_t ny, int64_t nx, T r, int64_t steps,
         T w = j > 0 ? u[i * nx + j - 1] : T(0);
         T e = j + 1 < nx ? u[i * nx + j + 1] : T(0);
         T d = (((n - T(4) * c) + s) + w) + e;
-        v[(size_t)(i * nx + j)] = c + d * r;
+        if constexpr (sizeof(T) == 4) v[(size_t)(i * nx + j)] = std::fma(d, r, c);
+        else v[(size_t)(i * nx + j)] = c + d * r;
       }
     }
     std::memcpy(u, v.data(), sizeof(T) * (size_t)(ny * nx));

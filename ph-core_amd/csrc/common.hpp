@@ -24,8 +24,17 @@ hipStream_t stream();
 int device();
 
 // Lazily grown device scratch, valid until the next call that asks for more;
-// all work on it is ordered by the single library stream.
+// all work on it is ordered by the single library stream.  Slots: 0-1
+// gathered map and reduction inputs, 2 reduction and heat partials, 3 flag
+// words, 4-7 the buffers of maps whose inputs share the output's buffer.
 int scratch(size_t bytes, void **out, int slot = 0);
+
+// ph_strided_copy when source and destination share a buffer: the copy in
+// the reference's lexicographic order (each element read after the writes
+// of the elements before it), run in parallel where that order cannot be
+// observed.  *done is false when no element reads another's destination,
+// and the ordinary copy kernels then give the same result.
+int ordered_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd, bool *done);
 
 size_t dtype_size(int dt);
 
@@ -54,6 +63,7 @@ void collapse(const ph_view_desc *const *descs, int nops, Layout *out);
 // PH_E_HIP before it is issued, so host code can exercise its device-error
 // handling without a faulting kernel.
 int launch_gate(const char *what);
+long launch_count();
 int check_launch(const char *what);
 
 // Zeroes n int32 words on the library stream with a kernel (a memset node

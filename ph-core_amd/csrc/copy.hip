@@ -358,62 +358,6 @@ void make_tile_args(const ph::Layout &L, int A, int B, int tile, TileArgs *t) {
   t->nblocks = others * t->ta * t->tb;
 }
 
-// One thread, lexicographic order: the aliasing-safe copy.
-template <typename E>
-__global__ void k_copy_sequential(E *dst, ph_view_desc dd, const E *src, ph_view_desc sd,
-                                  int64_t total) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int64_t coord[PH_MAX_RANK];
-  for (int i = 0; i < dd.view.rank; ++i) coord[i] = 0;
-  for (int64_t g = 0; g < total; ++g) {
-    dst[map_index(dd, coord)] = src[map_index(sd, coord)];
-    for (int i = dd.view.rank - 1; i >= 0; --i) {
-      if (++coord[i] < dd.view.shape[i]) break;
-      coord[i] = 0;
-    }
-  }
-}
-
-// One thread, lexicographic order, element k copied only where the Bool
-// mask holds: the aliasing-safe masked copy.
-template <typename E>
-__global__ void k_mask_sequential(E *dst, ph_view_desc dd, const E *src, ph_view_desc sd,
-                                  const unsigned char *mask, ph_view_desc md, int64_t total) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int64_t coord[PH_MAX_RANK];
-  for (int i = 0; i < dd.view.rank; ++i) coord[i] = 0;
-  for (int64_t g = 0; g < total; ++g) {
-    if (mask[map_index(md, coord)]) dst[map_index(dd, coord)] = src[map_index(sd, coord)];
-    for (int i = dd.view.rank - 1; i >= 0; --i) {
-      if (++coord[i] < dd.view.shape[i]) break;
-      coord[i] = 0;
-    }
-  }
-}
-
-template <typename E>
-int launch_mask_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
-                           const void *mask, const ph_view_desc *md) {
-  const int64_t total = ph::desc_count(dd);
-  if (total == 0) return PH_OK;
-  int rc = ph::launch_gate("sequential masked copy");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_mask_sequential<E>, dim3(1), dim3(64), 0, ph::stream(), (E *)dst, *dd, (const E *)src, *sd,
-                     (const unsigned char *)mask, *md, total);
-  return ph::check_launch("k_mask_sequential");
-}
-
-template <typename E>
-int launch_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd) {
-  const int64_t total = ph::desc_count(dd);
-  if (total == 0) return PH_OK;
-  int rc = ph::launch_gate("sequential copy");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_copy_sequential<E>, dim3(1), dim3(64), 0, ph::stream(), (E *)dst, *dd,
-                     (const E *)src, *sd, total);
-  return ph::check_launch("k_copy_sequential");
-}
-
 int64_t grid_for(int64_t blocks) {
   const int64_t cap = 1 << 20;
   return blocks < 1 ? 1 : (blocks > cap ? cap : blocks);
@@ -856,64 +800,21 @@ int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src, const ph
     if (dd->view.shape[i] != sd->view.shape[i])
       return ph::set_error(PH_E_SHAPE, "copy shapes differ on axis %d (%lld vs %lld)", i,
                            (long long)sd->view.shape[i], (long long)dd->view.shape[i]);
-  if (flags & PH_COPY_ZERO_REST)
-    return ph::set_error(PH_E_UNSUPPORTED,
-                         "PH_COPY_ZERO_REST needs the region form: use ph_fill_scatter");
   if (flags != 0) return ph::set_error(PH_E_ARGUMENT, "ph_strided_copy: unknown flags %d", flags);
+  if (ph::desc_count(dd) == 0) return PH_OK;
+  const size_t es = ph::dtype_size(dd->dtype);
+  if (src == dst) {
+    // the reference's order decides what a source reading the destination
+    // sees; the parallel kernels below are used when it cannot matter
+    bool done = false;
+    rc = ph::ordered_copy(dst, dd, src, sd, &done);
+    if (rc || done) return rc;
+  } else {
+    const char *d0 = (const char *)dst, *s0 = (const char *)src;
+    if (!(s0 + (size_t)sd->extent * es <= d0 || d0 + (size_t)dd->extent * es <= s0))
+      return ph::set_error(PH_E_ARGUMENT, "copy source overlaps the destination through another base pointer");
+  }
   return dispatch_copy(dst, dd, src, sd, nullptr);
-}
-
-int ph_strided_copy_sequential(void *dst, const ph_view_desc *dd, const void *src,
-                               const ph_view_desc *sd) {
-  PH_GUARD();
-  PH_REQUIRE_INIT();
-  int rc = ph::validate_desc(dd, "copy destination");
-  if (rc) return rc;
-  rc = ph::validate_desc(sd, "copy source");
-  if (rc) return rc;
-  if (ph::dtype_size(dd->dtype) != ph::dtype_size(sd->dtype))
-    return ph::set_error(PH_E_ARGUMENT, "copy between dtypes %d and %d", sd->dtype, dd->dtype);
-  if (dd->view.rank != sd->view.rank)
-    return ph::set_error(PH_E_SHAPE, "copy between ranks %d and %d", sd->view.rank, dd->view.rank);
-  for (int i = 0; i < dd->view.rank; ++i)
-    if (dd->view.shape[i] != sd->view.shape[i])
-      return ph::set_error(PH_E_SHAPE, "copy shapes differ on axis %d", i);
-  switch (ph::dtype_size(dd->dtype)) {
-    case 1: return launch_sequential<uint8_t>(dst, dd, src, sd);
-    case 2: return launch_sequential<uint16_t>(dst, dd, src, sd);
-    case 4: return launch_sequential<uint32_t>(dst, dd, src, sd);
-    case 8: return launch_sequential<uint64_t>(dst, dd, src, sd);
-  }
-  return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
-}
-
-int ph_masked_copy_sequential(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd,
-                              const void *mask, const ph_view_desc *md) {
-  PH_GUARD();
-  PH_REQUIRE_INIT();
-  int rc = ph::validate_desc(dd, "copy destination");
-  if (rc) return rc;
-  rc = ph::validate_desc(sd, "copy source");
-  if (rc) return rc;
-  rc = ph::validate_desc(md, "mask");
-  if (rc) return rc;
-  if (!dst || !src || !mask) return ph::set_error(PH_E_ARGUMENT, "masked copy: null pointer");
-  if (md->dtype != PH_BOOL) return ph::set_error(PH_E_ARGUMENT, "mask dtype %d, not Bool", md->dtype);
-  if (ph::dtype_size(dd->dtype) != ph::dtype_size(sd->dtype))
-    return ph::set_error(PH_E_ARGUMENT, "copy between dtypes %d and %d", sd->dtype, dd->dtype);
-  if (dd->view.rank != sd->view.rank || dd->view.rank != md->view.rank)
-    return ph::set_error(PH_E_SHAPE, "masked copy between ranks %d, %d and %d", sd->view.rank, dd->view.rank,
-                         md->view.rank);
-  for (int i = 0; i < dd->view.rank; ++i)
-    if (dd->view.shape[i] != sd->view.shape[i] || dd->view.shape[i] != md->view.shape[i])
-      return ph::set_error(PH_E_SHAPE, "masked copy shapes differ on axis %d", i);
-  switch (ph::dtype_size(dd->dtype)) {
-    case 1: return launch_mask_sequential<uint8_t>(dst, dd, src, sd, mask, md);
-    case 2: return launch_mask_sequential<uint16_t>(dst, dd, src, sd, mask, md);
-    case 4: return launch_mask_sequential<uint32_t>(dst, dd, src, sd, mask, md);
-    case 8: return launch_mask_sequential<uint64_t>(dst, dd, src, sd, mask, md);
-  }
-  return ph::set_error(PH_E_ARGUMENT, "unsupported dtype %d", dd->dtype);
 }
 
 int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape, int32_t rank,

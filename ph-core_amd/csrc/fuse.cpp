@@ -899,6 +899,7 @@ struct RedArgs {
   void *rowpart; void *colpart; void *rowfinal;
   double *full_sum, *full_max, *row_sum, *row_max, *col_sum, *col_max;
   int *flags;
+  i64 *rowzero;
 };
 )PH";
 
@@ -915,6 +916,7 @@ struct RedArgs {
   void *rowpart, *colpart, *rowfinal;
   double *full_sum, *full_max, *row_sum, *row_max, *col_sum, *col_max;
   int *flags;
+  int64_t *rowzero;  // per row: the first zero's (view-order index << 1 | sign), or INT64_MAX
 };
 
 struct RedPlan {
@@ -927,7 +929,24 @@ struct RedPlan {
   size_t state_bytes;
   const char *acc_t;   // float / double / i64
   bool vec_ok[PH_MAX_INPUTS];  // whole-vector loads of the lane's VEC columns
+  bool colex = false;  // the view's lexicographic order runs down the columns (a swapped transposed view)
 };
+
+// Statements evaluating the program at element (R, C) of the 2-D
+// arrangement; returns the value as a double expression.
+std::string emit_elem_at(const ph_prog *p, int n_in, const int *in_dt, const int64_t *rs, const int64_t *cs,
+                         const std::string &R, const std::string &C, const std::string &sfx, std::string *code) {
+  std::vector<std::string> inexpr;
+  for (int j = 0; j < n_in; ++j) {
+    const std::string x = fmt("e%d%s", j, sfx.c_str());
+    *code += fmt("      const %s %s = ph_ld((const %s *)A.in[%d], A.in_off[%d] + (%s) * %s + (%s) * %s);\n",
+                 ctype(compute_dt(in_dt[j])), x.c_str(), stype(in_dt[j]), j, j, R.c_str(), lit_i64(rs[j]).c_str(),
+                 C.c_str(), lit_i64(cs[j]).c_str());
+    inexpr.push_back(x);
+  }
+  const std::string res = emit_prog(p, inexpr, code, sfx);
+  return "(double)(" + res + ")";
+}
 
 // Tile kernel for floating-point accumulation (states SF / SD).  A
 // workgroup owns TR rows x 256*VEC columns; lane l of wavefront w owns VEC
@@ -1222,6 +1241,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   // block's 256 rows are folded in order (a pairwise tree of adjacent
   // ranges) into one state per block for the full result.
   const int64_t ncb = P.do_cols ? (P.cols + 63) / 64 : 0;
+  const bool fix = std::string(P.state) != "SI";  // float maxima follow Enumerable#max's `>`
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_finish(RedArgs A) {\n";
   s += fmt("  typedef %s S;\n", P.state);
   s += "  __shared__ S sh[256];\n  int fl = 0;\n";
@@ -1237,7 +1257,26 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
     s += fmt("    if (part == 0 && c < %s) {\n", lit_i64(P.cols).c_str());
     s += "      S r = sh[lane];\n      for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
-    s += "      if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n      if (A.col_max) A.col_max[c] = st_max(r);\n";
+    s += "      if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n";
+    if (fix) {
+      // Enumerable#max keeps the slice's first element unless a later one is
+      // greater: a leading NaN stays, and of equal zeros the first one's sign
+      s += "      if (A.col_max) {\n        double mx = st_max(r);\n";
+      {
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "c", "_cf", &s);
+        s += "        const double f0 = " + e_ + ";\n";
+      }
+      s += "        if (f0 != f0) mx = f0;\n";
+      s += fmt("        else if (mx == 0.0) {\n          for (i64 rr = 0; rr < %s; ++rr) {\n", lit_i64(P.rows).c_str());
+      {
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "rr", "c", "_cz", &s);
+        s += "            const double g = " + e_ + ";\n";
+      }
+      s += "            if (g == 0.0) { mx = g; break; }\n          }\n        }\n";
+      s += "        A.col_max[c] = mx;\n      }\n";
+    } else {
+      s += "      if (A.col_max) A.col_max[c] = st_max(r);\n";
+    }
     s += "    }\n  } else {\n";
   } else {
     s += "  {\n";
@@ -1247,7 +1286,29 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     s += "    const i64 i = rb * 256 + threadIdx.x;\n    S a; st_init(a);\n";
     s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
              lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
-    s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
+    s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n";
+    if (fix) {
+      // first element and, when the row's maximum is a zero, the first zero
+      // (its view-order index and sign also serve the full result)
+      s += "      const double m0 = st_max(a);\n      double mx = m0;\n      i64 zkey = 0x7fffffffffffffffLL;\n";
+      {
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "0", "_rf", &s);
+        s += "      const double f0 = " + e_ + ";\n";
+      }
+      s += fmt("      if (m0 == 0.0) {\n        for (i64 cc = 0; cc < %s; ++cc) {\n", lit_i64(P.cols).c_str());
+      {
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "cc", "_rz", &s);
+        s += "          const double g = " + e_ + ";\n";
+      }
+      const std::string key = P.colex ? fmt("cc * %s + i", lit_i64(P.rows).c_str()) : fmt("i * %s + cc", lit_i64(P.cols).c_str());
+      s += "          if (g == 0.0) { mx = g; zkey = ((" + key + ") << 1) | (__builtin_signbit(g) ? 1 : 0); break; }\n";
+      s += "        }\n      }\n      if (f0 != f0) mx = f0;\n";
+      s += "      if (A.row_max) A.row_max[i] = mx;\n";
+      if (P.do_full) s += "      A.rowzero[i] = zkey;\n      __threadfence();\n";
+      s += "    }\n";
+    } else {
+      s += "      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
+    }
     if (P.do_full) {
       const int64_t nrb = (P.rows + 255) / 256;
       s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
@@ -1266,8 +1327,27 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       s += "      const i64 b = threadIdx.x * per; const i64 e = (b + per < n) ? b + per : n;\n";
       s += "      S f; st_init(f);\n      for (i64 j = b; j < e; ++j) f = st_comb(f, ((const S *)A.rowfinal)[j]);\n";
       s += "      f = wave_fold(f);\n";
+      if (fix) {
+        // the first zero of the whole view: the least key over the rows
+        // (after wave_fold only lane 0 holds the whole maximum)
+        s += "      i64 zk = 0x7fffffffffffffffLL;\n      const double fm0 = __shfl(st_max(f), 0);\n";
+        s += "      if (A.full_max && fm0 == 0.0)\n        for (i64 j = threadIdx.x; j < " +
+             lit_i64(P.rows) + "; j += 64) zk = A.rowzero[j] < zk ? A.rowzero[j] : zk;\n";
+        s += "      for (int o = 32; o > 0; o >>= 1) { const i64 t = __shfl_xor(zk, o); zk = t < zk ? t : zk; }\n";
+      }
       s += "      if (threadIdx.x == 0) {\n        if (A.full_sum) { A.full_sum[0] = st_sum(f); fl |= st_ovf(f); }\n";
-      s += "        if (A.full_max) A.full_max[0] = st_max(f);\n      }\n    }\n";
+      if (fix) {
+        s += "        if (A.full_max) {\n          double mx = st_max(f);\n";
+        {
+          const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "0", "_ff", &s);
+          s += "          const double f0 = " + e_ + ";\n";
+        }
+        s += "          if (f0 != f0) mx = f0;\n";
+        s += "          else if (mx == 0.0 && zk != 0x7fffffffffffffffLL) mx = (zk & 1) ? -0.0 : 0.0;\n";
+        s += "          A.full_max[0] = mx;\n        }\n      }\n    }\n";
+      } else {
+        s += "        if (A.full_max) A.full_max[0] = st_max(f);\n      }\n    }\n";
+      }
     }
   }
   s += "  }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
@@ -1292,8 +1372,9 @@ struct View2D {
 // Runs the fused reduction over a 2-D arrangement (inputs affine).
 int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const int *in_dt,
                  const View2D &V, int acc, double *full_sum, double *full_max, double *row_sum,
-                 double *row_max, double *col_sum, double *col_max, int32_t *flags_out) {
+                 double *row_max, double *col_sum, double *col_max, int32_t *flags_out, bool colex) {
   RedPlan P;
+  P.colex = colex;
   P.rows = V.rows;
   P.cols = V.cols;
   P.vec = 4;
@@ -1339,7 +1420,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   key_add(&key, V.rs, sizeof(int64_t) * (size_t)n_in);
   key_add(&key, V.cs, sizeof(int64_t) * (size_t)n_in);
   key_add(&key, P.vec_ok, sizeof(bool) * (size_t)n_in);
-  const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.vec, P.do_rows, P.do_cols, P.do_full};
+  const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.vec, P.do_rows, P.do_cols, P.do_full, P.colex};
   key_add(&key, plan, sizeof(plan));
   key += P.state;
   hipFunction_t ktile, kfin;
@@ -1361,9 +1442,10 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
   const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
   const size_t rowfinal_b = P.do_rows ? (size_t)P.rows * P.state_bytes : 0;
+  const size_t rowzero_b = P.do_full ? (size_t)P.rows * sizeof(int64_t) : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   void *base;
-  rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + 256, &base, 2);
+  rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + al(rowzero_b) + 256, &base, 2);
   if (rc) return rc;
   int *dflags;
   rc = get_flags_buffer(&dflags, may_flag);
@@ -1378,6 +1460,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   A.rowpart = b;
   A.colpart = b + al(rowpart_b);
   A.rowfinal = b + al(rowpart_b) + al(colpart_b);
+  A.rowzero = (int64_t *)(b + al(rowpart_b) + al(colpart_b) + al(rowfinal_b));
   A.full_sum = full_sum;
   A.full_max = full_max;
   A.row_sum = row_sum;
@@ -1460,7 +1543,534 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
   return read_flags(dflags, may_flag, flags_out);
 }
 
+// ---------------------------------------------------------------------------
+// Inputs that share the output's buffer
+// ---------------------------------------------------------------------------
+// The reference produces the elements of a map (or a copy) one at a time in
+// lexicographic order of the output's shape and reads each element's inputs
+// when it gets there (src/n_array.cr:484-492 pulls `src.each` lazily,
+// :523-536 calls `mask.get` and `value.get` per coordinate).  An input that
+// reads the buffer being written therefore sees the writes of the elements
+// before it.  With the output map W injective, element L (its lexicographic
+// index) reading address a of the shared buffer depends on element
+// L' = W^-1(a) when that exists and L' < L; otherwise it reads the old value
+// (L' > L is written later; L' = L is its own element, read before written).
+//
+// 1. A classifying pass finds whether any element reads an address another
+//    element writes (a conflict) and whether any reads one written before it
+//    (a read after write).  Without conflicts the ordinary parallel kernels
+//    give the sequential result, in place.
+// 2. Conflicts but no read after write: every read sees the old value; the
+//    results go to scratch T, indexed by L, and are then written out.
+// 3. Reads after writes: Jacobi iteration on T.  T_0 reads old values
+//    everywhere; T_{m+1}(L) reads T_m(L') where L' < L and old values
+//    elsewhere.  Dependencies point only to smaller L, so the fixed point is
+//    unique and is the sequential result, and an element whose dependency
+//    chain is d deep is exact from T_d on.  A pass that reproduces T bit for
+//    bit has reached it, and its flags are the sequential run's.  Past
+//    kOrdMaxIter passes (a deep chain, such as a shift by one element), one
+//    device thread runs the loop in order, kSeqChunk elements per launch.
+// A plain copy takes pointer jumping instead of step 3: element L's value is
+// the old value at the read address of its chain's root, and
+// ceil(log2(depth)) passes over the chain links find every root.
+constexpr int kOrdMaxIter = 12;
+constexpr int64_t kSeqChunk = 1 << 16;
+constexpr int kCtlConflict = 1, kCtlRaw = 2, kCtlChanged = 4;
+
+// W^-1 for an affine output whose axes nest: sorted by decreasing |stride|,
+// each stride exceeds the span of all narrower axes, so an address has at
+// most one coordinate, found by dividing from the widest axis down.  Every
+// region, permutation and reversal of a row-major buffer nests.
+struct WriteInverse {
+  bool ok = false;
+  int64_t base = 0;  // the view's lowest address
+  int n = 0;         // axes of extent > 1, widest stride first
+  int64_t step[PH_MAX_RANK], ext[PH_MAX_RANK], lex[PH_MAX_RANK];
+  bool neg[PH_MAX_RANK];
+};
+
+WriteInverse write_inverse(const ph_view_desc &W) {
+  WriteInverse v;
+  if (W.nstage != 0) return v;
+  const ph_affine &a = W.view;
+  int64_t lex[PH_MAX_RANK];
+  int64_t st = 1;
+  for (int i = a.rank - 1; i >= 0; --i) {
+    lex[i] = st;
+    st *= a.shape[i];
+  }
+  v.base = a.offset;
+  int idx[PH_MAX_RANK];
+  for (int i = 0; i < a.rank; ++i) {
+    if (a.shape[i] <= 1) continue;
+    if (a.stride[i] == 0) return v;
+    idx[v.n++] = i;
+    if (a.stride[i] < 0) v.base += a.stride[i] * (a.shape[i] - 1);
+  }
+  std::sort(idx, idx + v.n, [&](int x, int y) { return std::llabs(a.stride[x]) > std::llabs(a.stride[y]); });
+  for (int k = 0; k < v.n; ++k) {
+    const int i = idx[k];
+    v.step[k] = std::llabs(a.stride[i]);
+    v.ext[k] = a.shape[i];
+    v.lex[k] = lex[i];
+    v.neg[k] = a.stride[i] < 0;
+  }
+  for (int k = 0; k < v.n; ++k) {
+    int64_t span = 0;
+    for (int j = k + 1; j < v.n; ++j) span += v.step[j] * (v.ext[j] - 1);
+    if (v.step[k] <= span) return v;
+  }
+  v.ok = true;
+  return v;
+}
+
+// Whether two descriptors address the same element at every coordinate.
+bool same_map(const ph_view_desc &a, const ph_view_desc &b) {
+  if (a.nstage != 0 || b.nstage != 0 || a.view.rank != b.view.rank || a.view.offset != b.view.offset) return false;
+  for (int i = 0; i < a.view.rank; ++i) {
+    if (a.view.shape[i] != b.view.shape[i]) return false;
+    if (a.view.shape[i] > 1 && a.view.stride[i] != b.view.stride[i]) return false;
+  }
+  return true;
+}
+
+// Lowest and highest element index a descriptor can reach.
+void desc_span(const ph_view_desc &d, int64_t *lo, int64_t *hi) {
+  if (d.nstage > 0) {
+    *lo = 0;
+    *hi = d.extent - 1;
+    return;
+  }
+  int64_t l = d.view.offset, h = d.view.offset;
+  for (int i = 0; i < d.view.rank; ++i) {
+    const int64_t span = (d.view.shape[i] - 1) * d.view.stride[i];
+    if (span < 0) l += span;
+    else h += span;
+  }
+  *lo = l;
+  *hi = h;
+}
+
+bool spans_overlap(const ph_view_desc &a, const ph_view_desc &b) {
+  int64_t al, ah, bl, bh;
+  desc_span(a, &al, &ah);
+  desc_span(b, &bl, &bh);
+  return !(ah < bl || bh < al);
+}
+
+// The coordinates c0.. of lexicographic index `L` over the view's shape.
+std::string emit_coords(const ph_affine &v, const std::string &L, const std::string &sfx) {
+  std::string s = fmt("      i64 rr%s = %s; (void)rr%s;\n", sfx.c_str(), L.c_str(), sfx.c_str());
+  for (int i = v.rank - 1; i >= 0; --i) {
+    if (i == 0) {
+      s += fmt("      const i64 c0%s = rr%s;\n", sfx.c_str(), sfx.c_str());
+    } else {
+      const std::string n = lit_i64(v.shape[i]);
+      s += fmt("      const i64 c%d%s = rr%s %% %s; rr%s /= %s;\n", i, sfx.c_str(), sfx.c_str(), n.c_str(), sfx.c_str(),
+               n.c_str());
+    }
+  }
+  return s;
+}
+
+// The element index descriptor `d` gives the coordinates c0..: the affine
+// view, then each reshape/tile stage (delinearise, re-linearise).
+std::string emit_addr(const ph_view_desc &d, const std::string &name, const std::string &sfx) {
+  std::string s = fmt("      i64 %s = %s", name.c_str(), lit_i64(d.view.offset).c_str());
+  for (int i = 0; i < d.view.rank; ++i)
+    if (d.view.shape[i] > 1 && d.view.stride[i] != 0)
+      s += fmt(" + c%d%s * %s", i, sfx.c_str(), lit_i64(d.view.stride[i]).c_str());
+  s += ";\n";
+  for (int k = 0; k < d.nstage; ++k) {
+    const ph_affine &st = d.stage[k];
+    s += fmt("      { i64 ix_ = %s; i64 o_ = %s;\n", name.c_str(), lit_i64(st.offset).c_str());
+    for (int j = st.rank - 1; j >= 0; --j) {
+      const std::string n = lit_i64(st.shape[j]);
+      s += fmt("        { const i64 q_ = ix_ / %s; o_ += (ix_ - q_ * %s) * %s; ix_ = q_; }\n", n.c_str(), n.c_str(),
+               lit_i64(st.stride[j]).c_str());
+    }
+    s += fmt("        %s = o_; }\n", name.c_str());
+  }
+  return s;
+}
+
+// ok<sfx>: address `a` is written by some element; lp<sfx>: that element's
+// lexicographic index.
+std::string emit_inverse(const WriteInverse &w, const std::string &a, const std::string &sfx) {
+  const char *x = sfx.c_str();
+  std::string s = fmt("      i64 r%s = %s - %s; bool ok%s = r%s >= 0; i64 lp%s = 0;\n", x, a.c_str(),
+                      lit_i64(w.base).c_str(), x, x, x);
+  for (int k = 0; k < w.n; ++k) {
+    const std::string st = lit_i64(w.step[k]);
+    const std::string c = w.neg[k] ? fmt("(%s - q_)", lit_i64(w.ext[k] - 1).c_str()) : std::string("q_");
+    s += fmt("      { const i64 q_ = r%s / %s; r%s -= q_ * %s; ok%s = ok%s && q_ < %s; lp%s += %s * %s; }\n", x,
+             st.c_str(), x, st.c_str(), x, x, lit_i64(w.ext[k]).c_str(), x, c.c_str(), lit_i64(w.lex[k]).c_str());
+  }
+  s += fmt("      ok%s = ok%s && r%s == 0;\n", x, x, x);
+  return s;
+}
+
+const char *raw_type(size_t esize) {
+  switch (esize) {
+    case 1: return "unsigned char";
+    case 2: return "unsigned short";
+    case 4: return "unsigned";
+    default: return "unsigned long long";
+  }
+}
+
+const char *kOrdPrelude = R"PH(
+struct OrdArgs { const void *in[8]; void *out; const void *tp; void *tn; const i64 *q0; i64 *q1; int *flags; i64 iter; i64 l0; i64 l1; };
+__device__ __forceinline__ bool ph_same(bool a, bool b) { return a == b; }
+__device__ __forceinline__ bool ph_same(int a, int b) { return a == b; }
+__device__ __forceinline__ bool ph_same(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool ph_same(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+__device__ __forceinline__ void ph_stp(unsigned char *p, i64 i, bool v) { p[i] = v ? 1 : 0; }
+__device__ __forceinline__ void ph_stp(int *p, i64 i, int v) { p[i] = v; }
+__device__ __forceinline__ void ph_stp(float *p, i64 i, float v) { p[i] = v; }
+__device__ __forceinline__ void ph_stp(double *p, i64 i, double v) { p[i] = v; }
+// control bits of a wavefront, one atomic
+__device__ __forceinline__ void ph_ctl(int *w, int ctl) {
+  const unsigned long long b1 = __ballot(ctl & 1), b2 = __ballot(ctl & 2), b4 = __ballot(ctl & 4);
+  const int v = (b1 ? 1 : 0) | (b2 ? 2 : 0) | (b4 ? 4 : 0);
+  if ((threadIdx.x & 63) == 0 && v) atomicOr(w, v);
+}
+)PH";
+
+// What the generated kernels need to know: the program (null: a copy of
+// input 0 of `esize`-byte elements), the operands and which inputs read the
+// output's buffer.
+struct OrdSpec {
+  const ph_prog *prog = nullptr;
+  int n_in = 1;
+  int in_dt[PH_MAX_INPUTS] = {0};
+  const ph_view_desc *in[PH_MAX_INPUTS] = {nullptr};
+  const ph_view_desc *out = nullptr;
+  unsigned hazard = 0;
+  size_t esize = 0;
+};
+
+std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
+  const int64_t n = ph::desc_count(S.out);
+  const std::string N = lit_i64(n);
+  const bool copy = S.prog == nullptr;
+  const std::string OST = copy ? raw_type(S.esize) : stype(S.out->dtype);
+  std::string s = kPrelude;
+  s += kOrdPrelude;
+  const std::string loop = "    for (i64 L = (i64)blockIdx.x * 256 + threadIdx.x; L < " + N +
+                           "; L += (i64)gridDim.x * 256) {\n";
+  auto in_st = [&](int j) { return copy ? std::string(raw_type(S.esize)) : std::string(stype(S.in_dt[j])); };
+  auto load = [&](int j, const std::string &ptr, const std::string &idx) {
+    return copy ? fmt("((const %s *)%s)[%s]", in_st(j).c_str(), ptr.c_str(), idx.c_str())
+                : fmt("ph_ld((const %s *)%s, %s)", in_st(j).c_str(), ptr.c_str(), idx.c_str());
+  };
+  auto vtype = [&](int j) { return copy ? std::string(raw_type(S.esize)) : std::string(ctype(compute_dt(S.in_dt[j]))); };
+  // element program over x0.. (a copy: x0 itself); defines `res`
+  auto body = [&](std::string *code) {
+    if (copy) {
+      *code += "      const auto res = x0;\n";
+      return;
+    }
+    std::vector<std::string> inexpr;
+    for (int j = 0; j < S.n_in; ++j) inexpr.push_back(fmt("x%d", j));
+    std::string b;
+    const std::string r = emit_prog(S.prog, inexpr, &b, "");
+    *code += "      const i64 ph_lidx = L; (void)ph_lidx;\n";
+    *code += b;
+    *code += fmt("      const auto res = %s;\n", r.c_str());
+  };
+
+  // classify: conflicts and reads after writes
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_classify(OrdArgs A) {\n  int ctl = 0;\n";
+  s += loop + emit_coords(S.out->view, "L", "");
+  for (int j = 0; j < S.n_in; ++j) {
+    if (!(S.hazard >> j & 1)) continue;
+    const std::string sx = fmt("_%d", j);
+    s += emit_addr(*S.in[j], "a" + sx, "") + emit_inverse(w, "a" + sx, sx);
+    s += fmt("      if (ok%s && lp%s != L) ctl |= 1; if (ok%s && lp%s < L) ctl |= 2;\n", sx.c_str(), sx.c_str(),
+             sx.c_str(), sx.c_str());
+  }
+  s += "    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+
+  // relax: one Jacobi pass into tn, reading tp where an earlier element wrote
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_relax(OrdArgs A) {\n  int fl = 0, ctl = 0;\n";
+  s += loop + emit_coords(S.out->view, "L", "");
+  for (int j = 0; j < S.n_in; ++j) {
+    const std::string sx = fmt("_%d", j);
+    s += emit_addr(*S.in[j], "a" + sx, "");
+    if (S.hazard >> j & 1) {
+      s += emit_inverse(w, "a" + sx, sx);
+      s += fmt("      const bool dep%s = ok%s && lp%s < L; if (dep%s) ctl |= 2;\n", sx.c_str(), sx.c_str(), sx.c_str(),
+               sx.c_str());
+      s += fmt("      const %s x%d = (A.iter > 0 && dep%s) ? %s : %s;\n", vtype(j).c_str(), j, sx.c_str(),
+               load(j, "A.tp", "lp" + sx).c_str(), load(j, fmt("A.in[%d]", j), "a" + sx).c_str());
+    } else {
+      s += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), "a" + sx).c_str());
+    }
+  }
+  body(&s);
+  if (copy) {
+    s += fmt("      if (A.iter > 0 && res != ((const %s *)A.tp)[L]) ctl |= 4;\n", OST.c_str());
+    s += fmt("      ((%s *)A.tn)[L] = res;\n", OST.c_str());
+  } else {
+    s += fmt("      if (A.iter > 0 && !ph_same(res, ph_ld((const %s *)A.tp, L))) ctl |= 4;\n", OST.c_str());
+    s += fmt("      ph_stp((%s *)A.tn, L, res);\n", OST.c_str());
+  }
+  s += "    }\n  if (fl) atomicOr(A.flags, fl);\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+
+  // write: out[W(L)] = tp[L]
+  s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_write(OrdArgs A) {\n";
+  s += loop + emit_coords(S.out->view, "L", "") + emit_addr(*S.out, "w_", "");
+  s += fmt("      ((%s *)A.out)[w_] = ((const %s *)A.tp)[L];\n    }\n}\n", OST.c_str(), OST.c_str());
+
+  // seq: the loop itself, in order, over [l0, l1), one thread
+  s += "extern \"C\" __global__ void __launch_bounds__(64) ph_ord_seq(OrdArgs A) {\n";
+  s += "  if (blockIdx.x != 0 || threadIdx.x != 0) return;\n  int fl = 0;\n";
+  s += "    for (i64 L = A.l0; L < A.l1; ++L) {\n" + emit_coords(S.out->view, "L", "");
+  for (int j = 0; j < S.n_in; ++j) {
+    const std::string sx = fmt("_%d", j);
+    s += emit_addr(*S.in[j], "a" + sx, "");
+    s += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), "a" + sx).c_str());
+  }
+  body(&s);
+  s += emit_addr(*S.out, "w_", "");
+  if (copy) s += fmt("      ((%s *)A.out)[w_] = res;\n", OST.c_str());
+  else s += fmt("      ph_stp((%s *)A.out, w_, res);\n", OST.c_str());
+  s += "    }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
+
+  if (copy) {
+    // chain links: q1[L] = the earlier element whose write L reads, or -1
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_link(OrdArgs A) {\n";
+    s += loop + emit_coords(S.out->view, "L", "") + emit_addr(*S.in[0], "a_0", "") + emit_inverse(w, "a_0", "_0");
+    s += "      A.q1[L] = (ok_0 && lp_0 < L) ? lp_0 : -1;\n    }\n}\n";
+    // one pointer-jumping pass: q1[L] = q0[q0[L]] where both are links
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_jump(OrdArgs A) {\n  int ctl = 0;\n";
+    s += loop + "      const i64 q = A.q0[L]; i64 nq = q;\n";
+    s += "      if (q >= 0) { const i64 qq = A.q0[q]; if (qq >= 0) { nq = qq; ctl |= 4; } }\n";
+    s += "      A.q1[L] = nq;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    // gather: tn[L] = the old value at the read address of L's root
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_gather(OrdArgs A) {\n";
+    s += loop + "      const i64 q = A.iter ? A.q0[L] : -1; const i64 R = q >= 0 ? q : L;\n";
+    s += emit_coords(S.out->view, "R", "") + emit_addr(*S.in[0], "a_0", "");
+    s += fmt("      ((%s *)A.tn)[L] = ((const %s *)A.in[0])[a_0];\n    }\n}\n", OST.c_str(), OST.c_str());
+  }
+  return s;
+}
+
+struct OrdKernels {
+  hipFunction_t classify = nullptr, relax = nullptr, write = nullptr, seq = nullptr;
+  hipFunction_t link = nullptr, jump = nullptr, gather = nullptr;
+};
+
+std::map<std::string, OrdKernels> &ord_cache() {
+  static std::map<std::string, OrdKernels> c;
+  return c;
+}
+
+int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
+  std::string key = S.prog ? "O" + prog_key(S.prog) : fmt("C%zu", S.esize);
+  key_add(&key, S.in_dt, sizeof(int) * (size_t)S.n_in);
+  key_add(&key, &S.hazard, sizeof(S.hazard));
+  key_add(&key, S.out, sizeof(ph_view_desc));
+  for (int j = 0; j < S.n_in; ++j) key_add(&key, S.in[j], sizeof(ph_view_desc));
+  auto hit = ord_cache().find(key);
+  if (hit != ord_cache().end()) {
+    *K = hit->second;
+    return PH_OK;
+  }
+  const std::string src = ord_source(S, w);
+  int rc = ph::jit_kernel(src, "ph_ord_classify", &K->classify);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_relax", &K->relax);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_write", &K->write);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_seq", &K->seq);
+  if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_link", &K->link);
+  if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_jump", &K->jump);
+  if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_gather", &K->gather);
+  if (rc) return rc;
+  ord_cache()[key] = *K;
+  return PH_OK;
+}
+
+struct OrdArgsHost {
+  const void *in[PH_MAX_INPUTS];
+  void *out;
+  const void *tp;
+  void *tn;
+  const int64_t *q0;
+  int64_t *q1;
+  int *flags;
+  int64_t iter, l0, l1;
+};
+
+int ord_launch(hipFunction_t fn, OrdArgsHost &A, int64_t n, bool one_thread, const char *what) {
+  size_t asz = sizeof(A);
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &asz, HIP_LAUNCH_PARAM_END};
+  int rc = ph::launch_gate(what);
+  if (rc) return rc;
+  const int64_t blocks = one_thread ? 1 : std::min<int64_t>((n + 255) / 256, 8192);
+  PH_HIP(hipModuleLaunchKernel(fn, (unsigned)(blocks < 1 ? 1 : blocks), 1, 1, one_thread ? 64 : 256, 1, 1, 0,
+                               ph::stream(), nullptr, cfg));
+  return PH_OK;
+}
+
+// Clears the two flag words, runs `fn`, and reads them back.
+int ord_pass(hipFunction_t fn, OrdArgsHost &A, int64_t n, int *words, const char *what) {
+  int rc = ph::clear_i32(A.flags, 2);
+  if (!rc) rc = ord_launch(fn, A, n, false, what);
+  if (rc) return rc;
+  PH_HIP(hipMemcpyAsync(words, A.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  return PH_OK;
+}
+
+// The sequential loop, in bounded launches; flags as the loop raises them.
+int ord_sequential(const OrdKernels &K, OrdArgsHost &A, int64_t n, int32_t *flags_out) {
+  int rc = ph::clear_i32(A.flags, 2);
+  if (rc) return rc;
+  for (int64_t l0 = 0; l0 < n; l0 += kSeqChunk) {
+    A.l0 = l0;
+    A.l1 = std::min(n, l0 + kSeqChunk);
+    rc = ord_launch(K.seq, A, 1, true, "ordered map (sequential)");
+    if (rc) return rc;
+  }
+  int words[2] = {0, 0};
+  PH_HIP(hipMemcpyAsync(words, A.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  if (flags_out) *flags_out = words[0];
+  return PH_OK;
+}
+
+// Runs S in the reference's order.  *done false: no element reads another's
+// output, and the caller's parallel kernels give the same result.
+int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *flags_out, bool *done) {
+  *done = true;
+  if (flags_out) *flags_out = 0;
+  const int64_t n = ph::desc_count(S.out);
+  const WriteInverse w = write_inverse(*S.out);
+  OrdKernels K;
+  int rc = ord_kernels(S, w, &K);
+  if (rc) return rc;
+  OrdArgsHost A;
+  std::memset(&A, 0, sizeof(A));
+  for (int j = 0; j < S.n_in; ++j) A.in[j] = ins[j];
+  A.out = out;
+  void *fw;
+  rc = ph::scratch(64, &fw, 3);
+  if (rc) return rc;
+  A.flags = (int *)fw;
+  if (!w.ok) return ord_sequential(K, A, n, flags_out);  // an output that does not nest
+  int words[2] = {0, 0};
+  rc = ord_pass(K.classify, A, n, words, "ordered map (classify)");
+  if (rc) return rc;
+  if (!(words[1] & kCtlConflict)) {
+    *done = false;
+    return PH_OK;
+  }
+  const size_t esize = S.prog ? ph::dtype_size(S.out->dtype) : S.esize;
+  void *t[2];
+  rc = ph::scratch((size_t)n * esize, &t[0], 4);
+  if (!rc) rc = ph::scratch((size_t)n * esize, &t[1], 5);
+  if (rc) return rc;
+  const bool raw = (words[1] & kCtlRaw) != 0;
+  int cur = 0;  // T holding the result
+  if (!S.prog) {
+    // a copy: chain links, pointer jumping to the roots, one gather
+    A.iter = 0;
+    if (raw) {
+      void *q[2];
+      rc = ph::scratch((size_t)n * sizeof(int64_t), &q[0], 6);
+      if (!rc) rc = ph::scratch((size_t)n * sizeof(int64_t), &q[1], 7);
+      if (rc) return rc;
+      A.q1 = (int64_t *)q[0];
+      rc = ord_launch(K.link, A, n, false, "ordered copy (links)");
+      if (rc) return rc;
+      int qc = 0;
+      for (int round = 0; round < 64; ++round) {
+        A.q0 = (const int64_t *)q[qc];
+        A.q1 = (int64_t *)q[qc ^ 1];
+        rc = ord_pass(K.jump, A, n, words, "ordered copy (jump)");
+        if (rc) return rc;
+        qc ^= 1;
+        if (!(words[1] & kCtlChanged)) break;
+      }
+      A.q0 = (const int64_t *)q[qc];
+      A.iter = 1;
+    }
+    A.tn = t[0];
+    rc = ord_launch(K.gather, A, n, false, "ordered copy (gather)");
+    if (rc) return rc;
+  } else {
+    int flags = 0;
+    bool exact = false;
+    for (int it = 0; it <= kOrdMaxIter; ++it) {
+      A.iter = it;
+      A.tp = t[cur];
+      A.tn = t[cur ^ 1];
+      rc = ord_pass(K.relax, A, n, words, "ordered map (pass)");
+      if (rc) return rc;
+      cur ^= 1;
+      flags = words[0];
+      if ((it == 0 && !(words[1] & kCtlRaw)) || (it > 0 && !(words[1] & kCtlChanged))) {
+        exact = true;
+        break;
+      }
+    }
+    if (!exact) return ord_sequential(K, A, n, flags_out);
+    if (flags_out) *flags_out = flags;
+  }
+  A.tp = t[cur];
+  return ord_launch(K.write, A, n, false, "ordered map (write)");
+}
+
+// The inputs of a map that read the output's buffer through another
+// mapping; an input overlapping the output through another base pointer,
+// or reading it as another element type, is refused.
+int hazard_inputs(int n_in, const void *const *ins, const ph_view_desc *in_descs, const void *out,
+                  const ph_view_desc *out_desc, unsigned *mask) {
+  *mask = 0;
+  const char *ob = (const char *)out;
+  const size_t oes = ph::dtype_size(out_desc->dtype);
+  for (int j = 0; j < n_in; ++j) {
+    const char *ib = (const char *)ins[j];
+    const size_t ies = ph::dtype_size(in_descs[j].dtype);
+    if (ib != ob) {
+      const bool disjoint = ib + (size_t)in_descs[j].extent * ies <= ob || ob + (size_t)out_desc->extent * oes <= ib;
+      if (!disjoint)
+        return ph::set_error(PH_E_ARGUMENT, "input %d overlaps the output through another base pointer", j);
+      continue;
+    }
+    if (same_map(in_descs[j], *out_desc)) continue;  // each element reads its own output first
+    if (in_descs[j].nstage == 0 && !spans_overlap(in_descs[j], *out_desc)) continue;
+    if (in_descs[j].dtype != out_desc->dtype)
+      return ph::set_error(PH_E_ARGUMENT, "input %d reads the output's buffer as another element type", j);
+    *mask |= 1u << j;
+  }
+  return PH_OK;
+}
+
 }  // namespace
+
+namespace ph {
+
+int ordered_copy(void *dst, const ph_view_desc *dd, const void *src, const ph_view_desc *sd, bool *done) {
+  *done = false;
+  if (same_map(*dd, *sd)) {  // every element copied onto itself
+    *done = true;
+    return PH_OK;
+  }
+  if (dd->nstage == 0 && sd->nstage == 0 && !spans_overlap(*dd, *sd)) return PH_OK;
+  OrdSpec S;
+  S.n_in = 1;
+  S.in_dt[0] = sd->dtype;
+  S.in[0] = sd;
+  S.out = dd;
+  S.hazard = 1;
+  S.esize = dtype_size(dd->dtype);
+  const void *ins[1] = {src};
+  return run_ordered(S, ins, dst, nullptr, done);
+}
+
+}  // namespace ph
 
 extern "C" {
 
@@ -1475,6 +2085,43 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   int rc = validate_prog(prog, n_in, in_dt, kind == 0);
   if (rc) return rc;
   if (rows < 1 || cols < 1) return ph::set_error(PH_E_ARGUMENT, "empty shape");
+  if (kind == 3 || kind == 4) {
+    // the kernels of a map whose input 0 reads the output's buffer
+    // transposed (3), or of a copy reversing a buffer onto itself (4)
+    if (kind == 3 && rows != cols) return ph::set_error(PH_E_ARGUMENT, "kind 3 needs a square shape");
+    ph_view_desc d[PH_MAX_INPUTS + 1];
+    OrdSpec S;
+    S.n_in = kind == 3 ? n_in : 1;
+    for (int j = 0; j <= S.n_in; ++j) {
+      std::memset(&d[j], 0, sizeof(d[j]));
+      d[j].dtype = j < S.n_in ? in_dt[j] : (kind == 3 ? prog->out_dtype : in_dt[0]);
+      d[j].extent = rows * cols;
+      d[j].view.rank = 2;
+      d[j].view.shape[0] = rows;
+      d[j].view.shape[1] = cols;
+      d[j].view.stride[0] = cols;
+      d[j].view.stride[1] = 1;
+    }
+    if (kind == 3) {
+      d[0].view.stride[0] = 1;
+      d[0].view.stride[1] = cols;
+    } else {
+      d[0].view.offset = rows * cols - 1;
+      d[0].view.stride[0] = -cols;
+      d[0].view.stride[1] = -1;
+    }
+    S.prog = kind == 3 ? prog : nullptr;
+    for (int j = 0; j < S.n_in; ++j) {
+      S.in_dt[j] = in_dt[j];
+      S.in[j] = &d[j];
+    }
+    S.out = &d[S.n_in];
+    S.hazard = 1;
+    S.esize = ph::dtype_size(in_dt[0]);
+    if (kind == 3 && in_dt[0] != prog->out_dtype)
+      return ph::set_error(PH_E_ARGUMENT, "kind 3: input 0 must have the output's dtype");
+    return ph::jit_compile_only(ord_source(S, write_inverse(*S.out)));
+  }
   if (kind == 2) {
     if (prog_uses_index(prog)) return ph::set_error(PH_E_ARGUMENT, "the tiled map has no element index");
     ph_view_desc d[PH_MAX_INPUTS + 1];
@@ -1569,6 +2216,30 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
     return ph::set_error(PH_E_OVERFLOW, "element index of %lld elements exceeds Int32",
                          (long long)ph::desc_count(out_desc));
   if (ph::desc_count(out_desc) == 0) return PH_OK;
+  // inputs that read the output's buffer through another mapping: the
+  // reference's element order decides what they see
+  unsigned hz = 0;
+  rc = hazard_inputs(n_in, ins, in_descs, out, out_desc, &hz);
+  if (rc) return rc;
+  if (hz) {
+    OrdSpec S;
+    S.prog = prog;
+    S.n_in = n_in;
+    for (int j = 0; j < n_in; ++j) {
+      S.in_dt[j] = in_dt[j];
+      S.in[j] = &in_descs[j];
+    }
+    S.out = out_desc;
+    S.hazard = hz;
+    bool done = false;
+    int32_t fl = 0;
+    rc = run_ordered(S, ins, out, &fl, &done);
+    if (rc) return rc;
+    if (done) {
+      if (flags_out) *flags_out = fl;
+      return PH_OK;
+    }
+  }
   // inputs with stages are gathered first, and transposed inputs (while
   // the two scratch slots last) are copied through the tiled transpose
   // when the output is contiguous along its innermost axis
@@ -1662,12 +2333,12 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   return read_flags(dflags, may_flag, flags_out);
 }
 
-int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
-                    const ph_view_desc *in_descs, int32_t acc, double *full_sum, double *full_max,
-                    double *row_sum, double *row_max, double *col_sum, double *col_max,
-                    int32_t *flags_out) {
-  PH_GUARD();
-  PH_REQUIRE_INIT();
+// `colex`: the caller's view is the transpose of in_descs, so its
+// lexicographic order runs down their columns.
+static int reduce2d_all_impl(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                             const ph_view_desc *in_descs, int32_t acc, double *full_sum, double *full_max,
+                             double *row_sum, double *row_max, double *col_sum, double *col_max,
+                             int32_t *flags_out, bool colex) {
   if (flags_out) *flags_out = 0;
   int in_dt[PH_MAX_INPUTS];
   int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
@@ -1678,8 +2349,8 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
   if (swap_transposed_2d(prog, n_in, in_descs)) {
     ph_view_desc sw[PH_MAX_INPUTS];
     for (int j = 0; j < n_in; ++j) sw[j] = swapped_2d(in_descs[j]);
-    return ph_reduce2d_all(prog, n_in, ins, sw, acc, full_sum, full_max, col_sum, col_max, row_sum, row_max,
-                           flags_out);
+    return reduce2d_all_impl(prog, n_in, ins, sw, acc, full_sum, full_max, col_sum, col_max, row_sum, row_max,
+                             flags_out, !colex);
   }
   View2D V;
   const void *ptrs[PH_MAX_INPUTS];
@@ -1699,14 +2370,22 @@ int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
     V.off[j] = local[j].view.offset;
   }
   return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, full_sum, full_max, row_sum, row_max, col_sum,
-                      col_max, flags_out);
+                      col_max, flags_out, colex);
 }
 
-int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
-              int32_t keep_axis, int32_t ops, int32_t acc, double *out_sum, double *out_max,
-              int32_t *flags_out) {
+int ph_reduce2d_all(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                    const ph_view_desc *in_descs, int32_t acc, double *full_sum, double *full_max,
+                    double *row_sum, double *row_max, double *col_sum, double *col_max,
+                    int32_t *flags_out) {
   PH_GUARD();
   PH_REQUIRE_INIT();
+  return reduce2d_all_impl(prog, n_in, ins, in_descs, acc, full_sum, full_max, row_sum, row_max, col_sum, col_max,
+                           flags_out, false);
+}
+
+static int reduce_impl(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
+                       int32_t keep_axis, int32_t ops, int32_t acc, double *out_sum, double *out_max,
+                       int32_t *flags_out, bool colex) {
   if (flags_out) *flags_out = 0;
   int in_dt[PH_MAX_INPUTS];
   int rc = check_inputs(prog, n_in, ins, in_descs, in_dt);
@@ -1724,8 +2403,8 @@ int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const p
   if (rank == 2 && swap_transposed_2d(prog, n_in, in_descs)) {
     ph_view_desc sw[PH_MAX_INPUTS];
     for (int j = 0; j < n_in; ++j) sw[j] = swapped_2d(in_descs[j]);
-    return ph_reduce(prog, n_in, ins, sw, keep_axis < 0 ? keep_axis : 1 - keep_axis, ops, acc, out_sum, out_max,
-                     flags_out);
+    return reduce_impl(prog, n_in, ins, sw, keep_axis < 0 ? keep_axis : 1 - keep_axis, ops, acc, out_sum, out_max,
+                       flags_out, !colex);
   }
   // Arrange as 2-D: rows x cols in lexicographic order of the reduced data.
   //   keep -1         : [prod(shape[:-1]), shape[-1]]      -> full
@@ -1820,13 +2499,21 @@ int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const p
   }
   if (keep_axis == -1)
     return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, sum, mx, nullptr, nullptr, nullptr, nullptr,
-                        flags_out);
+                        flags_out, colex);
   if (col_mode)
     return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, nullptr, nullptr, nullptr, nullptr, sum, mx,
-                        flags_out);
+                        flags_out, colex);
   (void)row_mode;
   return run_reduce2d(prog, n_in, ptrs, in_dt, V, acc, nullptr, nullptr, sum, mx, nullptr, nullptr,
-                      flags_out);
+                      flags_out, colex);
+}
+
+int ph_reduce(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
+              int32_t keep_axis, int32_t ops, int32_t acc, double *out_sum, double *out_max,
+              int32_t *flags_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  return reduce_impl(prog, n_in, ins, in_descs, keep_axis, ops, acc, out_sum, out_max, flags_out, false);
 }
 
 }  // extern "C"

@@ -17,6 +17,10 @@
 // overflows, so uN - 4u rounds once whether it is computed as a multiply and
 // a subtract or as one fused multiply-add.  The fused form runs only when the
 // host has shown that 4u cannot overflow in the passes ahead (fused_safe).
+// Float32 grids (the build's own variant of the workload; the reference's
+// example is Float64) also compute u + d * r as one fused multiply-add,
+// which rounds once instead of twice: one instruction less per cell and
+// step, with the accuracy stated against the Float64 run.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -158,7 +162,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_heat_multi(const T *__restrict_
       if constexpr (kFused) nc = fma_t(-four, c[i], n[i]);  // exact 4u: one rounding either way
       else nc = n[i] - four * c[i];
       const T d = ((nc + s[i]) + w) + e;
-      o[i] = c[i] + d * r;
+      if constexpr (sizeof(T) == 4) o[i] = fma_t(d, r, c[i]);  // Float32: u + d*r rounded once
+      else o[i] = c[i] + d * r;
     }
   };
 

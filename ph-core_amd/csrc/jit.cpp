@@ -1,6 +1,7 @@
 // hipRTC module cache: compile once per (source, target), keep the module for
 // the life of the process, and persist the code object on disk so a later
-// process (or a test run) skips the compile.
+// process (or a test run) skips the compile.  The disk cache is keyed on the
+// source, the target, the compile options and the hipRTC version.
 #include "jit.hpp"
 
 #include <hip/hiprtc.h>
@@ -78,6 +79,27 @@ void write_file_atomic(const std::string &path, const std::vector<char> &data) {
   std::rename(tmp.c_str(), path.c_str());
 }
 
+// Options of every hipRTC compile (besides the target): no contraction of
+// a * b + c into one rounding, and no packed FP32 instructions
+// (v_pk_add/mul/fma_f32) in generated code, exactly as for the
+// ahead-of-time kernels.
+const std::vector<std::string> &compile_options() {
+  static const std::vector<std::string> o = {"-O3", "-ffp-contract=off", "-std=c++17", "-fno-fast-math",
+                                             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  return o;
+}
+
+// Everything a code object depends on: the source, the target, the options
+// and the hipRTC version.
+std::string cache_key(const std::string &source, const std::string &arch) {
+  int major = 0, minor = 0;
+  hiprtcVersion(&major, &minor);
+  std::string k = source + "\n// target " + arch + "\n// hiprtc " + std::to_string(major) + "." +
+                  std::to_string(minor) + "\n// options";
+  for (const std::string &o : compile_options()) k += " " + o;
+  return k + "\n";
+}
+
 int compile(const std::string &source, const std::string &arch, std::vector<char> *code) {
   // PH_JIT_DUMP=<dir>: keep a copy of every generated source (for inspection)
   if (const char *dump = std::getenv("PH_JIT_DUMP")) {
@@ -91,12 +113,10 @@ int compile(const std::string &source, const std::string &arch, std::vector<char
   if (hiprtcCreateProgram(&prog, source.c_str(), "ph_fused.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS)
     return set_error(PH_E_HIP, "hiprtcCreateProgram failed");
-  std::string arch_opt = "--offload-arch=" + arch;
-  // no packed FP32 instructions (v_pk_add/mul/fma_f32) in generated code,
-  // exactly as for the ahead-of-time kernels
-  const char *opts[] = {arch_opt.c_str(), "-O3", "-ffp-contract=off", "-std=c++17", "-fno-fast-math",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  const std::string arch_opt = "--offload-arch=" + arch;
+  std::vector<const char *> opts = {arch_opt.c_str()};
+  for (const std::string &o : compile_options()) opts.push_back(o.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   if (r != HIPRTC_SUCCESS) {
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
@@ -147,14 +167,17 @@ static int get_module(const std::string &source, hipModule_t *out) {
     return PH_OK;
   }
   const std::string arch = jit_arch();
+  const std::string key = cache_key(source, arch);
   char name[64];
-  std::snprintf(name, sizeof(name), "%016llx-%s.co",
-                (unsigned long long)fnv1a(source + "\n//" + arch), arch.c_str());
+  std::snprintf(name, sizeof(name), "%016llx-%s", (unsigned long long)fnv1a(key), arch.c_str());
   const std::string dir = cache_dir();
-  const std::string path = dir + "/" + name;
-  std::vector<char> code;
+  const std::string path = dir + "/" + name + ".co", key_path = dir + "/" + name + ".key";
+  std::vector<char> code, stored;
   hipModule_t mod = nullptr;
-  if (read_file(path, &code) && hipModuleLoadData(&mod, code.data()) == hipSuccess) {
+  // a cached object is used only when the key stored beside it is this one
+  // (the file name is a 64-bit hash; the stored key rules out a collision)
+  if (read_file(key_path, &stored) && std::string(stored.begin(), stored.end()) == key &&
+      read_file(path, &code) && hipModuleLoadData(&mod, code.data()) == hipSuccess) {
     c[source].module = mod;
     *out = mod;
     return PH_OK;
@@ -167,6 +190,7 @@ static int get_module(const std::string &source, hipModule_t *out) {
     return set_error(PH_E_HIP, "hipModuleLoadData: %s", hipGetErrorString(e));
   mkdirs(dir);
   write_file_atomic(path, code);
+  write_file_atomic(key_path, std::vector<char>(key.begin(), key.end()));
   c[source].module = mod;
   *out = mod;
   return PH_OK;

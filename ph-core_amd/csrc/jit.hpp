@@ -8,8 +8,8 @@
 namespace ph {
 
 // Compiles `source` (once per process; also cached on disk under
-// $PH_JIT_CACHE_DIR, default ~/.cache/phhip-jit, keyed by a hash of the
-// source and the target) and returns the kernel named `name`.
+// $PH_JIT_CACHE_DIR, default ~/.cache/phhip-jit, keyed by the source, the
+// target, the compile options and the hipRTC version) and returns the kernel named `name`.
 int jit_kernel(const std::string &source, const char *name, hipFunction_t *out);
 
 // Same module, another kernel of it (the module must already be compiled).

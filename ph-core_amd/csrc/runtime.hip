@@ -19,7 +19,7 @@ struct Scratch {
   void *ptr = nullptr;
   size_t bytes = 0;
 };
-Scratch g_scratch[4];
+Scratch g_scratch[8];
 long g_launches = 0;
 long g_fail_after = -2;  // -2: not read yet, -1: disabled
 }  // namespace
@@ -163,6 +163,8 @@ void collapse(const ph_view_desc *const *descs, int nops, Layout *out) {
   out->rank = r;
 }
 
+long launch_count() { return g_launches; }
+
 int launch_gate(const char *what) {
   if (g_fail_after == -2) {
     const char *e = std::getenv("PH_HIP_FAIL_AFTER");
@@ -184,6 +186,9 @@ int check_launch(const char *what) {
 }  // namespace ph
 
 extern "C" {
+
+long ph_launch_count(void) { return ph::launch_count(); }
+
 
 int ph_init(int device) {
   PH_GUARD();

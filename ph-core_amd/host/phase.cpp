@@ -1077,7 +1077,60 @@ static ph_view_desc contiguous_desc(DType dt, const Shape &shape, int64_t extent
 static MirrorStats g_mirror;
 MirrorStats mirror_stats() { return g_mirror; }
 
+// Whether any element of the program can raise a flag (checked Int32
+// arithmetic, floored division or modulo, a negative Int32 exponent).
+static bool prog_may_raise(const ph_prog &p) {
+  // registers holding an Int32 constant (a divisor known when the program
+  // is built: // and % by one other than 0 and -1 cannot raise)
+  bool is_const[256] = {false};
+  int64_t cval[256] = {0};
+  for (int k = 0; k < p.n_insn; ++k) {
+    const ph_insn &in = p.insn[k];
+    const bool konst = in.op == PH_OP_CONST && in.dt == PH_I32;
+    if (in.dt == PH_I32) {
+      switch (in.op) {
+        case PH_OP_ADD: case PH_OP_SUB: case PH_OP_MUL: case PH_OP_IPOW: case PH_OP_NEG: case PH_OP_ABS:
+          return true;
+        case PH_OP_FLOORDIV: case PH_OP_MOD:
+          if (!is_const[in.b] || cval[in.b] == 0 || cval[in.b] == -1) return true;
+          break;
+        default:
+          break;
+      }
+    }
+    is_const[in.dst] = konst;
+    cval[in.dst] = konst ? in.imm.i : 0;
+  }
+  return false;
+}
+
+void Storage::run_pending_map() {
+  std::shared_ptr<PendingMap> pm = std::move(pending_map);
+  pending_map.reset();
+  if (!pm || elements == 0) return;
+  std::vector<const void *> ptrs;
+  std::vector<ph_view_desc> descs;
+  for (const Operand &o : pm->inputs) {
+    ptrs.push_back(o.storage->dev_read());
+    descs.push_back(o.desc);
+  }
+  const ph_prog prog = pm->body.compile((int)pm->inputs.size());
+  int32_t flags = 0;
+  check(ph_map(&prog, (int32_t)pm->inputs.size(), ptrs.data(), descs.data(), dptr, &pm->out, &flags),
+        "map (pending operator result)");
+}
+
+void Storage::before_write() {
+  if (readers.empty()) return;
+  std::vector<std::weak_ptr<Storage>> rs;
+  rs.swap(readers);
+  for (const auto &w : rs)
+    if (auto r = w.lock())
+      if (r->pending_map) r->run_pending_map();
+}
+
 const void *Storage::dev_read() {
+  if (pending_map) run_pending_map();
   if (pending_fill) {
     Scalar v = *pending_fill;
     pending_fill.reset();
@@ -1118,6 +1171,7 @@ void Storage::forget_host() {
 }
 
 void *Storage::dev_write() {
+  before_write();
   dev_read();
   if (held_) {
     std::fill(page_.begin(), page_.end(), (uint8_t)0);
@@ -1128,6 +1182,7 @@ void *Storage::dev_write() {
 
 void Storage::ensure_pages(size_t off, size_t len) {
   if (off + len > bytes) throw IndexError("host mirror access out of range");
+  if (pending_map) run_pending_map();
   if (host_.size() != bytes) {
     host_.assign(bytes, 0);
     page_.assign((bytes + kPage - 1) / kPage, (uint8_t)0);
@@ -1159,6 +1214,7 @@ const unsigned char *Storage::host_read(size_t off, size_t len) {
 }
 
 unsigned char *Storage::host_write(size_t off, size_t len) {
+  before_write();
   ensure_pages(off, len);
   if (len > 0)
     for (size_t p = off / kPage; p <= (off + len - 1) / kPage; ++p)
@@ -1268,17 +1324,18 @@ static bool view_is_rowmajor(const ph_affine &v) {
   return true;
 }
 
-static void push_stage(ph_view_desc &d, const ph_affine &stage_map, const ph_affine &new_view) {
-  if (d.nstage >= PH_MAX_STAGES)
-    throw ArgumentError("view chain has more than " + std::to_string(PH_MAX_STAGES) +
-                        " reshape/tile stages; materialise it with to_narr first");
+// Adds a reshape/tile stage; false (d unchanged) when the descriptor has no
+// room for another, and the view then keeps the transform for later.
+static bool push_stage(ph_view_desc &d, const ph_affine &stage_map, const ph_affine &new_view) {
+  if (d.nstage >= PH_MAX_STAGES) return false;
   for (int s = d.nstage; s > 0; --s) d.stage[s] = d.stage[s - 1];
   d.stage[0] = stage_map;
   d.nstage += 1;
   d.view = new_view;
+  return true;
 }
 
-static void compose_reshape(ph_view_desc &d, const Shape &new_shape) {
+static bool compose_reshape(ph_view_desc &d, const Shape &new_shape) {
   ph_affine nv;
   std::memset(&nv, 0, sizeof(nv));
   nv.rank = (int32_t)new_shape.size();
@@ -1291,12 +1348,12 @@ static void compose_reshape(ph_view_desc &d, const Shape &new_shape) {
   if (view_is_rowmajor(d.view)) {  // linear index maps straight through
     nv.offset = d.view.offset;
     d.view = nv;
-    return;
+    return true;
   }
-  push_stage(d, d.view, nv);
+  return push_stage(d, d.view, nv);
 }
 
-static void compose_tile(ph_view_desc &d, const Shape &src_shape, const std::vector<int64_t> &counts) {
+static bool compose_tile(ph_view_desc &d, const Shape &src_shape, const std::vector<int64_t> &counts) {
   bool need_mod = false;
   for (size_t i = 0; i < src_shape.size(); ++i)
     if (src_shape[i] != 1 && counts[i] != 1) need_mod = true;
@@ -1307,9 +1364,9 @@ static void compose_tile(ph_view_desc &d, const Shape &src_shape, const std::vec
   }
   if (!need_mod) {
     d.view = nv;
-    return;
+    return true;
   }
-  if (2 * src_shape.size() > PH_MAX_RANK) throw ArgumentError("tile of more than 4 axes needs a materialised copy");
+  if (2 * src_shape.size() > PH_MAX_RANK) return false;
   // linearise the tiled coordinate, delinearise as [k0, s0, k1, s1, ...]
   ph_affine stage;
   std::memset(&stage, 0, sizeof(stage));
@@ -1330,7 +1387,7 @@ static void compose_tile(ph_view_desc &d, const Shape &src_shape, const std::vec
     lin.stride[i] = st;
     st *= nv.shape[i];
   }
-  push_stage(d, stage, lin);
+  return push_stage(d, stage, lin);
 }
 
 // ===========================================================================
@@ -1400,6 +1457,8 @@ void NArray::set_element(const Coord &coord, const Scalar &v) {
 
 static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
                        const NArray *out_in);
+static bool map_into(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                     const std::shared_ptr<Storage> &dst, const ph_view_desc &od);
 
 // gather of an arbitrary operand (view / ProcView) into a new NArray
 static NArray materialize_operand(const Operand &o) {
@@ -1582,11 +1641,11 @@ std::vector<NArray> NArray::slices(int axis) const {
 
 // Reshape `src` to the destination's logical shape when the two are
 // compatible up to trailing ones (ShapeUtil.compatible_shapes?).
-static ph_view_desc fit_src_desc(const ph_view_desc &src, const Shape &dst_shape) {
+static std::optional<ph_view_desc> fit_src_desc(const ph_view_desc &src, const Shape &dst_shape) {
   ph_view_desc d = src;
   if (d.nstage > 0) {
     // re-express the (equal-size) source in the destination shape: reshape it
-    compose_reshape(d, dst_shape);
+    if (!compose_reshape(d, dst_shape)) return std::nullopt;
     return d;
   }
   // same leading extents; drop or add trailing unit axes
@@ -1674,28 +1733,6 @@ static std::optional<BoxForm> box_form(const ph_view_desc &d, const Shape &buf_s
   return f;
 }
 
-static bool ranges_overlap(const ph_view_desc &a, const ph_view_desc &b) {
-  auto range = [](const ph_view_desc &d, int64_t *lo, int64_t *hi) {
-    if (d.nstage > 0) {
-      *lo = 0;
-      *hi = d.extent - 1;
-      return;
-    }
-    int64_t l = d.view.offset, h = d.view.offset;
-    for (int i = 0; i < d.view.rank; ++i) {
-      int64_t span = (d.view.shape[i] - 1) * d.view.stride[i];
-      if (span < 0) l += span;
-      else h += span;
-    }
-    *lo = l;
-    *hi = h;
-  };
-  int64_t al, ah, bl, bh;
-  range(a, &al, &ah);
-  range(b, &bl, &bh);
-  return !(ah < bl || bh < al);
-}
-
 // dst (storage + view descriptor over buf_shape) <- src, elementwise over the
 // destination's logical shape.
 static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc &dd, const Shape &buf_shape,
@@ -1705,29 +1742,29 @@ static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc
   if (n == 0) return;
   Shape dshape(dd.view.shape, dd.view.shape + dd.view.rank);
   Operand src = src_in;
+  std::optional<ph_view_desc> fd = fit_src_desc(src.desc, dshape);
+  if (!fd) {  // the source reshaped to dst's shape needs one stage more: gather it first
+    NArray tmp = materialize_operand(src);
+    src = tmp.operand();
+    fd = fit_src_desc(src.desc, dshape);
+  }
   if (src.body || compute_dtype((DType)src.desc.dtype) != compute_dtype((DType)dd.dtype) ||
       dtype_size((DType)src.desc.dtype) != dtype_size((DType)dd.dtype)) {
-    // a ProcView (or a different element type): evaluate into the target type
+    // a ProcView (or a different element type): evaluated element by element
+    // into the destination, in the reference's order where it reads it
     DType target = (DType)dd.dtype;
     if (src.value_dtype() != compute_dtype(target))
       throw ArgumentError(std::string("cannot store ") + dtype_name(src.value_dtype()) + " values into a " +
                           dtype_name(target) + " array");
-    NArray tmp = materialize_operand(src);
+    Operand fitted = src;
+    fitted.desc = *fd;
+    fitted.shape = dshape;
+    if (map_into({fitted}, [](const std::vector<Expr> &x) { return x[0]; }, dst, dd)) return;
+    NArray tmp = materialize_operand(src);  // BF16 storage: rounded on the way
     src = tmp.operand();
-    src.storage = tmp.storage;
+    fd = fit_src_desc(src.desc, dshape);
   }
-  if (src.storage == dst) {
-    // same buffer: the reference reads each source element after the writes
-    // that precede it in lexicographic order; copy the source first unless
-    // the two address ranges are disjoint
-    ph_view_desc sfit = fit_src_desc(src.desc, dshape);
-    if (ranges_overlap(sfit, dd)) {
-      void *p = dst->dev_write();
-      check(ph_strided_copy_sequential(p, &dd, p, &sfit), "scatter (aliased)");
-      return;
-    }
-  }
-  ph_view_desc sd = fit_src_desc(src.desc, dshape);
+  const ph_view_desc sd = *fd;
   const void *sp = src.storage->dev_read();
   if (dst->pending_fill && src.storage != dst && dst->host_clean()) {
     if (auto f = box_form(dd, buf_shape)) {
@@ -1735,6 +1772,7 @@ static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc
       // is written, so the (clean) host mirror is dropped
       unsigned char bits[8];
       scalar_bits(*dst->pending_fill, dst->dtype, bits);
+      dst->before_write();
       check(ph_fill_scatter(dst->dptr, (int32_t)dst->dtype, buf_shape.data(), (int32_t)buf_shape.size(),
                             f->first.data(), f->step.data(), f->count.data(), f->axis.data(), sp, &sd, bits),
             "fill + scatter");
@@ -1789,28 +1827,67 @@ void NArray::set_mask(const Operand &mask, const Operand &src) {
                                             " does not match array shape " + shape_to_s(shape_) + ".");
   if (mask.value_dtype() != DType::Bool) throw ArgumentError("mask must be Bool");
   Operand self = operand();
-  const ph_view_desc own = desc();
-  if (src.storage == storage && !src.body && std::memcmp(&src.desc, &own, sizeof(own)) != 0 && size() > 0) {
-    // the source is another view of this array: the reference's loop
-    // (n_array.cr:523-536) reads each source element after the writes that
-    // precede it, which a parallel map does not; run it in order
-    NArray m;
-    Operand mo = mask;
-    if (mask.body || (DType)mask.desc.dtype != DType::Bool) {
-      m = materialize_operand(mask);
-      mo = m.operand();
-    }
-    void *p = storage->dev_write();
-    check(ph_masked_copy_sequential(p, &own, p, &src.desc, mo.storage->dev_read(), &mo.desc), "mask (aliased)");
-    return;
-  }
   map_core({mask, src, self}, [&](const std::vector<Expr> &x) { return Expr::select(x[0], x[1], x[2]); }, this);
 }
 
 // ===========================================================================
 // View / MutableView / ProcView
 // ===========================================================================
+// Folds transform `t` into the descriptor, or keeps it in the tail once the
+// descriptor has no room for its stage.
+void View::push(const Transform &t, const Shape &new_shape, const std::function<bool(ph_view_desc &)> &compose) {
+  if (tail_ == 0) {
+    ph_view_desc d = desc_;
+    if (compose(d)) {
+      desc_ = d;
+    } else {
+      mid_shape_ = shape_;
+      tail_ = 1;
+    }
+  } else {
+    ++tail_;
+  }
+  transforms.insert(transforms.begin(), t);
+  shape_ = new_shape;
+}
+
+static void apply_transform(View &v, const Transform &t) {
+  switch (t.kind) {
+    case Transform::Region: v.restrict_to(t.region); break;
+    case Transform::Reshape: v.reshape_(t.b); break;
+    case Transform::Permute: v.permute_(&t.a); break;
+    case Transform::Reverse: v.reverse_(); break;
+    case Transform::Tile: v.tile_(t.b); break;
+  }
+}
+
+// The tail's transforms, oldest first, on `on` (a view of the intermediate).
+void View::replay_tail(View &on) const {
+  for (size_t i = tail_; i-- > 0;) apply_transform(on, transforms[i]);
+}
+
+int64_t View::buffer_index(const Coord &c) const {
+  if (tail_ == 0) return desc_index(desc_, c);
+  const Coord b = apply(c);  // the whole chain, to a coordinate of the buffer's shape
+  int64_t idx = 0, st = 1;
+  for (size_t i = b.size(); i-- > 0;) {
+    idx += b[i] * st;
+    st *= src_shape[i];
+  }
+  return idx;
+}
+
 Operand View::operand() const {
+  if (tail_ > 0) {
+    // gathered now, when the view is used, so it reads the source as it is
+    Operand mid;
+    mid.storage = storage;
+    mid.desc = desc_;
+    mid.shape = mid_shape_;
+    View v = materialize_operand(mid).view();
+    replay_tail(v);
+    return v.operand();
+  }
   Operand o;
   o.storage = storage;
   o.desc = desc_;
@@ -1819,12 +1896,13 @@ Operand View::operand() const {
 }
 
 View &View::restrict_to(const IndexRegion &region) {
-  shape_ = region.shape();
-  compose_region(desc_, region);
   Transform t;
   t.kind = Transform::Region;
   t.region = region;
-  transforms.insert(transforms.begin(), t);
+  push(t, region.shape(), [&](ph_view_desc &d) {
+    compose_region(d, region);
+    return true;
+  });
   return *this;
 }
 
@@ -1846,13 +1924,11 @@ View &View::reshape_(const Shape &ns) {
                      " elements) to " + shape_to_s(ns) + " (" + std::to_string(ShapeUtil::shape_to_size(ns)) +
                      " elements) because reshape cannot add or remove elements.");
   if (ns.size() > PH_MAX_RANK) throw DimensionError("at most 8 dimensions");
-  compose_reshape(desc_, ns);
   Transform t;
   t.kind = Transform::Reshape;
   t.a = shape_;
   t.b = ns;
-  transforms.insert(transforms.begin(), t);
-  shape_ = ns;
+  push(t, ns, [&](ph_view_desc &d) { return compose_reshape(d, ns); });
   return *this;
 }
 
@@ -1875,21 +1951,24 @@ View &View::permute_(const std::vector<int64_t> *order) {
   }
   Shape ns(pattern.size());
   for (size_t i = 0; i < pattern.size(); ++i) ns[i] = shape_[(size_t)pattern[i]];
-  compose_permute(desc_, pattern);
   Transform t;
   t.kind = Transform::Permute;
   t.a = pattern;
-  transforms.insert(transforms.begin(), t);
-  shape_ = ns;
+  push(t, ns, [&](ph_view_desc &d) {
+    compose_permute(d, pattern);
+    return true;
+  });
   return *this;
 }
 
 View &View::reverse_() {
-  compose_reverse(desc_);
   Transform t;
   t.kind = Transform::Reverse;
   t.a = shape_;
-  transforms.insert(transforms.begin(), t);
+  push(t, shape_, [](ph_view_desc &d) {
+    compose_reverse(d);
+    return true;
+  });
   return *this;
 }
 
@@ -1901,13 +1980,12 @@ View &View::tile_(const std::vector<int64_t> &counts) {
     if (c < 0) throw ArgumentError("negative tile count");
   Shape ns(shape_.size());
   for (size_t i = 0; i < ns.size(); ++i) ns[i] = shape_[i] * counts[i];
-  compose_tile(desc_, shape_, counts);
   Transform t;
   t.kind = Transform::Tile;
   t.a = shape_;
   t.b = counts;
-  transforms.insert(transforms.begin(), t);
-  shape_ = ns;
+  const Shape cur = shape_;
+  push(t, ns, [&](ph_view_desc &d) { return compose_tile(d, cur, counts); });
   return *this;
 }
 
@@ -1979,7 +2057,7 @@ Coord View::apply(const Coord &coord) const {
 
 Scalar View::get(const Coord &coord) const {
   Coord c = CoordUtil::canonicalize_coord(coord, shape_);
-  return fetch_one(storage, desc_index(desc_, c));
+  return fetch_one(storage, buffer_index(c));
 }
 
 MutableView MutableView::mutable_view(const RegionLiteral &lit) const {
@@ -1988,7 +2066,27 @@ MutableView MutableView::mutable_view(const RegionLiteral &lit) const {
   return m;
 }
 
+// A write through a chain longer than one descriptor: the view the
+// descriptor covers is gathered into a temporary, the tail of the chain is
+// written through on the temporary, and the temporary goes back through the
+// descriptor (its unwritten elements rewrite the values they hold).
+void MutableView::deep_write(const std::function<void(MutableView &)> &fn) {
+  Operand mid;
+  mid.storage = storage;
+  mid.desc = desc_;
+  mid.shape = mid_shape_;
+  NArray m = materialize_operand(mid);
+  MutableView t = m.mutable_view();
+  replay_tail(t);
+  fn(t);
+  scatter_into(storage, desc_, src_shape, m.operand());
+}
+
 void MutableView::unsafe_set_chunk(const IndexRegion &region, const Operand &src) {
+  if (deep()) {
+    deep_write([&](MutableView &t) { t.unsafe_set_chunk(region, src); });
+    return;
+  }
   View v = *this;
   v.restrict_to(region);
   scatter_into(storage, v.desc_, src_shape, src);
@@ -2003,6 +2101,10 @@ void MutableView::set_chunk(const RegionLiteral &lit, const Operand &src) {
 }
 
 void MutableView::set_chunk(const RegionLiteral &lit, const Scalar &value) {
+  if (deep()) {
+    deep_write([&](MutableView &t) { t.set_chunk(lit, value); });
+    return;
+  }
   View v = *this;
   v.restrict_to(IndexRegion::from_literal(lit, shape_));
   if (v.size() == 0) return;
@@ -2013,7 +2115,7 @@ void MutableView::set_chunk(const RegionLiteral &lit, const Scalar &value) {
 
 void MutableView::set_element(const Coord &coord, const Scalar &v) {
   Coord c = CoordUtil::canonicalize_coord(coord, shape_);
-  store_one(storage, desc_index(desc_, c), v);
+  store_one(storage, buffer_index(c), v);
 }
 
 Operand ProcView::operand() const {
@@ -2048,54 +2150,141 @@ Scalar ProcView::get(const Coord &coord) const {
 // ===========================================================================
 // Fused map over several operands of one shape, into `out` (a contiguous
 // NArray of the result type), or into a new NArray when out is null.
-static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
-                       const NArray *out_in) {
+namespace {
+// A fused map's kernel inputs (distinct storage and descriptor pairs) and
+// its body over them.
+struct MapPlan {
+  std::vector<Operand> inputs;
+  Expr body;
+};
+
+// Whether operand `o` reads the whole pending result of its storage (an
+// NArray left pending by an operator), so its map can be spliced in.
+bool whole_pending(const Operand &o) {
+  return o.storage->pending_map && std::memcmp(&o.desc, &o.storage->pending_map->out, sizeof(ph_view_desc)) == 0;
+}
+
+// `fuse`: pending operator results are spliced into the program instead of
+// being run first, except those that read `dst` (the buffer the map writes):
+// the reference computed them before the write, so they stay a snapshot.
+MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                 bool fuse, const Storage *dst) {
   if (ops.empty()) throw ArgumentError("map_with needs an operand");
   for (size_t i = 1; i < ops.size(); ++i)
     if (ops[i].shape != ops[0].shape)
       throw ShapeError("Could not simultaneously map MultiIndexables with shapes " + shape_to_s(ops[i].shape) +
                        " and " + shape_to_s(ops[0].shape) + ".");
-  // kernel inputs: distinct (storage, descriptor) pairs
-  std::vector<const Operand *> inputs;
-  std::vector<Expr> leaf;
-  for (const Operand &o : ops) {
+  MapPlan p;
+  std::function<Expr(const Operand &)> leaf = [&](const Operand &o) -> Expr {
+    if (fuse && whole_pending(o)) {
+      const PendingMap &pm = *o.storage->pending_map;
+      bool reads_dst = false;
+      for (const Operand &i : pm.inputs) reads_dst = reads_dst || i.storage.get() == dst;
+      if (!reads_dst) {
+        std::vector<Expr> subs;
+        for (const Operand &i : pm.inputs) subs.push_back(leaf(i));
+        Expr e = pm.body.substitute(subs);
+        return o.body ? o.body->substitute({e}) : e;
+      }
+    }
     int idx = -1;
-    for (size_t k = 0; k < inputs.size(); ++k)
-      if (inputs[k]->storage == o.storage && std::memcmp(&inputs[k]->desc, &o.desc, sizeof(ph_view_desc)) == 0)
+    for (size_t k = 0; k < p.inputs.size(); ++k)
+      if (p.inputs[k].storage == o.storage && std::memcmp(&p.inputs[k].desc, &o.desc, sizeof(ph_view_desc)) == 0)
         idx = (int)k;
     if (idx < 0) {
-      if (inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
-      inputs.push_back(&o);
-      idx = (int)inputs.size() - 1;
+      if (p.inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
+      Operand in = o;
+      in.body.reset();
+      p.inputs.push_back(in);
+      idx = (int)p.inputs.size() - 1;
     }
     Expr e = Expr::input(idx, (DType)o.desc.dtype);
-    if (o.body) e = o.body->substitute({e});
-    leaf.push_back(e);
+    return o.body ? o.body->substitute({e}) : e;
+  };
+  std::vector<Expr> leaves;
+  for (const Operand &o : ops) leaves.push_back(leaf(o));
+  p.body = fn(leaves);
+  return p;
+}
+
+// plan_map fusing pending operands where the fused program stays within a
+// map's limits (inputs and instructions); otherwise they run first.
+MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                 const Storage *dst) {
+  bool any = false;
+  for (const Operand &o : ops) any = any || whole_pending(o);
+  if (any) {
+    try {
+      MapPlan p = plan_map(ops, fn, true, dst);
+      (void)p.body.compile((int)p.inputs.size());
+      return p;
+    } catch (const ArgumentError &) {
+      // too large to fuse: the pending operands run on their own
+    }
   }
-  Expr body = fn(leaf);
+  return plan_map(ops, fn, false, dst);
+}
+
+// Runs the plan into the view `od` of `dst`.  Inputs that read dst's
+// buffer see the writes of earlier elements, as the reference's loops do.
+void launch_map(const MapPlan &p, const std::shared_ptr<Storage> &dst, const ph_view_desc &od) {
+  ph_prog prog = p.body.compile((int)p.inputs.size());
+  std::vector<const void *> ptrs;
+  std::vector<ph_view_desc> descs;
+  for (const Operand &o : p.inputs) {
+    ptrs.push_back(o.storage->dev_read());
+    descs.push_back(o.desc);
+  }
+  void *op = dst->dev_write();
+  int32_t flags = 0;
+  check(ph_map(&prog, (int32_t)p.inputs.size(), ptrs.data(), descs.data(), op, &od, &flags), "map");
+  check_flags(flags, "elementwise");
+}
+}  // namespace
+
+// Fused map over several operands of one shape, into `out` (a contiguous
+// NArray of the result type), or into a new NArray when out is null.  A new
+// result whose program cannot raise is left pending (Storage::pending_map),
+// so that the operator that uses it next fuses it into its own kernel.
+static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                       const NArray *out_in) {
+  MapPlan p = plan_map(ops, fn, out_in ? out_in->storage.get() : nullptr);
   NArray out;
   if (out_in) {
     out = *out_in;
-    if (compute_dtype(out.dtype()) != body.dtype() || out.dtype() == DType::BF16)
-      throw ArgumentError(std::string("cannot store ") + dtype_name(body.dtype()) + " values into a " +
+    if (compute_dtype(out.dtype()) != p.body.dtype() || out.dtype() == DType::BF16)
+      throw ArgumentError(std::string("cannot store ") + dtype_name(p.body.dtype()) + " values into a " +
                           dtype_name(out.dtype()) + " array");
   } else {
-    out = NArray::empty(ops[0].shape, body.dtype());
+    out = NArray::empty(ops[0].shape, p.body.dtype());
   }
   if (out.size() == 0) return out;
-  ph_prog prog = body.compile((int)inputs.size());
-  std::vector<const void *> ptrs;
-  std::vector<ph_view_desc> descs;
-  for (const Operand *o : inputs) {
-    ptrs.push_back(o->storage->dev_read());
-    descs.push_back(o->desc);
+  if (!out_in && !prog_may_raise(p.body.compile((int)p.inputs.size()))) {
+    auto pm = std::make_shared<PendingMap>();
+    pm->inputs = p.inputs;
+    pm->body = p.body;
+    pm->out = out.desc();
+    for (const Operand &i : pm->inputs) {
+      auto &rd = i.storage->readers;
+      rd.erase(std::remove_if(rd.begin(), rd.end(), [](const std::weak_ptr<Storage> &w) { return w.expired(); }),
+               rd.end());
+      rd.push_back(out.storage);
+    }
+    out.storage->pending_map = pm;
+    return out;
   }
-  void *op = out.storage->dev_write();
-  ph_view_desc od = out.desc();
-  int32_t flags = 0;
-  check(ph_map(&prog, (int32_t)inputs.size(), ptrs.data(), descs.data(), op, &od, &flags), "map");
-  check_flags(flags, "elementwise");
+  launch_map(p, out.storage, out.desc());
   return out;
+}
+
+// The same into the view `od` of `dst` (a scatter of a lazily evaluated
+// source); false when the map cannot store dst's element type.
+static bool map_into(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
+                     const std::shared_ptr<Storage> &dst, const ph_view_desc &od) {
+  MapPlan p = plan_map(ops, fn, dst.get());
+  if (dst->dtype == DType::BF16 || compute_dtype(dst->dtype) != p.body.dtype()) return false;
+  launch_map(p, dst, od);
+  return true;
 }
 
 NArray map_with(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn) {

@@ -217,12 +217,27 @@ class Expr {
 // device next uses the storage.  A device use that may write the storage
 // drops the mirror.  A pending fill is applied to host pages without any
 // copy, and issued on the device at its first use there.
-struct Storage {
+struct PendingMap;
+
+struct Storage : std::enable_shared_from_this<Storage> {
   static constexpr size_t kPage = size_t(1) << 16;
   size_t bytes = 0;
   int64_t elements = 0;
   DType dtype = DType::F64;
   std::optional<Scalar> pending_fill;  // fill not yet issued on the device
+  // A fused elementwise map not yet run, whose result is this buffer.
+  // Operator results are left pending so that the next operator fuses them
+  // into its own kernel: `(a + b.tile(n, 1)) * c` runs as one map.  The map
+  // runs at the first use that does not fuse it.  Every storage it reads
+  // lists it among its `readers`, and a write to one of them runs it first,
+  // so the result is what the operator would have computed at once.  Only
+  // programs that cannot raise (no checked Int32 arithmetic) are left
+  // pending, so no error is raised later than the operator.
+  std::shared_ptr<PendingMap> pending_map;
+  std::vector<std::weak_ptr<Storage>> readers;
+  void run_pending_map();
+  // Runs the pending maps that read this buffer, before it is written.
+  void before_write();
   ~Storage();
   void *dptr = nullptr;  // device memory: reach it through dev_read / dev_write
   // The device copy, current: the pending fill issued, dirty pages written back.
@@ -273,6 +288,14 @@ struct Operand {
   Shape shape;
   std::optional<Expr> body;
   DType value_dtype() const;
+};
+
+// A pending map: its (not pending) inputs, its body over them, and the
+// contiguous descriptor of its result.
+struct PendingMap {
+  std::vector<Operand> inputs;
+  Expr body;
+  ph_view_desc out;
 };
 
 // Device-resident NArray (src/n_array.cr): row-major buffer + shape.
@@ -356,7 +379,14 @@ class View {
   Shape src_shape;
   Shape shape_;
   std::vector<Transform> transforms;  // newest first (compose! unshifts)
+  // desc_ folds the oldest transforms, up to the descriptor's room for
+  // reshape/tile stages; the newest tail_ transforms (ComposedTransform has
+  // no bound, src/view_util/transforms.cr:28-101) are applied, when the view
+  // is used, to a gathered copy of the view desc_ describes (of shape
+  // mid_shape_).
   ph_view_desc desc_;
+  size_t tail_ = 0;
+  Shape mid_shape_;
 
   const Shape &shape() const { return shape_; }
   int dimensions() const { return (int)shape_.size(); }
@@ -379,6 +409,12 @@ class View {
   ProcView process(const Expr &body) const;      // (:119-121)
   Coord apply(const Coord &coord) const;         // ComposedTransform#apply
   Scalar get(const Coord &coord) const;
+  bool deep() const { return tail_ > 0; }
+
+ protected:
+  void push(const Transform &t, const Shape &new_shape, const std::function<bool(ph_view_desc &)> &compose);
+  void replay_tail(View &on) const;
+  int64_t buffer_index(const Coord &coord) const;
 };
 
 class MutableView : public View {
@@ -388,6 +424,9 @@ class MutableView : public View {
   void set_chunk(const RegionLiteral &lit, const Scalar &value);
   void unsafe_set_chunk(const IndexRegion &region, const Operand &src);
   void set_element(const Coord &coord, const Scalar &v);
+
+ private:
+  void deep_write(const std::function<void(MutableView &)> &fn);
 };
 
 // ProcView (src/proc_view.cr): a view plus one element block.

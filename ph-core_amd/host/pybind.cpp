@@ -272,6 +272,10 @@ PYBIND11_MODULE(_phase, m) {
   m.def("init", &phase::init, py::arg("device") = 0);
   m.def("device_available", &phase::device_available);
   m.def("jit_compiles", []() { return ph_jit_compiles(); });
+  m.def("launch_count", []() { return ph_launch_count(); }, "kernel launches issued so far");
+  m.def("sync", []() {
+    if (ph_sync() != 0) throw std::runtime_error(ph_last_error());
+  }, "waits for the library stream");
 
   // --- range syntax / utils
   py::class_<RangeLiteral>(m, "RangeLiteral")
@@ -450,6 +454,10 @@ PYBIND11_MODULE(_phase, m) {
       .def("is_scalar", &NArray::scalar_)
       .def("to_numpy", &to_numpy)
       .def("get", [](const NArray &a, py::args c) { return from_scalar(a.get(args_ints(c))); })
+      .def("materialize", [](NArray &a) -> NArray & {
+        a.storage->dev_read();  // runs a pending operator result (or fill) now
+        return a;
+      }, "issues the array's pending work on the device (an operator result left pending, a fill)")
       .def("set_element", [](NArray &a, const Coord &c, py::object v) { a.set_element(c, to_scalar(v)); })
       .def("__getitem__", [](const NArray &a, py::object key) { return a.get_chunk(to_region(key)); })
       .def("get_chunk", [](const NArray &a, py::object lit, bool drop) { return a.get_chunk(to_region(lit), drop); },
@@ -574,6 +582,7 @@ PYBIND11_MODULE(_phase, m) {
       .def("descriptor", [](const View &v) {
         const ph_view_desc &d = v.desc();
         py::dict r;
+        r["tail"] = v.tail_;  // transforms past the descriptor's stages
         r["rank"] = d.view.rank;
         r["offset"] = d.view.offset;
         r["shape"] = std::vector<int64_t>(d.view.shape, d.view.shape + d.view.rank);

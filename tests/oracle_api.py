@@ -111,6 +111,8 @@ def lib():
         L.or_set_chunk_lazy.restype = C.c_int64
         L.or_set_mask_lazy.argtypes = [C.c_void_p, C.c_int, i64p, C.c_int, C.c_char_p, C.c_char_p, C.c_void_p,
                                        C.c_int, C.POINTER(Input), C.POINTER(C.c_int)]
+        L.or_sum_seq_f64_from.argtypes = [C.c_void_p, C.c_int64, C.c_double]
+        L.or_sum_seq_f64_from.restype = C.c_double
         L.or_sum_seq_f64.argtypes = [C.c_void_p, C.c_int64]
         L.or_sum_seq_f64.restype = C.c_double
         L.or_concatenate.argtypes = [C.c_int, C.POINTER(C.c_void_p), i64p, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -440,6 +442,12 @@ def eval_expr(expr, inputs, bf16=()):
 def sum_seq(x):
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
     return lib().or_sum_seq_f64(_ptr(x), x.size)
+
+
+def sum_seq_from(x, start):
+    """The sequential Float64 sum continued from the running total `start`."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    return lib().or_sum_seq_f64_from(_ptr(x), x.size, float(start))
 
 
 def max_seq(x):

@@ -55,7 +55,7 @@ def test_reductions_ragged_shapes(ph, storage):
             assert s == int(a.astype(np.int64).sum())
         else:
             ex = O.sum_exact(h64)
-            assert abs(s - ex) <= 1e-11 * np.abs(h64).sum() + 1e-30, (shape, s, ex)
+            assert abs(s - ex) <= 1e-12 * np.abs(h64).sum() + 1e-30, (shape, s, ex)
         assert float(d.max()) == float(h64.max()), shape
         # slices(k).map(&.sum) / (&.max) along every axis
         for ax in range(len(shape)):
@@ -68,7 +68,7 @@ def test_reductions_ragged_shapes(ph, storage):
             else:
                 want = np.array([O.sum_exact(np.take(h64, j, axis=ax)) for j in range(shape[ax])])
                 scale = np.array([np.abs(np.take(h64, j, axis=ax)).sum() for j in range(shape[ax])])
-                assert np.all(np.abs(ss - want) <= 1e-11 * scale + 1e-30), (shape, ax)
+                assert np.all(np.abs(ss - want) <= 1e-12 * scale + 1e-30), (shape, ax)
 
 
 def test_int32_running_sum_overflow(ph):
@@ -243,4 +243,27 @@ def test_heat_unfused_when_4u_may_overflow(ph, dt):
         ref, _ = O.heat2d(u0, r, 34, every=0)
         got = u.to_numpy()
         assert np.array_equal(got, ref, equal_nan=True), r
-        assert np.array_equal(np.signbit(got), np.signbit(ref)), r
+        # every number bit for bit, zeros' signs included; a NaN's sign is
+        # not fixed by IEEE 754 arithmetic (and the Float32 form's fused
+        # multiply-add propagates NaN operands differently from the host's)
+        num = ~np.isnan(ref)
+        assert np.array_equal(got[num].view(np.uint64 if dt == np.float64 else np.uint32),
+                              ref[num].view(np.uint64 if dt == np.float64 else np.uint32)), r
+
+
+def test_heat_fp32_twelve_step_passes(ph):
+    """Float32 runs of passes of twelve steps (three halo lanes): the fused
+    first operation (runs of 32 steps or more), the flush sweeps (rows in
+    whole vectors, a wavefront wide or more) and the masked sweep (ragged or
+    narrow grids), bit-exact against the oracle's Float32 formula (u + d*r
+    rounded once)."""
+    rng = np.random.default_rng(79)
+    for ny, nx in [(65, 257), (200, 249), (130, 5), (300, 1000), (1000, 2048), (97, 260), (40, 256)]:
+        for steps, every in [(36, 0), (48, 24), (37, 0)]:
+            u0 = (rng.standard_normal((ny, nx)) * 10).astype(np.float32)
+            u = ph.NArray.from_numpy(u0)
+            e = ph.heat2d(u, 0.2, steps, every=every)
+            ref, eref = O.heat2d(u0, 0.2, steps, every=every)
+            assert np.array_equal(u.to_numpy(), ref), (ny, nx, steps, every)
+            if every:
+                assert np.allclose(e, eref, rtol=1e-13, atol=1e-300), (ny, nx, steps)

@@ -248,6 +248,24 @@ def test_tiled_map_programs_compile_for_gfx950(host):
         host.compile_check(host.input(0, host.DType.I32) + host.element_index(), [host.DType.I32], 64, 64, 2)
 
 
+def test_ordered_map_programs_compile_for_gfx950(host):
+    """The kernels that run a map or a copy in the reference's element order
+    when an input reads the output's own buffer (classify, Jacobi pass,
+    write-out, sequential loop; for copies the chain links, pointer jumping
+    and gather) are generated and compiled (no launch, no device)."""
+    for dt, fn in BLOCKS[:3] + BLOCKS[8:10]:
+        d = getattr(host.DType, DT[dt][0])
+        e = fn(host.placeholder(d))
+        if e.dtype != d:
+            continue
+        host.compile_check(e, [d], 48, 48, 3)
+    m = host.input(1, host.DType.Bool)
+    sel = host.select(m, host.input(0, host.DType.I32) * 2, host.input(2, host.DType.I32))
+    host.compile_check(sel, [host.DType.I32, host.DType.Bool, host.DType.I32], 40, 40, 3)
+    for d in (host.DType.I32, host.DType.F64, host.DType.BF16, host.DType.Bool):
+        host.compile_check(host.placeholder(d), [d], 30, 70, 4)
+
+
 def test_element_index_programs(host):
     """A block that reads the element's lexicographic index (map_with_index,
     n_array.cr:605-611) compiles into the map kernel in its row-tiled and its

@@ -604,3 +604,39 @@ def test_concatenate_vs_numpy():
             s[axis] = int(rng.integers(0, 5))
             arrs.append(rng.integers(0, 200, s).astype(dt))
         assert np.array_equal(O.concatenate(arrs, axis), np.concatenate(arrs, axis=axis))
+
+
+def test_max_follows_enumerable_greater_than():
+    """Enumerable#max keeps the first element and replaces it only by a
+    greater one (`>`): a leading NaN is kept (nothing compares
+    greater than it), a later NaN is skipped, and of equal zeros the first
+    one's sign survives.  Known answers from that definition."""
+    nan = float("nan")
+    assert math.isnan(O.max_seq(np.array([nan, 1.0, 5.0])))
+    assert O.max_seq(np.array([1.0, nan, 2.0])) == 2.0
+    assert O.max_seq(np.array([3.0, nan])) == 3.0
+    assert math.copysign(1, O.max_seq(np.array([-0.0, 0.0]))) == -1
+    assert math.copysign(1, O.max_seq(np.array([0.0, -0.0]))) == 1
+    assert math.copysign(1, O.max_seq(np.array([-1.0, -0.0, 0.0, -2.0]))) == -1
+    assert math.copysign(1, O.max_seq(np.array([-3.0, 0.0, -0.0]))) == 1
+    s = O.slices_reduce(np.array([[nan, 2.0], [-0.0, 0.0]]), 0, 1)
+    assert math.isnan(s[0]) and math.copysign(1, s[1]) == -1
+    s = O.slices_reduce(np.array([[nan, 2.0], [-0.0, 0.0]]), 1, 1)
+    assert math.isnan(s[0]) and s[1] == 2.0
+
+
+def test_sequential_sum_in_pieces():
+    """A sequential sum fed in pieces equals the one-shot sequential sum (the
+    same additions in the same order), and differs from a pairwise sum on
+    data where the order shows (absorption of small terms)."""
+    rng = np.random.default_rng(12)
+    x = np.concatenate([[1e16], rng.random(1000), [-1e16], rng.random(1000)])
+    s = 0.0
+    for k in range(0, x.size, 137):
+        s = O.sum_seq_from(x[k:k + 137], s)
+    assert s == O.sum_seq(x)
+    acc = 0.0
+    for v in x:
+        acc += v
+    assert s == acc
+    assert s != math.fsum(x)
