@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--heat-steps", type=int, default=2000)
     ap.add_argument("--no-unbatched", action="store_true",
                     help="skip timing the batched workload as per-array launches (60000 kernels)")
+    ap.add_argument("--only", type=lambda v: set(v.split(",")), default=None,
+                    help="run only these extra workloads (c1,copy,c3,c4,c5,concat,json,path), for profiling")
     return ap.parse_args()
 
 
@@ -397,180 +399,201 @@ def other_workloads(phase, ev, args):
     import numpy as np
     D = phase.DType
     out = {}
-    # C1 broadcast at the roofline shape (16384^2 FP64; 1024^2 is the parity case)
-    n = 16384
-    a = phase.NArray.random([n, n], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
-    b = phase.NArray.random([1, n], D.F64, seed=0, offset=n * n, a=-1.0, b=1.0)
-    c = phase.NArray.random([n, n], D.F64, seed=0, offset=n * n + n, a=-1.0, b=1.0)
-    bt = b.tile(n, 1)
-    by = 24 * n * n + 8 * n
-    # the operator form as ph-core code writes it: a + b.tile is left
-    # pending and fused into the `* c` kernel when the result is used
-    fo = lambda: ((a + bt) * c).materialize()  # noqa: E731
-    fo()
-    mso = time_loop(ev, fo, 10, phase)
-    # the same with the intermediate forced (two kernels and a temporary, as
-    # ph-core's eager operators evaluate it)
-    def f2():
-        t = (a + bt).materialize()
-        return (t * c).materialize()
-    f2()
-    ms2 = time_loop(ev, f2, 10, phase)
-    # map_with: one kernel, same per-element operations and roundings
-    fz = lambda: a.map_with(bt, c, block=lambda x, y, z: (x + y) * z).materialize()  # noqa: E731
-    fz()
-    msf = time_loop(ev, fz, 10, phase)
-    out["c1_broadcast_f64_16384"] = {
-        "operators": {"ms": round(mso, 4), "achieved_gbs": round(by / mso / 1e6, 1),
-                      "frac": round(by / mso / 1e6 / PEAK_HBM_GBS, 4),
-                      "note": "(a + b.tile) * c with ph-core's operators: the pending a + b.tile fuses into the "
-                              "product's kernel (one kernel)"},
-        "two_kernels": {"ms": round(ms2, 4), "achieved_gbs": round(by / ms2 / 1e6, 1),
-                        "note": "the same with the intermediate forced: two kernels and a temporary"},
-        "fused": {"ms": round(msf, 4), "achieved_gbs": round(by / msf / 1e6, 1),
-                  "frac": round(by / msf / 1e6 / PEAK_HBM_GBS, 4),
-                  "note": "a.map_with(b.tile, c) { (x + y) * z }: one kernel, stride-0 broadcast"},
-        "bytes": by}
-    del a, b, c, bt
-    tr = load_traffic() or {}
-    out["c1_broadcast_f64_16384"]["fused"]["traffic"] = tr.get("ph_map_fused_c1")
+    want = lambda name: not args.only or name in args.only  # noqa: E731
+    if want("c1"):
+        # C1 broadcast at the roofline shape (16384^2 FP64; 1024^2 is the parity case)
+        n = 16384
+        a = phase.NArray.random([n, n], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
+        b = phase.NArray.random([1, n], D.F64, seed=0, offset=n * n, a=-1.0, b=1.0)
+        c = phase.NArray.random([n, n], D.F64, seed=0, offset=n * n + n, a=-1.0, b=1.0)
+        bt = b.tile(n, 1)
+        by = 24 * n * n + 8 * n
+        # the operator form as ph-core code writes it: a + b.tile is left
+        # pending and fused into the `* c` kernel when the result is used
+        fo = lambda: ((a + bt) * c).materialize()  # noqa: E731
+        fo()
+        mso = time_loop(ev, fo, 10, phase)
+        # the same with the intermediate forced (two kernels and a temporary, as
+        # ph-core's eager operators evaluate it)
+        def f2():
+            t = (a + bt).materialize()
+            return (t * c).materialize()
+        f2()
+        ms2 = time_loop(ev, f2, 10, phase)
+        # map_with: one kernel, same per-element operations and roundings
+        fz = lambda: a.map_with(bt, c, block=lambda x, y, z: (x + y) * z).materialize()  # noqa: E731
+        fz()
+        msf = time_loop(ev, fz, 10, phase)
+        out["c1_broadcast_f64_16384"] = {
+            "operators": {"ms": round(mso, 4), "achieved_gbs": round(by / mso / 1e6, 1),
+                          "frac": round(by / mso / 1e6 / PEAK_HBM_GBS, 4),
+                          "note": "(a + b.tile) * c with ph-core's operators: the pending a + b.tile fuses into the "
+                                  "product's kernel (one kernel)"},
+            "two_kernels": {"ms": round(ms2, 4), "achieved_gbs": round(by / ms2 / 1e6, 1),
+                            "note": "the same with the intermediate forced: two kernels and a temporary"},
+            "fused": {"ms": round(msf, 4), "achieved_gbs": round(by / msf / 1e6, 1),
+                      "frac": round(by / msf / 1e6 / PEAK_HBM_GBS, 4),
+                      "note": "a.map_with(b.tile, c) { (x + y) * z }: one kernel, stride-0 broadcast"},
+            "bytes": by}
+        del a, b, c, bt
+        tr = load_traffic() or {}
+        out["c1_broadcast_f64_16384"]["fused"]["traffic"] = tr.get("ph_map_fused_c1")
 
-    # C1 at its parity shape, 1024^2 FP64 (25 MB: cache-resident, launch-bound)
-    m = 1024
-    a = phase.NArray.random([m, m], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
-    b = phase.NArray.random([1, m], D.F64, seed=0, offset=m * m, a=-1.0, b=1.0)
-    c = phase.NArray.random([m, m], D.F64, seed=0, offset=m * m + m, a=-1.0, b=1.0)
-    bt = b.tile(m, 1)
-    f1 = lambda: ((a + bt) * c).materialize()  # noqa: E731
-    f1()
-    ms1 = time_loop(ev, f1, 50, phase)
-    out["c1_broadcast_f64_1024"] = {"ms": round(ms1, 4), "bytes": 24 * m * m + 8 * m,
-                                    "achieved_gbs": round((24 * m * m + 8 * m) / ms1 / 1e6, 1),
-                                    "note": "(a + b.tile) * c with the operators (one fused kernel); inputs fit "
-                                            "the 256 MB MALL"}
-    if not args.no_cpu:
-        out["c1_broadcast_f64_1024"]["cpu"] = cpu_c1(a.to_numpy(), b.to_numpy(), c.to_numpy())
-    del a, b, c, bt
+    if want("c1"):
+        # C1 at its parity shape, 1024^2 FP64 (25 MB: cache-resident, launch-bound)
+        m = 1024
+        a = phase.NArray.random([m, m], D.F64, seed=0, offset=0, a=-1.0, b=1.0)
+        b = phase.NArray.random([1, m], D.F64, seed=0, offset=m * m, a=-1.0, b=1.0)
+        c = phase.NArray.random([m, m], D.F64, seed=0, offset=m * m + m, a=-1.0, b=1.0)
+        bt = b.tile(m, 1)
+        f1 = lambda: ((a + bt) * c).materialize()  # noqa: E731
+        f1()
+        ms1 = time_loop(ev, f1, 50, phase)
+        out["c1_broadcast_f64_1024"] = {"ms": round(ms1, 4), "bytes": 24 * m * m + 8 * m,
+                                        "achieved_gbs": round((24 * m * m + 8 * m) / ms1 / 1e6, 1),
+                                        "note": "(a + b.tile) * c with the operators (one fused kernel); inputs fit "
+                                                "the 256 MB MALL"}
+        if not args.no_cpu:
+            out["c1_broadcast_f64_1024"]["cpu"] = cpu_c1(a.to_numpy(), b.to_numpy(), c.to_numpy())
+        del a, b, c, bt
 
-    # the device's copy rate for a working set well beyond the MALL (2 GiB)
-    big = phase.NArray.random([1 << 29], D.I32, seed=9, dist="int31")
-    big.clone()
-    msb = time_loop(ev, lambda: big.clone(), 5, phase)
-    out["copy_2gib"] = {"kernel": "k_copy_rows (contiguous)", "bytes": 2 << 31, "ms": round(msb, 4),
-                        "achieved_gbs": round((2 << 31) / msb / 1e6, 1),
-                        "note": "practical HBM copy ceiling for multi-GB working sets"}
-    del big
+    if want("copy"):
+        # the device's copy rate for a working set well beyond the MALL (2 GiB)
+        big = phase.NArray.random([1 << 29], D.I32, seed=9, dist="int31")
+        big.clone()
+        msb = time_loop(ev, lambda: big.clone(), 5, phase)
+        out["copy_2gib"] = {"kernel": "k_copy_rows (contiguous)", "bytes": 2 << 31, "ms": round(msb, 4),
+                            "achieved_gbs": round((2 << 31) / msb / 1e6, 1),
+                            "note": "practical HBM copy ceiling for multi-GB working sets"}
+        # the runtime's own device-to-device copy of the same 2 GiB
+        # (hipMemcpyAsync through ph_d2d), independent of this build's kernels
+        dst = phase.NArray.empty([1 << 29], D.I32)
+        d2d = ev.lib.ph_d2d
+        d2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        pd, ps = dst.device_ptr, big.device_ptr
+        assert d2d(pd, ps, 1 << 31) == 0
+        msd = time_loop(ev, lambda: d2d(pd, ps, 1 << 31), 5, phase)
+        out["copy_2gib"]["hipmemcpy_d2d"] = {"ms": round(msd, 4), "achieved_gbs": round((2 << 31) / msd / 1e6, 1),
+                                             "note": "hipMemcpyAsync device to device, 2 GiB"}
+        del dst
+        del big
 
-    # C3 fused ProcView + six reductions (F32 and BF16 storage), one pass
-    n = 16384
-    for st, esz in [("F32", 4), ("BF16", 2)]:
-        x = phase.NArray.random([n, n], getattr(D, st), seed=2, dist="normal", a=0.0, b=1.0)
-        pv = x.process(lambda v: (v + 4) ** 2 * 0.5 - v)
-        pv.reduce2d_all_device()
-        ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10, phase)
-        by = esz * n * n
-        xh = x.to_numpy()
-        if not args.no_cpu and st == "F32":
-            out["c3_cpu"] = cpu_c3(xh)
-        out[f"c3_procview_reduce_{st.lower()}_16384"] = {
-            "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
-            "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
-            "traffic": tr.get(f"ph_red_tile_{st.lower()}"),
-            **({} if args.no_cpu else c3_errors(xh, st, pv.reduce2d_all())),
-            "note": "one pass: sum and max in full, along slices(0) and slices(1); FP64 sums inside each tile, compensated FP32 pairs across tiles"}
-        del xh
-        del x, pv
+    if want("c3"):
+        # C3 fused ProcView + six reductions (F32 and BF16 storage), one pass
+        n = 16384
+        for st, esz in [("F32", 4), ("BF16", 2)]:
+            x = phase.NArray.random([n, n], getattr(D, st), seed=2, dist="normal", a=0.0, b=1.0)
+            pv = x.process(lambda v: (v + 4) ** 2 * 0.5 - v)
+            pv.reduce2d_all_device()
+            ms = time_loop(ev, lambda: pv.reduce2d_all_device(), 10, phase)
+            by = esz * n * n
+            xh = x.to_numpy()
+            if not args.no_cpu and st == "F32":
+                out["c3_cpu"] = cpu_c3(xh)
+            out[f"c3_procview_reduce_{st.lower()}_16384"] = {
+                "ms": round(ms, 4), "achieved_gbs": round(by / ms / 1e6, 1),
+                "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4), "bytes": by,
+                "traffic": tr.get(f"ph_red_tile_{st.lower()}"),
+                **({} if args.no_cpu else c3_errors(xh, st, pv.reduce2d_all())),
+                "note": "one pass: sum and max in full, along slices(0) and slices(1); FP64 sums inside each tile, compensated FP32 pairs across tiles"}
+            del xh
+            del x, pv
 
-    # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64
-    n = 8192
-    steps = args.heat_steps
-    from phase.workloads import heat_initial_field
-    u0 = heat_initial_field(n, seed=3)
-    res = {}
-    fields = {}
-    # the same run with 1, 2 and 4 steps per pass for both precisions (one
-    # step per pass is the reference's loop structure; 1 and 2 are
-    # HBM-bound in both precisions), next to
-    # the default schedule (12 steps per pass for FP32, 8 for FP64, where
-    # both are limited by VALU: FP32 runs at FP64's rate without packed FP32
-    # math)
-    per_pass = {}
-    for kk in ("1", "2", "4"):
-        os.environ["PH_HEAT_K"] = kk
-        same = {}
+    if want("c4"):
+        # C4 heat 8192^2, r = 0.2, 16 Gaussian bumps (seed 3), FP32 and FP64
+        n = 8192
+        steps = args.heat_steps
+        from phase.workloads import heat_initial_field
+        u0 = heat_initial_field(n, seed=3)
+        res = {}
+        fields = {}
+        # the same run with 1, 2 and 4 steps per pass for both precisions (one
+        # step per pass is the reference's loop structure; 1 and 2 are
+        # HBM-bound in both precisions), next to
+        # the default schedule (12 steps per pass for FP32, 8 for FP64, where
+        # both are limited by VALU: FP32 runs at FP64's rate without packed FP32
+        # math)
+        per_pass = {}
+        for kk in ("1", "2", "4"):
+            os.environ["PH_HEAT_K"] = kk
+            same = {}
+            for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
+                u = phase.NArray.from_numpy(u0.astype(npdt))
+                ev.sync()
+                aa, bb = ev.new(), ev.new()
+                ev.record(aa)
+                phase.heat2d(u, 0.2, steps, every=100)
+                ev.record(bb)
+                same[dt] = round(n * n * steps / (ev.ms(aa, bb) * 1e-3) / 1e9, 2)
+                del u
+            per_pass[kk] = same
+            del os.environ["PH_HEAT_K"]
         for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
             u = phase.NArray.from_numpy(u0.astype(npdt))
             ev.sync()
             aa, bb = ev.new(), ev.new()
             ev.record(aa)
-            phase.heat2d(u, 0.2, steps, every=100)
+            e = phase.heat2d(u, 0.2, steps, every=100)
             ev.record(bb)
-            same[dt] = round(n * n * steps / (ev.ms(aa, bb) * 1e-3) / 1e9, 2)
-            del u
-        per_pass[kk] = same
-        del os.environ["PH_HEAT_K"]
-    for dt, npdt in [("F64", np.float64), ("F32", np.float32)]:
-        u = phase.NArray.from_numpy(u0.astype(npdt))
-        ev.sync()
-        aa, bb = ev.new(), ev.new()
-        ev.record(aa)
-        e = phase.heat2d(u, 0.2, steps, every=100)
-        ev.record(bb)
-        ms = ev.ms(aa, bb)
-        res[dt] = {"ms_total": round(ms, 2), "gcells": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
-                   "energy_first_last": [e[0], e[-1]]}
-        fields[dt] = u
-    # relative L2 of the FP32 run against the FP64 run
-    f32 = fields["F32"].to_numpy().astype(np.float64)
-    f64 = fields["F64"].to_numpy()
-    rel = float(np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)))
-    out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
-                           "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
-                           "four_steps_per_pass": {"f64_gcells": per_pass["4"]["F64"], "f32_gcells": per_pass["4"]["F32"],
-                                                   "f32_over_f64": round(per_pass["4"]["F32"] / per_pass["4"]["F64"], 3)},
-                           "two_steps_per_pass": {"f64_gcells": per_pass["2"]["F64"], "f32_gcells": per_pass["2"]["F32"],
-                                                  "f32_over_f64": round(per_pass["2"]["F32"] / per_pass["2"]["F64"], 3)},
-                           "one_step_per_pass": {"f64_gcells": per_pass["1"]["F64"], "f32_gcells": per_pass["1"]["F32"],
-                                                 "f32_over_f64": round(per_pass["1"]["F32"] / per_pass["1"]["F64"], 3),
-                                                 "frac_of_roofline_f64": round(per_pass["1"]["F64"] / 500.0, 3),
-                                                 "frac_of_roofline_f32": round(per_pass["1"]["F32"] / 1000.0, 3)},
-                           "rel_l2_f32_vs_f64": rel,
-                           "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0,
-                           "traffic_per_launch_of_four_steps": {"f64": tr.get("k_heat_multi_f64"),
-                                                                "f32": tr.get("k_heat_multi_f32")}}
-    del fields, u
-    if not args.no_cpu:
-        out["c4_heat_8192"]["cpu"] = cpu_c4(u0)
-    out["c5_batched_20000"] = batched_workload(phase, ev, args)
+            ms = ev.ms(aa, bb)
+            res[dt] = {"ms_total": round(ms, 2), "gcells": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
+                       "energy_first_last": [e[0], e[-1]]}
+            fields[dt] = u
+        # relative L2 of the FP32 run against the FP64 run
+        f32 = fields["F32"].to_numpy().astype(np.float64)
+        f64 = fields["F64"].to_numpy()
+        rel = float(np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)))
+        out["c4_heat_8192"] = {"steps": steps, "f64": res["F64"], "f32": res["F32"],
+                               "f32_over_f64": round(res["F32"]["gcells"] / res["F64"]["gcells"], 3),
+                               "four_steps_per_pass": {"f64_gcells": per_pass["4"]["F64"], "f32_gcells": per_pass["4"]["F32"],
+                                                       "f32_over_f64": round(per_pass["4"]["F32"] / per_pass["4"]["F64"], 3)},
+                               "two_steps_per_pass": {"f64_gcells": per_pass["2"]["F64"], "f32_gcells": per_pass["2"]["F32"],
+                                                      "f32_over_f64": round(per_pass["2"]["F32"] / per_pass["2"]["F64"], 3)},
+                               "one_step_per_pass": {"f64_gcells": per_pass["1"]["F64"], "f32_gcells": per_pass["1"]["F32"],
+                                                     "f32_over_f64": round(per_pass["1"]["F32"] / per_pass["1"]["F64"], 3),
+                                                     "frac_of_roofline_f64": round(per_pass["1"]["F64"] / 500.0, 3),
+                                                     "frac_of_roofline_f32": round(per_pass["1"]["F32"] / 1000.0, 3)},
+                               "rel_l2_f32_vs_f64": rel,
+                               "roofline_gcells_f64": 500.0, "roofline_gcells_f32": 1000.0,
+                               "traffic_per_launch_of_four_steps": {"f64": tr.get("k_heat_multi_f64"),
+                                                                    "f32": tr.get("k_heat_multi_f32")}}
+        del fields, u
+        if not args.no_cpu:
+            out["c4_heat_8192"]["cpu"] = cpu_c4(u0)
+    if want("c5"):
+        out["c5_batched_20000"] = batched_workload(phase, ev, args)
 
-    # NArray.concatenate of 4 x [4096, 4096] Float32 along axis 1 (one
-    # strided copy per input into its slab of the [4096, 16384] result)
-    parts = [phase.NArray.random([4096, 4096], D.F32, seed=40 + k) for k in range(4)]
-    cat = lambda: phase.NArray.concatenate_all(*parts, axis=1)  # noqa: E731
-    cat()
-    ms = time_loop(ev, cat, 10, phase)
-    by = 2 * 4 * 4096 * 4096 * 4
-    out["concatenate_f32_4x4096x4096_axis1"] = {"ms": round(ms, 4), "bytes": by,
-                                                "achieved_gbs": round(by / ms / 1e6, 1),
-                                                "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
-    if not args.no_cpu:
-        O = _oracle()
-        hp = [p.to_numpy() for p in parts]
-        out["concatenate_f32_4x4096x4096_axis1"]["cpu"] = _cpu_leg(
-            lambda: O.concatenate(hp, axis=1), by, "NArray.concatenate restated by the oracle", "full size")
-        del hp
-    del parts
-    # JSON / YAML text of a 1024 x 1024 Float64 array (device -> host -> text
-    # and back; host-bound)
-    a = phase.NArray.random([1024, 1024], D.F64, seed=41, a=-1.0, b=1.0)
-    t0 = time.perf_counter()
-    j = a.to_json()
-    t1 = time.perf_counter()
-    phase.NArray.from_json(j, D.F64)
-    t2 = time.perf_counter()
-    out["json_f64_1024x1024"] = {"to_json_ms": round((t1 - t0) * 1e3, 1), "from_json_ms": round((t2 - t1) * 1e3, 1),
-                                 "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing, 1 thread"}
-    out["path_ops_i32_8192"] = path_ops(phase, ev, args)
+    if want("concat"):
+        # NArray.concatenate of 4 x [4096, 4096] Float32 along axis 1 (one
+        # strided copy per input into its slab of the [4096, 16384] result)
+        parts = [phase.NArray.random([4096, 4096], D.F32, seed=40 + k) for k in range(4)]
+        cat = lambda: phase.NArray.concatenate_all(*parts, axis=1)  # noqa: E731
+        cat()
+        ms = time_loop(ev, cat, 10, phase)
+        by = 2 * 4 * 4096 * 4096 * 4
+        out["concatenate_f32_4x4096x4096_axis1"] = {"ms": round(ms, 4), "bytes": by,
+                                                    "achieved_gbs": round(by / ms / 1e6, 1),
+                                                    "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
+        if not args.no_cpu:
+            O = _oracle()
+            hp = [p.to_numpy() for p in parts]
+            out["concatenate_f32_4x4096x4096_axis1"]["cpu"] = _cpu_leg(
+                lambda: O.concatenate(hp, axis=1), by, "NArray.concatenate restated by the oracle", "full size")
+            del hp
+        del parts
+    if want("json"):
+        # JSON / YAML text of a 1024 x 1024 Float64 array (device -> host -> text
+        # and back; host-bound)
+        a = phase.NArray.random([1024, 1024], D.F64, seed=41, a=-1.0, b=1.0)
+        t0 = time.perf_counter()
+        j = a.to_json()
+        t1 = time.perf_counter()
+        phase.NArray.from_json(j, D.F64)
+        t2 = time.perf_counter()
+        out["json_f64_1024x1024"] = {"to_json_ms": round((t1 - t0) * 1e3, 1), "from_json_ms": round((t2 - t1) * 1e3, 1),
+                                     "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing, 1 thread"}
+    if want("path"):
+        out["path_ops_i32_8192"] = path_ops(phase, ev, args)
     return out
 
 

@@ -1252,6 +1252,12 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     s += fmt("    const i64 t0 = part * %s / 4, t1 = (part + 1) * %s / 4;\n", lit_i64(P.ntr).c_str(),
              lit_i64(P.ntr).c_str());
     s += "    S a; st_init(a);\n";
+    if (fix) {
+      // the column's first element, loaded before the fold so its latency overlaps it
+      s += fmt("    double f0 = 0.0;\n    if (part == 0 && c < %s && A.col_max) {\n", lit_i64(P.cols).c_str());
+      const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "c", "_cf", &s);
+      s += "      f0 = " + e_ + ";\n    }\n";
+    }
     s += fmt("    if (c < %s) {\n#pragma unroll 4\n      for (i64 t = t0; t < t1; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + c]);\n    }\n",
              lit_i64(P.cols).c_str(), lit_i64(P.cols).c_str());
     s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
@@ -1262,10 +1268,6 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       // Enumerable#max keeps the slice's first element unless a later one is
       // greater: a leading NaN stays, and of equal zeros the first one's sign
       s += "      if (A.col_max) {\n        double mx = st_max(r);\n";
-      {
-        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "c", "_cf", &s);
-        s += "        const double f0 = " + e_ + ";\n";
-      }
       s += "        if (f0 != f0) mx = f0;\n";
       s += fmt("        else if (mx == 0.0) {\n          for (i64 rr = 0; rr < %s; ++rr) {\n", lit_i64(P.rows).c_str());
       {
@@ -1284,6 +1286,12 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   if (P.do_rows) {
     s += fmt("    const i64 rb = (i64)blockIdx.x - %s;\n", lit_i64(ncb).c_str());
     s += "    const i64 i = rb * 256 + threadIdx.x;\n    S a; st_init(a);\n";
+    if (fix) {
+      // the row's first element, loaded before the fold so its latency overlaps it
+      s += fmt("    double f0 = 0.0;\n    if (i < %s) {\n", lit_i64(P.rows).c_str());
+      const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "0", "_rf", &s);
+      s += "      f0 = " + e_ + ";\n    }\n";
+    }
     s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
              lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
     s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n";
@@ -1291,10 +1299,6 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       // first element and, when the row's maximum is a zero, the first zero
       // (its view-order index and sign also serve the full result)
       s += "      const double m0 = st_max(a);\n      double mx = m0;\n      i64 zkey = 0x7fffffffffffffffLL;\n";
-      {
-        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "0", "_rf", &s);
-        s += "      const double f0 = " + e_ + ";\n";
-      }
       s += fmt("      if (m0 == 0.0) {\n        for (i64 cc = 0; cc < %s; ++cc) {\n", lit_i64(P.cols).c_str());
       {
         const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "cc", "_rz", &s);

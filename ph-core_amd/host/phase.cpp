@@ -1105,9 +1105,11 @@ static bool prog_may_raise(const ph_prog &p) {
 }
 
 void Storage::run_pending_map() {
-  std::shared_ptr<PendingMap> pm = std::move(pending_map);
-  pending_map.reset();
-  if (!pm || elements == 0) return;
+  std::shared_ptr<PendingMap> pm = pending_map;  // kept until the map has been issued
+  if (!pm || elements == 0) {
+    pending_map.reset();
+    return;
+  }
   std::vector<const void *> ptrs;
   std::vector<ph_view_desc> descs;
   for (const Operand &o : pm->inputs) {
@@ -1118,6 +1120,7 @@ void Storage::run_pending_map() {
   int32_t flags = 0;
   check(ph_map(&prog, (int32_t)pm->inputs.size(), ptrs.data(), descs.data(), dptr, &pm->out, &flags),
         "map (pending operator result)");
+  pending_map.reset();
 }
 
 void Storage::before_write() {
@@ -2195,6 +2198,13 @@ MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const
       if (p.inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
       Operand in = o;
       in.body.reset();
+      int staged = 0;
+      for (const Operand &q : p.inputs) staged += q.desc.nstage > 0 ? 1 : 0;
+      if (in.desc.nstage > 0 && staged >= 2) {
+        // ph_map takes at most two reshaped/tiled operands: gather this one
+        NArray g = materialize_operand(in);
+        in = g.operand();
+      }
       p.inputs.push_back(in);
       idx = (int)p.inputs.size() - 1;
     }
@@ -2259,7 +2269,12 @@ static NArray map_core(const std::vector<Operand> &ops, const std::function<Expr
     out = NArray::empty(ops[0].shape, p.body.dtype());
   }
   if (out.size() == 0) return out;
-  if (!out_in && !prog_may_raise(p.body.compile((int)p.inputs.size()))) {
+  const ph_prog prog = p.body.compile((int)p.inputs.size());
+  bool uses_index = false;
+  for (int k = 0; k < prog.n_insn; ++k) uses_index = uses_index || prog.insn[k].op == PH_OP_INDEX;
+  // left pending only when running it later cannot raise: no checked Int32
+  // arithmetic, and no element index past Int32 (ph_map refuses that shape)
+  if (!out_in && !prog_may_raise(prog) && !(uses_index && out.size() > INT32_MAX)) {
     auto pm = std::make_shared<PendingMap>();
     pm->inputs = p.inputs;
     pm->body = p.body;

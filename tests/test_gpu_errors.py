@@ -38,8 +38,9 @@ def test_every_launch_refused():
         h = np.arange(1000, dtype=np.int32)
         x = phase.NArray.from_numpy(h)          # host-to-device copy: no launch
         assert np.array_equal(x.to_numpy(), h)  # device-to-host copy: no launch
-        for op in (lambda: x + 1, lambda: x.sum(), lambda: x.view("..2..").to_narr(),
-                   lambda: x.process(lambda v: v * 2).to_narr()):
+        for op in (lambda: (x + 1).materialize(), lambda: x.sum(), lambda: x.view("..2..").to_narr(),
+                   lambda: x.process(lambda v: v * 2).to_narr().materialize(),
+                   lambda: (x.wadd(x) * 1.5).materialize()):
             try:
                 op()
                 raise SystemExit("a refused launch did not raise")
@@ -69,7 +70,7 @@ def test_refusal_after_some_launches_keeps_earlier_results():
         done = []
         try:
             for k in range(100000):
-                y = x * float(k) + 1.0
+                y = (x * float(k) + 1.0).materialize()
                 done.append((k, y))
         except phase.DeviceError as e:
             assert "PH_HIP_FAIL_AFTER=40" in str(e), str(e)
