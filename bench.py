@@ -400,6 +400,7 @@ def other_workloads(phase, ev, args):
     D = phase.DType
     out = {}
     want = lambda name: not args.only or name in args.only  # noqa: E731
+    tr = load_traffic() or {}
     if want("c1"):
         # C1 broadcast at the roofline shape (16384^2 FP64; 1024^2 is the parity case)
         n = 16384
@@ -436,7 +437,6 @@ def other_workloads(phase, ev, args):
                       "note": "a.map_with(b.tile, c) { (x + y) * z }: one kernel, stride-0 broadcast"},
             "bytes": by}
         del a, b, c, bt
-        tr = load_traffic() or {}
         out["c1_broadcast_f64_16384"]["fused"]["traffic"] = tr.get("ph_map_fused_c1")
 
     if want("c1"):
