@@ -340,6 +340,22 @@ int ph_heat2d(int32_t dtype, void *u, void *tmp, int64_t ny, int64_t nx,
               double r, int64_t steps, int64_t energy_every,
               double *energy_out);
 
+/* ph_heat2d without any allocation.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same results, bit for bit.  ph_heat2d keeps the partial
+ * sums of its FP64 energy reductions in library scratch that grows on
+ * demand; this variant keeps them in a fixed 32 KiB device array (2048
+ * groups, the most the reduction uses for any grid), whose address ph_init
+ * looks up on its device, and it skips ph_heat2d's one read-back of max|u|
+ * (the fused first operation it enables changes no result).  Every call on a
+ * device shares that array, and each reduction writes every slot it later
+ * reads, so nothing needs clearing; calls that overlap on one device (from
+ * several threads or processes sharing the library's state) race. */
+int ph_heat2d_noalloc(int32_t dtype, void *u, void *tmp, int64_t ny, int64_t nx,
+                      double r, int64_t steps, int64_t energy_every,
+                      double *energy_out);
+
 /* Sum of squares of (a - b) and of b, both in FP64 (a, b of dtype F32 or
  * F64, n elements): the relative L2 distance report of a reduced-precision
  * field against an FP64 one.  out: device, 2 doubles. */

@@ -98,6 +98,11 @@ lib LibPhHip
                                      col_max : Float64*, flags_out : Int32*) : Int32
   fun heat2d = ph_heat2d(dtype : Int32, u : Void*, tmp : Void*, ny : Int64, nx : Int64, r : Float64,
                          steps : Int64, energy_every : Int64, energy_out : Float64*) : Int32
+  # allocation-free variant (no ph-core counterpart): energy partials in a
+  # fixed per-device array, for callers that cannot allocate device memory
+  fun heat2d_noalloc = ph_heat2d_noalloc(dtype : Int32, u : Void*, tmp : Void*, ny : Int64, nx : Int64,
+                                         r : Float64, steps : Int64, energy_every : Int64,
+                                         energy_out : Float64*) : Int32
   fun l2_diff = ph_l2_diff(dtype_a : Int32, a : Void*, dtype_b : Int32, b : Void*, n : Int64,
                            out : Float64*) : Int32
   fun batch_create = ph_batch_create(handle : Void**, n : Int32, ranks : Int32*, shapes : Int64*,

@@ -64,6 +64,10 @@ void collapse(const ph_view_desc *const *descs, int nops, Layout *out);
 // handling without a faulting kernel.
 int launch_gate(const char *what);
 long launch_count();
+
+// Looks up, for `device` (current), the device address of the fixed array
+// ph_heat2d_noalloc keeps its partial sums in (heat.hip).
+int heat_noalloc_setup(int device);
 int check_launch(const char *what);
 
 // Zeroes n int32 words on the library stream with a kernel (a memset node

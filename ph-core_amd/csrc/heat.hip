@@ -409,15 +409,24 @@ __global__ __launch_bounds__(256) void k_sum_pass2(const D2 *__restrict__ part, 
     for (int k = 0; k < K; ++k) out[k] = acc[k].s + acc[k].c;
 }
 
+// `part_buf` (cap bytes), when given, holds the per-group partials instead
+// of library scratch: at most 2048 groups, so 2048 * K * 16 bytes cover any n.
 template <typename TA, typename TB, int K>
-int fp64_sum(const TA *a, const TB *b, int64_t n, double *out) {
+int fp64_sum(const TA *a, const TB *b, int64_t n, double *out, void *part_buf = nullptr, size_t cap = 0) {
   const int64_t nparts_max = 2048;
   int64_t chunk = (n + nparts_max - 1) / nparts_max;
   if (chunk < 4096) chunk = 4096;
   const int64_t nparts = (n + chunk - 1) / chunk;
-  void *part;
-  int rc = ph::scratch((size_t)(nparts > 0 ? nparts : 1) * K * sizeof(D2), &part, 2);
-  if (rc) return rc;
+  const size_t need = (size_t)(nparts > 0 ? nparts : 1) * K * sizeof(D2);
+  void *part = part_buf;
+  int rc = PH_OK;
+  if (part) {
+    if (need > cap)
+      return ph::set_error(PH_E_ARGUMENT, "fp64 sum: %zu bytes of partials exceed the fixed %zu", need, cap);
+  } else {
+    rc = ph::scratch(need, &part, 2);
+    if (rc) return rc;
+  }
   rc = ph::launch_gate("fp64 sum");
   if (rc) return rc;
   if (nparts > 0)
@@ -432,14 +441,23 @@ int fp64_sum(const TA *a, const TB *b, int64_t n, double *out) {
 // (compute units x workgroups per compute unit at the kernel's register
 // use), cached per device and kernel.
 int64_t resident_groups(const void *f) {
-  static std::map<std::pair<int, const void *>, int64_t> cache;
+  // a fixed table (no allocation on any call): one slot per kernel and device
+  struct Slot {
+    int dev;
+    const void *f;
+    int64_t groups;
+  };
+  static Slot cache[128];
+  static int used = 0;
   const int dev = ph::device();
-  auto it = cache.find({dev, f});
-  if (it != cache.end()) return it->second;
+  for (int k = 0; k < used; ++k)
+    if (cache[k].dev == dev && cache[k].f == f) return cache[k].groups;
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 64 * kWaves, 0) != hipSuccess || per < 1) per = 1;
-  return cache[{dev, f}] = (int64_t)cus * per;
+  const int64_t g = (int64_t)cus * per;
+  if (used < 128) cache[used++] = Slot{dev, f, g};
+  return g;
 }
 
 // Band height for a pass of K steps.  Every band is a row of independent
@@ -595,9 +613,12 @@ int fused_safe(const T *u, int64_t n, T r, bool *safe, int64_t *steps_ok) {
   return PH_OK;
 }
 
+// part_buf / part_cap: a fixed buffer for the energy partials (the
+// allocation-free entry point); allow_fused: the fused first operation may
+// be used (it needs one read-back of max|u|).
 template <typename T>
 int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int64_t every,
-             double *energy) {
+             double *energy, void *part_buf = nullptr, size_t part_cap = 0, bool allow_fused = true) {
   constexpr int V = Vec<T>::n;
   // Steps per pass, measured on MI355X at 8192^2: 12 for Float32 (three
   // halo lanes; 8, 10, 14 and 16 are slower) and 8 for Float64 (its 252
@@ -624,11 +645,11 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
   int rc;
   // The fused form needs one read-back of max|u| (a synchronisation), so
   // short runs keep the separate multiply and subtract.
-  const bool try_fused = steps >= 32;
+  const bool try_fused = allow_fused && steps >= 32;
   bool fused = false;
   int64_t fused_until = 0;
   if (energy && every > 0) {
-    rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
+    rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++, part_buf, part_cap);
     if (rc) return rc;
   }
   // Steps run kmax at a time up to each energy checkpoint; the remainder
@@ -654,7 +675,7 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
     nxt = sw;
     t += k;
     if (energy && every > 0 && t % every == 0) {
-      rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++);
+      rc = fp64_sum<T, T, 1>(cur, cur, ny * nx, energy + ei++, part_buf, part_cap);
       if (rc) return rc;
     }
   }
@@ -664,7 +685,57 @@ int heat_run(T *u, T *tmp, int64_t ny, int64_t nx, double r, int64_t steps, int6
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The allocation-free heat run (ph_heat2d_noalloc) keeps its FP64 energy
+// partials in this fixed device array instead of library scratch: 2048
+// groups of one compensated pair, the most fp64_sum ever uses.  Its device
+// address is looked up once per device when the library is initialised on
+// it; host code uses that address, never the array's name.  Inside namespace
+// ph the bare name would find the function ph::scratch (library scratch), so
+// the lookup names the array as ::scratch.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[2048 * sizeof(D2)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_noalloc_scratch[kMaxDevices] = {nullptr};
+}  // namespace
+
+namespace ph {
+
+int heat_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(PH_E_ARGUMENT, "device %d beyond the %d the fixed heat scratch tracks", device, kMaxDevices);
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(::scratch));
+  if (e != hipSuccess || !p)
+    return set_error(PH_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_noalloc_scratch[device] = p;
+  return PH_OK;
+}
+
+}  // namespace ph
+
 extern "C" {
+
+int ph_heat2d_noalloc(int32_t dtype, void *u, void *tmp, int64_t ny, int64_t nx, double r, int64_t steps,
+                      int64_t energy_every, double *energy_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (ny <= 0 || nx <= 0) return ph::set_error(PH_E_DIMENSION, "heat grid %lld x %lld", (long long)ny, (long long)nx);
+  if (steps < 0) return ph::set_error(PH_E_ARGUMENT, "negative step count");
+  if (!u || !tmp || u == tmp) return ph::set_error(PH_E_ARGUMENT, "heat2d needs two distinct buffers");
+  if (energy_every > 0 && !energy_out) return ph::set_error(PH_E_ARGUMENT, "energy_out is null");
+  const int dev = ph::device();
+  void *sp = dev >= 0 && dev < kMaxDevices ? g_noalloc_scratch[dev] : nullptr;
+  if (!sp) return ph::set_error(PH_E_NOINIT, "the fixed heat scratch of device %d was not set up", dev);
+  const size_t cap = sizeof(::scratch);
+  if (dtype == PH_F32)
+    return heat_run<float>((float *)u, (float *)tmp, ny, nx, r, steps, energy_every, energy_out, sp, cap, false);
+  if (dtype == PH_F64)
+    return heat_run<double>((double *)u, (double *)tmp, ny, nx, r, steps, energy_every, energy_out, sp, cap, false);
+  return ph::set_error(PH_E_ARGUMENT, "heat2d: dtype %d (F32 or F64)", dtype);
+}
 
 int ph_heat2d(int32_t dtype, void *u, void *tmp, int64_t ny, int64_t nx, double r, int64_t steps,
               int64_t energy_every, double *energy_out) {

@@ -204,6 +204,12 @@ int ph_init(int device) {
     return ph::set_error(PH_E_ARGUMENT, "device %d outside [0, %d)", device, n);
   PH_HIP(hipSetDevice(device));
   PH_HIP(hipStreamCreateWithFlags(&ph::g_stream, hipStreamNonBlocking));
+  const int rc = ph::heat_noalloc_setup(device);
+  if (rc) {
+    (void)hipStreamDestroy(ph::g_stream);
+    ph::g_stream = nullptr;
+    return rc;
+  }
   ph::g_device = device;
   ph::g_inited = true;
   return PH_OK;
