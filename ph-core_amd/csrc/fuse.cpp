@@ -916,7 +916,7 @@ struct RedArgs {
   void *rowpart, *colpart, *rowfinal;
   double *full_sum, *full_max, *row_sum, *row_max, *col_sum, *col_max;
   int *flags;
-  int64_t *rowzero;  // per row: the first zero's (view-order index << 1 | sign), or INT64_MAX
+  int64_t *rowzero;  // per block of 256 rows: its first zero's (view-order index << 1 | sign), or INT64_MAX
 };
 
 struct RedPlan {
@@ -1291,6 +1291,14 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       s += fmt("    double f0 = 0.0;\n    if (i < %s) {\n", lit_i64(P.rows).c_str());
       const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "0", "_rf", &s);
       s += "      f0 = " + e_ + ";\n    }\n";
+      s += "    i64 zkey = 0x7fffffffffffffffLL;\n";
+      if (P.do_full) {
+        // the view's first element for the full maximum, likewise loaded early
+        // (only the row block that finishes last uses it)
+        s += "    double ff0 = 0.0;\n    if (threadIdx.x == 0 && A.full_max) {\n";
+        const std::string e2 = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "0", "_ff", &s);
+        s += "      ff0 = " + e2 + ";\n    }\n";
+      }
     }
     s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
              lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
@@ -1298,7 +1306,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     if (fix) {
       // first element and, when the row's maximum is a zero, the first zero
       // (its view-order index and sign also serve the full result)
-      s += "      const double m0 = st_max(a);\n      double mx = m0;\n      i64 zkey = 0x7fffffffffffffffLL;\n";
+      s += "      const double m0 = st_max(a);\n      double mx = m0;\n";
       s += fmt("      if (m0 == 0.0) {\n        for (i64 cc = 0; cc < %s; ++cc) {\n", lit_i64(P.cols).c_str());
       {
         const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "cc", "_rz", &s);
@@ -1308,23 +1316,30 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       s += "          if (g == 0.0) { mx = g; zkey = ((" + key + ") << 1) | (__builtin_signbit(g) ? 1 : 0); break; }\n";
       s += "        }\n      }\n      if (f0 != f0) mx = f0;\n";
       s += "      if (A.row_max) A.row_max[i] = mx;\n";
-      if (P.do_full) s += "      A.rowzero[i] = zkey;\n      __threadfence();\n";
       s += "    }\n";
     } else {
       s += "      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
     }
     if (P.do_full) {
       const int64_t nrb = (P.rows + 255) / 256;
+      // (with the fix-up, the block's least first-zero key is folded alongside)
+      if (fix) s += "    __shared__ i64 zsh[256];\n    zsh[threadIdx.x] = zkey;\n";
       s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
       s += "    for (int off = 1; off < 256; off <<= 1) {\n      S o = sh[threadIdx.x];\n";
-      s += "      if ((threadIdx.x & (2 * off - 1)) == 0) o = st_comb(o, sh[threadIdx.x + off]);\n";
-      s += "      __syncthreads();\n      sh[threadIdx.x] = o;\n      __syncthreads();\n    }\n";
+      if (fix) s += "      i64 zo = zsh[threadIdx.x];\n";
+      s += "      if ((threadIdx.x & (2 * off - 1)) == 0) {\n        o = st_comb(o, sh[threadIdx.x + off]);\n";
+      if (fix) s += "        zo = zsh[threadIdx.x + off] < zo ? zsh[threadIdx.x + off] : zo;\n";
+      s += "      }\n      __syncthreads();\n      sh[threadIdx.x] = o;\n";
+      if (fix) s += "      zsh[threadIdx.x] = zo;\n";
+      s += "      __syncthreads();\n    }\n";
       // The row block that finishes last (a ticket counter, reset by the
       // tile kernel) folds the row blocks' states in order for the full
       // result: one wavefront, each lane a run of consecutive blocks, then
       // an ordered fold across the lanes.
       s += "    __shared__ int last;\n";
-      s += "    if (threadIdx.x == 0) {\n      ((S *)A.rowfinal)[rb] = sh[0];\n      __threadfence();\n";
+      s += "    if (threadIdx.x == 0) {\n      ((S *)A.rowfinal)[rb] = sh[0];\n";
+      if (fix) s += "      A.rowzero[rb] = zsh[0];\n";
+      s += "      __threadfence();\n";
       s += fmt("      last = atomicAdd(A.flags + 1, 1) == %s - 1;\n    }\n    __syncthreads();\n", lit_i64(nrb).c_str());
       s += "    if (last && threadIdx.x < 64) {\n      __threadfence();\n";
       s += fmt("      const i64 n = %s; const i64 per = (n + 63) / 64;\n", lit_i64(nrb).c_str());
@@ -1332,21 +1347,17 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       s += "      S f; st_init(f);\n      for (i64 j = b; j < e; ++j) f = st_comb(f, ((const S *)A.rowfinal)[j]);\n";
       s += "      f = wave_fold(f);\n";
       if (fix) {
-        // the first zero of the whole view: the least key over the rows
+        // the first zero of the whole view: the least key over the row blocks
         // (after wave_fold only lane 0 holds the whole maximum)
         s += "      i64 zk = 0x7fffffffffffffffLL;\n      const double fm0 = __shfl(st_max(f), 0);\n";
         s += "      if (A.full_max && fm0 == 0.0)\n        for (i64 j = threadIdx.x; j < " +
-             lit_i64(P.rows) + "; j += 64) zk = A.rowzero[j] < zk ? A.rowzero[j] : zk;\n";
+             lit_i64(nrb) + "; j += 64) zk = A.rowzero[j] < zk ? A.rowzero[j] : zk;\n";
         s += "      for (int o = 32; o > 0; o >>= 1) { const i64 t = __shfl_xor(zk, o); zk = t < zk ? t : zk; }\n";
       }
       s += "      if (threadIdx.x == 0) {\n        if (A.full_sum) { A.full_sum[0] = st_sum(f); fl |= st_ovf(f); }\n";
       if (fix) {
         s += "        if (A.full_max) {\n          double mx = st_max(f);\n";
-        {
-          const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "0", "_ff", &s);
-          s += "          const double f0 = " + e_ + ";\n";
-        }
-        s += "          if (f0 != f0) mx = f0;\n";
+        s += "          if (ff0 != ff0) mx = ff0;\n";
         s += "          else if (mx == 0.0 && zk != 0x7fffffffffffffffLL) mx = (zk & 1) ? -0.0 : 0.0;\n";
         s += "          A.full_max[0] = mx;\n        }\n      }\n    }\n";
       } else {
@@ -1446,7 +1457,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
   const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
   const size_t rowfinal_b = P.do_rows ? (size_t)P.rows * P.state_bytes : 0;
-  const size_t rowzero_b = P.do_full ? (size_t)P.rows * sizeof(int64_t) : 0;
+  const size_t rowzero_b = P.do_full ? (size_t)((P.rows + 255) / 256) * sizeof(int64_t) : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   void *base;
   rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + al(rowzero_b) + 256, &base, 2);

@@ -2167,9 +2167,34 @@ bool whole_pending(const Operand &o) {
   return o.storage->pending_map && std::memcmp(&o.desc, &o.storage->pending_map->out, sizeof(ph_view_desc)) == 0;
 }
 
+// Bytes per result element that a pending map's inputs read; an input whose
+// buffer holds fewer elements than the result (a broadcast) counts in part.
+double pending_read_bytes(const PendingMap &pm) {
+  int64_t n = 1;
+  for (int k = 0; k < pm.out.view.rank; ++k) n *= pm.out.view.shape[k];
+  double b = 0.0;
+  for (const Operand &i : pm.inputs)
+    b += (double)dtype_size((DType)i.desc.dtype) * std::min(1.0, (double)i.desc.extent / (double)std::max<int64_t>(n, 1));
+  return b;
+}
+
+// Whether a pending result is spliced into one more consumer.  One whose
+// inputs read at most an eighth more bytes per element than the result
+// holds always is (splicing costs next to no traffic; a broadcast operand
+// adds its small share).  One that reads more (a Bool mask computed from
+// Int32 values) is spliced into its first consumer only: a second consumer
+// runs it into its buffer and reads that, so a result used many times is
+// computed once.
+bool splice_pending(PendingMap &pm) {
+  if (pm.spliced > 0 && pending_read_bytes(pm) > 1.125 * (double)dtype_size((DType)pm.out.dtype)) return false;
+  ++pm.spliced;
+  return true;
+}
+
 // `fuse`: pending operator results are spliced into the program instead of
-// being run first, except those that read `dst` (the buffer the map writes):
-// the reference computed them before the write, so they stay a snapshot.
+// being run first (splice_pending), except those that read `dst` (the
+// buffer the map writes): the reference computed them before the write, so
+// they stay a snapshot.
 MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const std::vector<Expr> &)> &fn,
                  bool fuse, const Storage *dst) {
   if (ops.empty()) throw ArgumentError("map_with needs an operand");
@@ -2180,10 +2205,10 @@ MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const
   MapPlan p;
   std::function<Expr(const Operand &)> leaf = [&](const Operand &o) -> Expr {
     if (fuse && whole_pending(o)) {
-      const PendingMap &pm = *o.storage->pending_map;
+      PendingMap &pm = *o.storage->pending_map;
       bool reads_dst = false;
       for (const Operand &i : pm.inputs) reads_dst = reads_dst || i.storage.get() == dst;
-      if (!reads_dst) {
+      if (!reads_dst && splice_pending(pm)) {
         std::vector<Expr> subs;
         for (const Operand &i : pm.inputs) subs.push_back(leaf(i));
         Expr e = pm.body.substitute(subs);

@@ -296,6 +296,7 @@ struct PendingMap {
   std::vector<Operand> inputs;
   Expr body;
   ph_view_desc out;
+  int spliced = 0;  // consumers it has been spliced into so far
 };
 
 // Device-resident NArray (src/n_array.cr): row-major buffer + shape.

@@ -93,3 +93,37 @@ def test_unused_pending_result_costs_nothing(ph):
     assert ph.launch_count() == k0
     s = (a * 3.0 + 1.0).sum()  # a reduction runs the pending map first
     assert s == 4000.0
+
+
+def test_mask_used_many_times_is_computed_once_more(ph):
+    """A pending Bool mask computed from Int32 values reads 4 bytes per
+    element to give 1: it is spliced into its first consumer, and the second
+    consumer runs it into its own buffer (two launches), so the later ones
+    read the stored mask (one launch each).  A pending result that reads no
+    more than it holds (a broadcast sum) is spliced every time.  Every
+    result bit-exact against the oracle."""
+    rng = np.random.default_rng(107)
+    xh = rng.integers(-2 ** 31, 2 ** 31, (300, 200), dtype=np.int64).astype(np.int32)
+    zh = rng.integers(-2 ** 31, 2 ** 31, (300, 200), dtype=np.int64).astype(np.int32)
+    x = ph.NArray.from_numpy(xh)
+    mask = x.process(lambda v: (v & 1) == 0).to_narr()
+    mh = (xh & 1) == 0
+    launches = []
+    for k in range(4):
+        z = ph.NArray.from_numpy(zh)
+        ph.sync()
+        k0 = ph.launch_count()
+        z[mask] = 7 + k
+        z.materialize()
+        launches.append(ph.launch_count() - k0)
+        assert np.array_equal(z.to_numpy(), O.set_mask(zh.copy(), mh, 7 + k)), k
+    assert launches == [1, 2, 1, 1], launches
+    an, bn = rng.uniform(-1, 1, (300, 200)), rng.uniform(-1, 1, (1, 200))
+    t = ph.NArray.from_numpy(an) + ph.NArray.from_numpy(bn).tile(300, 1)
+    exp, _ = O.eval_expr("(add x0 x1)", [an, np.tile(bn, (300, 1))])
+    for k in range(3):
+        ph.sync()
+        k0 = ph.launch_count()
+        got = (t * float(k + 2)).to_numpy()
+        assert ph.launch_count() - k0 == 1
+        assert np.array_equal(got, exp * float(k + 2)), k

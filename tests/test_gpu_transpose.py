@@ -94,8 +94,9 @@ def test_large_transpose(ph):
 
 
 def test_maps_over_transposed_operands(ph):
-    """Maps whose operands are transposed views: up to two are copied
-    through the tiled transpose first, any further ones are read in place;
+    """Maps whose operands are transposed views: one or two are staged
+    through LDS inside the map itself (ph_map_tiled); with three, two are
+    copied through the tiled transpose first and the third is read in place;
     bit-exact against the oracle either way."""
     from crystal import trace
     rng = np.random.default_rng(59)
@@ -116,6 +117,65 @@ def test_maps_over_transposed_operands(ph):
     pv = ph.NArray.from_numpy(f).view().permute(1, 0).process(lambda v: (v + 4) ** 2 * 0.5 - v)
     exp, _ = O.eval_expr(trace(lambda v: (v + 4) ** 2 * 0.5 - v), [np.ascontiguousarray(f.T)])
     assert np.array_equal(pv.to_narr().to_numpy(), exp)
+
+
+def _one_launch(ph, make):
+    """The materialised result of make() and the kernel launches it took."""
+    r = make()
+    n0 = ph.launch_count()
+    r = r.materialize()
+    return r, ph.launch_count() - n0
+
+
+def test_tiled_map_layouts(ph):
+    """The LDS-tiled map (one launch, no staging copy) beyond one transposed
+    Int32 operand: two transposed Float64 operands (two 64 x 65 LDS tiles,
+    66.5 KB), a rank-3 permuted view (the loop over the outer axis), a
+    transposed view that is also reversed (descending unit strides), and
+    extents that are not multiples of 64; then an Int32 division by a zero
+    read through the tiled path raises DivisionByZeroError.  Bit-exact
+    against the oracle."""
+    from crystal import trace
+    rng = np.random.default_rng(61)
+    # two transposed Float64 operands and a contiguous one
+    hs = [rng.standard_normal((300, 257)) for _ in range(2)]
+    c = rng.standard_normal((257, 300))
+    xs = [ph.NArray.from_numpy(h).view().permute(1, 0) for h in hs]
+    dc = ph.NArray.from_numpy(c)
+    fn = lambda a, b, d: (a + b) * d - a  # noqa: E731
+    exp, _ = O.eval_expr(trace(fn, 3), [np.ascontiguousarray(hs[0].T), np.ascontiguousarray(hs[1].T), c])
+    got, n = _one_launch(ph, lambda: xs[0].map_with(xs[1], dc, block=fn))
+    assert n == 1, n
+    assert np.array_equal(got.to_numpy().view(np.uint64), exp.view(np.uint64))
+    # rank 3: (5, 40, 70) viewed as (5, 70, 40); the outer axis is looped
+    h3 = rng.integers(-2 ** 31, 2 ** 31, (5, 40, 70), dtype=np.int64).astype(np.int32)
+    c3 = rng.integers(-2 ** 31, 2 ** 31, (5, 70, 40), dtype=np.int64).astype(np.int32)
+    fn2 = lambda a, b: a.wadd(b) ^ (a & 4095)  # noqa: E731
+    exp, _ = O.eval_expr(trace(fn2, 2), [np.ascontiguousarray(h3.transpose(0, 2, 1)), c3])
+    got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(h3).view().permute(0, 2, 1).map_with(
+        ph.NArray.from_numpy(c3), block=fn2))
+    assert n == 1, n
+    assert np.array_equal(got.to_numpy(), exp)
+    # transposed and reversed on both axes, 200 x 131
+    hr = rng.integers(-2 ** 31, 2 ** 31, (200, 131), dtype=np.int64).astype(np.int32)
+    cr = rng.integers(-2 ** 31, 2 ** 31, (131, 200), dtype=np.int64).astype(np.int32)
+    vr = np.ascontiguousarray(hr[::-1, ::-1].T)
+    exp, _ = O.eval_expr(trace(fn2, 2), [vr, cr])
+    got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(hr).view().reverse().permute(1, 0).map_with(
+        ph.NArray.from_numpy(cr), block=fn2))
+    assert n == 1, n
+    assert np.array_equal(got.to_numpy(), exp)
+    # an Int32 divisor with a zero, read through the tiled path
+    num = rng.integers(-1000, 1000, (131, 200), dtype=np.int64).astype(np.int32)
+    den = rng.integers(1, 50, (200, 131), dtype=np.int64).astype(np.int32)
+    xden = ph.NArray.from_numpy(den).view().permute(1, 0)
+    exp, fl = O.eval_expr(trace(lambda a, b: a // b, 2), [num, np.ascontiguousarray(den.T)])
+    assert fl == 0
+    assert np.array_equal(ph.NArray.from_numpy(num).map_with(xden, block=lambda a, b: a // b).to_numpy(), exp)
+    den[77, 13] = 0
+    xden = ph.NArray.from_numpy(den).view().permute(1, 0)
+    with pytest.raises(ph.DivisionByZeroError):
+        ph.NArray.from_numpy(num).map_with(xden, block=lambda a, b: a // b).to_numpy()
 
 
 def test_reductions_over_transposed_views_int32(ph):
