@@ -731,6 +731,34 @@ def batched_workload(phase, ev, args):
         "note": "op_bytes counts each op's own traffic (12 B/element + 8 B/slice element); the fused pass "
                 "moves pass_bytes (8 B/element + 4 B/slice element), whose rate is the roofline figure",
     }
+    # the same work as a general batch: per record a map program (v + s), a
+    # reduction program (v, summed) and a region ("..2.." on every axis),
+    # run by the kernel generated for those programs
+    recs = [{"shape": list(sh), "map": 0, "red": 1, "ops": "sum", "region": ["..2.."] * len(sh),
+             "scalar": float(scalars[k])} for k, sh in enumerate(shapes)]
+    g = phase.GBatch(phase.DType.F32, phase.DType.F32, [lambda v, s: v + s, lambda v, s: v], recs)
+    g.set_x(phase.NArray.random([g.x.size], phase.DType.F32, seed=4, a=-1.0, b=1.0))
+    gchunk, ggrid, gdyn = g.tune()
+    g.run(0)
+    gms_graph = time_loop(ev, lambda: g.run(0), 20, phase)
+    gms_direct = time_loop(ev, lambda: g.run(1), 20, phase)
+    gms_unbatched = None
+    if not args.no_unbatched:
+        ev.sync()
+        g.run(2)
+        gms_unbatched = time_loop(ev, lambda: g.run(2), 2)
+    res["general"] = {
+        "kernel": "ph_gbatch (generated for the records' programs) + ph_gbatch_final",
+        "schedule": {"chunk": gchunk, "grid": ggrid, "queue": bool(gdyn)},
+        "graph_ms": round(gms_graph, 4), "direct_ms": round(gms_direct, 4),
+        "unbatched_ms": round(gms_unbatched, 3) if gms_unbatched else None,
+        "speedup_vs_unbatched": round(gms_unbatched / gms_graph, 2) if gms_unbatched else None,
+        "pass_gbs": round(pass_bytes / gms_graph / 1e6, 1),
+        "pass_frac": round(pass_bytes / gms_graph / 1e6 / PEAK_HBM_GBS, 4),
+        "note": "records carry their own programs and regions (ph_gbatch); unbatched launches the same "
+                "kernel once per array",
+    }
+    del g
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_api as O

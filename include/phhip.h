@@ -415,6 +415,77 @@ int ph_batch_schedule(void *handle, int64_t chunk, int32_t grid,
 int ph_batch_destroy(void *handle);
 
 /* ---------------------------------------------------------------------------
+ * General batches.  No reference counterpart: ph-core runs per-array work in
+ * a host loop over its NArrays.  Record k describes one array x_k (dtype
+ * x_dtype: I32, F32 or F64; rank 1..3; extents in [1, 2^30]; fewer than 2^31
+ * elements; row-major) and the work done on it:
+ *   map >= 0    y_k = progs[map] applied to every element, as
+ *               `x_k.map { |v| ... }` (src/n_array.cr:589-595); y has y_dtype
+ *               (I32, F32, F64 or Bool), the programs' result type;
+ *   red >= 0    v = progs[red] applied to every element (F32 or F64); with
+ *               red_ops bit PH_RED_SUM sums[k] = v.sum, with PH_RED_MAX
+ *               maxes[k] = v.max (Enumerable#sum / #max over `each`,
+ *               src/multi_indexable.cr:18, :656-658);
+ *   slice != 0  z_k = x_k.view(region).to_narr, the region given per axis as
+ *               first, step (non-zero; a negative step runs the axis
+ *               backwards) and count (src/multi_indexable.cr:881-887,
+ *               src/view.cr:123), every index inside the array, else
+ *               PH_E_INDEX.
+ * A program reads input 0, the element (x_dtype), and input 1, the record's
+ * scalar converted to x_dtype.  The whole batch runs as one pass over the
+ * packed x: one persistent kernel that hipRTC generates for the batch's
+ * programs, fed from a device table of work items (largest first), plus a
+ * small kernel that folds each array's partial results; mode 0 replays them
+ * as a HIP graph.  Sums are FP64 per lane and folded in a fixed order, so
+ * they do not depend on the schedule; maxima are exactly the `>` loop's
+ * result (the first of equal maxima, so the first zero's sign; NaN when the
+ * first value is NaN, other NaNs skipped).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t rank;
+  int32_t map;      /* program index for y_k, or -1 */
+  int32_t red;      /* program index for the reductions, or -1 */
+  int32_t red_ops;  /* PH_RED_SUM | PH_RED_MAX, 0 when red < 0 */
+  int32_t slice;    /* 1: copy the region into z_k */
+  int32_t pad_;
+  int64_t shape[3]; /* the first `rank` entries */
+  int64_t first[3], step[3], count[3]; /* the region, `rank` entries */
+  double scalar;
+} ph_batch_rec;
+
+/* n_progs in [0, 16]; the records may use at most 16 distinct
+ * (map, red, red_ops, slice) combinations. */
+int ph_gbatch_create(void **handle, int32_t x_dtype, int32_t y_dtype,
+                     int32_t n_progs, const ph_prog *progs, int32_t n,
+                     const ph_batch_rec *recs);
+/* Elements the caller allocates: x and y hold x_elements of their dtypes
+ * (each array starts at a multiple of 16 elements, and the kernel reads and
+ * writes up to the end of that padding), z holds z_elements of x_dtype.
+ * x_k and y_k are at offsets_xy[k], z_k at offsets_z[k] (either may be
+ * NULL, else n entries). */
+int ph_gbatch_sizes(void *handle, int64_t *x_elements, int64_t *z_elements,
+                    int64_t *offsets_xy, int64_t *offsets_z);
+/* Runs the batch over device buffers: x, y, z, sums[n], maxes[n] (those the
+ * records write nothing into may be NULL; x and y 16-byte aligned).  mode 0:
+ * replay of the HIP graph captured for these buffers (captured on first use
+ * and again when the buffers change); 1: the kernels launched directly; 2:
+ * the kernel launched once per array (the work without the batching); 3:
+ * each record through ph_map, ph_reduce and ph_strided_copy in turn (those
+ * compile a kernel per distinct shape: a cross-check for small batches).
+ * flags_out (may be NULL): the PH_FLAG_* bits the
+ * programs raised (Int32 overflow, division by zero); y then holds the
+ * wrapped values, as ph_map leaves them. */
+int ph_gbatch_run(void *handle, const void *x, void *y, void *z, double *sums,
+                  double *maxes, int32_t mode, int32_t *flags_out);
+/* Times candidate schedules (work-item size, workgroups per CU, queue or
+ * static assignment) on the given buffers, keeps the fastest and reports it
+ * (outputs may be NULL).  Overwrites y, z, sums and maxes. */
+int ph_gbatch_tune(void *handle, const void *x, void *y, void *z,
+                   double *sums, double *maxes, int64_t *chunk_out,
+                   int32_t *grid_out, int32_t *dynamic_out);
+int ph_gbatch_destroy(void *handle);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no reference counterpart).
  * ph_compile_check generates and compiles, without launching, the fused
  * kernels ph_map (kind 0) and ph_reduce2d_all (kind 1) would use for `prog`
@@ -423,8 +494,11 @@ int ph_batch_destroy(void *handle);
  * a cols x rows array (rows and cols of 16 or more).  Kind 3 compiles the
  * kernels of a map whose input 0 reads the output's own buffer transposed
  * (rows == cols; input 0 has the output's dtype), kind 4 those of a copy
- * of in_dtypes[0] elements that reverses a buffer onto itself.  It needs no
- * GPU, so program lowering can be checked on a build host.
+ * of in_dtypes[0] elements that reverses a buffer onto itself, kind 5 the
+ * general batch's kernels with `prog` as map and (for a float result)
+ * reduction program (two inputs of one dtype: the element and the scalar;
+ * rows and cols unused).  It needs no GPU, so program lowering can be
+ * checked on a build host.
  * ------------------------------------------------------------------------- */
 int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes,
                      int64_t rows, int64_t cols, int32_t kind);

@@ -116,6 +116,30 @@ lib LibPhHip
   fun batch_tune = ph_batch_tune(handle : Void*, chunk : Int64*, grid : Int32*, dynamic : Int32*) : Int32
   fun batch_schedule = ph_batch_schedule(handle : Void*, chunk : Int64, grid : Int32, dynamic : Int32) : Int32
   fun batch_destroy = ph_batch_destroy(handle : Void*) : Int32
+
+  # general batches: per record a map program, a reduction program and a region
+  struct BatchRec
+    rank : Int32
+    map : Int32
+    red : Int32
+    red_ops : Int32
+    slice : Int32
+    pad_ : Int32
+    shape : StaticArray(Int64, 3)
+    first : StaticArray(Int64, 3)
+    step : StaticArray(Int64, 3)
+    count : StaticArray(Int64, 3)
+    scalar : Float64
+  end
+  fun gbatch_create = ph_gbatch_create(handle : Void**, x_dtype : Int32, y_dtype : Int32, n_progs : Int32,
+                                       progs : Prog*, n : Int32, recs : BatchRec*) : Int32
+  fun gbatch_sizes = ph_gbatch_sizes(handle : Void*, x_elements : Int64*, z_elements : Int64*,
+                                     offsets_xy : Int64*, offsets_z : Int64*) : Int32
+  fun gbatch_run = ph_gbatch_run(handle : Void*, x : Void*, y : Void*, z : Void*, sums : Float64*,
+                                 maxes : Float64*, mode : Int32, flags_out : Int32*) : Int32
+  fun gbatch_tune = ph_gbatch_tune(handle : Void*, x : Void*, y : Void*, z : Void*, sums : Float64*,
+                                   maxes : Float64*, chunk : Int64*, grid : Int32*, dynamic : Int32*) : Int32
+  fun gbatch_destroy = ph_gbatch_destroy(handle : Void*) : Int32
   fun jit_compiles = ph_jit_compiles : LibC::Long
   fun launch_count = ph_launch_count : LibC::Long
 end

@@ -65,6 +65,11 @@ void collapse(const ph_view_desc *const *descs, int nops, Layout *out);
 int launch_gate(const char *what);
 long launch_count();
 
+// Generates and compiles (no launch) the general batch kernels for `prog`
+// as map and, when its result is a float, as reduction program, over
+// elements of x_dtype (ph_compile_check kind 5).
+int gbatch_compile_check(const ph_prog *prog, int x_dtype);
+
 // Looks up, for `device` (current), the device address of the fixed array
 // ph_heat2d_noalloc keeps its partial sums in (heat.hip).
 int heat_noalloc_setup(int device);

@@ -34,9 +34,12 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gen.hpp"
 #include "jit.hpp"
 
-namespace {
+// Program lowering shared with the general batch (gen.hpp).
+namespace ph {
+namespace gen {
 
 const char *ctype(int dt) {
   switch (dt) {
@@ -94,7 +97,7 @@ bool is_float(int dt) { return dt == PH_F32 || dt == PH_F64; }
 // Validates `p` against the input storage dtypes and checks every operand's
 // dtype; fills reg dtypes on the way.  PH_OP_INDEX (an element's lexicographic
 // index) is accepted only where `allow_index` says the kernel defines it.
-int validate_prog(const ph_prog *p, int n_in, const int *in_dt, bool allow_index = false) {
+int validate_prog(const ph_prog *p, int n_in, const int *in_dt, bool allow_index) {
   if (!p) return ph::set_error(PH_E_ARGUMENT, "null program");
   if (p->n_insn < 1 || p->n_insn > PH_MAX_INSN)
     return ph::set_error(PH_E_ARGUMENT, "program length %d", p->n_insn);
@@ -345,7 +348,8 @@ std::string emit_prog(const ph_prog *p, const std::vector<std::string> &inexpr, 
   return regv[p->out_reg];
 }
 
-const char *kPrelude = R"PH(
+extern const char *const kPrelude;
+const char *const kPrelude = R"PH(
 typedef long long i64;
 __device__ __forceinline__ float ph_ld(const unsigned short *p, i64 i) { return __uint_as_float(((unsigned)p[i]) << 16); }
 __device__ __forceinline__ bool ph_ld(const unsigned char *p, i64 i) { return p[i] != 0; }
@@ -395,6 +399,38 @@ __device__ __forceinline__ int ph_imod(int a, int b, int &fl) {
 __device__ __forceinline__ int ph_ineg(int a, int &fl) { if (a == (-2147483647 - 1)) fl |= 1; return (int)(0u - (unsigned)a); }
 __device__ __forceinline__ int ph_iabs(int a, int &fl) { if (a == (-2147483647 - 1)) fl |= 1; return a < 0 ? (int)(0u - (unsigned)a) : a; }
 )PH";
+
+// Whether any element can raise a flag.  Int32 // and % by a constant other
+// than 0 and -1 cannot (the divisor is known when the program is built, so
+// neither division by zero nor the MIN / -1 overflow can occur): a
+// coordinate computed from the element index then costs no read-back.
+bool prog_may_flag(const ph_prog *p) {
+  bool isc[PH_NREG] = {false};
+  int64_t cv[PH_NREG] = {0};
+  for (int k = 0; k < p->n_insn; ++k) {
+    const ph_insn &in = p->insn[k];
+    if (in.op == PH_OP_IPOW || in.op == PH_OP_WIPOW) return true;
+    if (in.dt == PH_I32 && (in.op == PH_OP_FLOORDIV || in.op == PH_OP_MOD)) {
+      const bool safe = in.b < PH_NREG && isc[in.b] && cv[in.b] != 0 && cv[in.b] != -1;
+      if (!safe) return true;
+    }
+    if (in.dt == PH_I32 && (in.op == PH_OP_ADD || in.op == PH_OP_SUB || in.op == PH_OP_MUL ||
+                            in.op == PH_OP_NEG || in.op == PH_OP_ABS))
+      return true;
+    if (in.dst < PH_NREG) {
+      isc[in.dst] = in.op == PH_OP_CONST && in.dt == PH_I32;
+      cv[in.dst] = in.imm.i;
+    }
+  }
+  return false;
+}
+
+}  // namespace gen
+}  // namespace ph
+
+namespace {
+
+using namespace ph::gen;
 
 // ---------------------------------------------------------------------------
 // Map
@@ -618,32 +654,6 @@ int read_flags(int *dflags, bool may_flag, int32_t *flags_out) {
   PH_HIP(hipStreamSynchronize(ph::stream()));
   *flags_out = v;
   return PH_OK;
-}
-
-// Whether any element of `p` can raise a PH_FLAG_* condition.
-// Whether any element can raise a flag.  Int32 // and % by a constant other
-// than 0 and -1 cannot (the divisor is known when the program is built, so
-// neither division by zero nor the MIN / -1 overflow can occur): a
-// coordinate computed from the element index then costs no read-back.
-bool prog_may_flag(const ph_prog *p) {
-  bool isc[PH_NREG] = {false};
-  int64_t cv[PH_NREG] = {0};
-  for (int k = 0; k < p->n_insn; ++k) {
-    const ph_insn &in = p->insn[k];
-    if (in.op == PH_OP_IPOW || in.op == PH_OP_WIPOW) return true;
-    if (in.dt == PH_I32 && (in.op == PH_OP_FLOORDIV || in.op == PH_OP_MOD)) {
-      const bool safe = in.b < PH_NREG && isc[in.b] && cv[in.b] != 0 && cv[in.b] != -1;
-      if (!safe) return true;
-    }
-    if (in.dt == PH_I32 && (in.op == PH_OP_ADD || in.op == PH_OP_SUB || in.op == PH_OP_MUL ||
-                            in.op == PH_OP_NEG || in.op == PH_OP_ABS))
-      return true;
-    if (in.dst < PH_NREG) {
-      isc[in.dst] = in.op == PH_OP_CONST && in.dt == PH_I32;
-      cv[in.dst] = in.imm.i;
-    }
-  }
-  return false;
 }
 
 // Compiled fused kernels, keyed by everything their source depends on (the
@@ -2100,6 +2110,12 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   int rc = validate_prog(prog, n_in, in_dt, kind == 0);
   if (rc) return rc;
   if (rows < 1 || cols < 1) return ph::set_error(PH_E_ARGUMENT, "empty shape");
+  if (kind == 5) {
+    // the general batch's kernels: inputs are the element and the scalar
+    if (n_in != 2 || in_dt[0] != in_dt[1] || (in_dt[0] != PH_I32 && in_dt[0] != PH_F32 && in_dt[0] != PH_F64))
+      return ph::set_error(PH_E_ARGUMENT, "kind 5 takes two inputs of one dtype (I32, F32 or F64)");
+    return ph::gbatch_compile_check(prog, in_dt[0]);
+  }
   if (kind == 3 || kind == 4) {
     // the kernels of a map whose input 0 reads the output's buffer
     // transposed (3), or of a copy reversing a buffer onto itself (4)

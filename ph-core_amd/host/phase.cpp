@@ -2689,4 +2689,124 @@ std::vector<double> Batch::sums() const {
   return h;
 }
 
+GBatch::GBatch(DType x_dtype_, DType y_dtype_, const std::vector<Expr> &progs, const std::vector<GBatchRecord> &recs)
+    : x_dtype(x_dtype_), y_dtype(y_dtype_), records(recs) {
+  init();
+  if (records.size() > (size_t)INT32_MAX) throw ArgumentError("too many arrays");
+  std::vector<ph_prog> ps;
+  for (const Expr &e : progs) ps.push_back(e.compile(2));
+  std::vector<ph_batch_rec> rs(records.size());
+  for (size_t k = 0; k < records.size(); ++k) {
+    const GBatchRecord &g = records[k];
+    ph_batch_rec &r = rs[k];
+    std::memset(&r, 0, sizeof(r));
+    if (g.shape.empty() || g.shape.size() > 3) throw DimensionError("batched arrays have rank 1, 2 or 3");
+    for (int64_t d : g.shape)
+      if (d < 0) throw DimensionError("Cannot create NArray: One or more of the provided dimensions was negative.");
+    r.rank = (int32_t)g.shape.size();
+    for (size_t i = 0; i < g.shape.size(); ++i) r.shape[i] = g.shape[i];
+    r.map = g.map;
+    r.red = g.red;
+    r.red_ops = g.red >= 0 ? g.red_ops : 0;
+    r.scalar = g.scalar;
+    Shape zs;
+    if (g.slice) {
+      // the region as `view` canonicalises it (IndexRegion.new(literal, shape))
+      const IndexRegion ir = IndexRegion::from_literal(g.region, g.shape);
+      r.slice = 1;
+      for (size_t i = 0; i < g.shape.size(); ++i) {
+        r.first[i] = ir.first[i];
+        r.step[i] = ir.step[i];
+        r.count[i] = ir.proper_shape[i];
+      }
+      zs = ir.reduced_shape;
+    }
+    z_shapes.push_back(zs);
+  }
+  check(ph_gbatch_create(&handle_, (int32_t)x_dtype, (int32_t)y_dtype, (int32_t)ps.size(), ps.data(),
+                         (int32_t)rs.size(), rs.data()),
+        "batch");
+  int64_t nx = 0, nz = 0;
+  offsets_xy.resize(records.size());
+  offsets_z.resize(records.size());
+  check(ph_gbatch_sizes(handle_, &nx, &nz, offsets_xy.data(), offsets_z.data()), "batch sizes");
+  x = NArray::empty({nx}, x_dtype);
+  y = NArray::empty({nx}, y_dtype);
+  z = NArray::empty({nz}, x_dtype);
+  sums_dev = std::make_unique<DeviceDoubles>(std::max<int64_t>(count(), 1));
+  maxes_dev = std::make_unique<DeviceDoubles>(std::max<int64_t>(count(), 1));
+}
+
+GBatch::~GBatch() {
+  if (handle_) ph_gbatch_destroy(handle_);
+}
+
+NArray GBatch::piece(const NArray &flat, DType dt, int64_t off, const Shape &shape) const {
+  NArray r = NArray::empty(shape, dt);
+  const int64_t n = r.size();
+  const size_t es = dtype_size(dt);
+  if (n)
+    check(ph_d2d(r.device_ptr(), (const char *)flat.storage->dev_read() + (size_t)off * es, (size_t)n * es),
+          "batch element copy");
+  return r;
+}
+
+NArray GBatch::x_at(int64_t k) const {
+  check_k(k, count());
+  return piece(x, x_dtype, offsets_xy[(size_t)k], records[(size_t)k].shape);
+}
+NArray GBatch::y_at(int64_t k) const {
+  check_k(k, count());
+  return piece(y, y_dtype, offsets_xy[(size_t)k], records[(size_t)k].shape);
+}
+NArray GBatch::z_at(int64_t k) const {
+  check_k(k, count());
+  if (!records[(size_t)k].slice) throw ArgumentError("batch record has no region");
+  return piece(z, x_dtype, offsets_z[(size_t)k], z_shapes[(size_t)k]);
+}
+
+void GBatch::set_x(const NArray &flat) {
+  if (flat.dtype() != x_dtype || flat.size() != x.size()) throw ArgumentError("set_x: wrong dtype or size");
+  if (x.size())
+    check(ph_d2d(x.storage->dev_write(), flat.storage->dev_read(), (size_t)x.size() * dtype_size(x_dtype)),
+          "batch set_x");
+}
+
+void GBatch::prepare_run() {
+  x.storage->dev_read();
+  y.storage->dev_write();
+  z.storage->dev_write();
+}
+
+void GBatch::run(int mode) {
+  prepare_run();
+  int32_t flags = 0;
+  check(ph_gbatch_run(handle_, x.storage->dev_read(), y.storage->dev_write(), z.storage->dev_write(), sums_dev->ptr,
+                      maxes_dev->ptr, mode, &flags),
+        "batch run");
+  check_flags(flags, "batch");
+}
+
+std::tuple<int64_t, int, int> GBatch::tune() {
+  int64_t chunk = 0;
+  int32_t grid = 0, dynamic = 0;
+  prepare_run();
+  check(ph_gbatch_tune(handle_, x.storage->dev_read(), y.storage->dev_write(), z.storage->dev_write(), sums_dev->ptr,
+                       maxes_dev->ptr, &chunk, &grid, &dynamic),
+        "batch tune");
+  return {chunk, (int)grid, (int)dynamic};
+}
+
+std::vector<double> GBatch::sums() const {
+  std::vector<double> h = sums_dev->to_host();
+  h.resize((size_t)count());
+  return h;
+}
+
+std::vector<double> GBatch::maxes() const {
+  std::vector<double> h = maxes_dev->to_host();
+  h.resize((size_t)count());
+  return h;
+}
+
 }  // namespace phase

@@ -532,6 +532,60 @@ class Batch {
   NArray piece(const NArray &flat, int64_t off, const Shape &shape) const;
 };
 
+// One record of a general batch (ph_gbatch_*): array k's shape, the indices
+// of its map program (results into y) and reduction program (its sum and /
+// or maximum), the region copied into z (range literals, as `view` takes
+// them, canonicalised against the shape), and its scalar (program input 1).
+struct GBatchRecord {
+  Shape shape;
+  int map = -1;
+  int red = -1;
+  int red_ops = 0;  // PH_RED_SUM | PH_RED_MAX
+  bool slice = false;
+  RegionLiteral region;
+  double scalar = 0.0;
+};
+
+// A general batch of small arrays packed into one device buffer: for each
+// record the work `x_k.map { |v| map(v, s_k) }`, `x_k.map { |v| red(v, s_k)
+// }.sum / .max` and `x_k.view(region).to_narr` a host loop would do, in one
+// persistent launch (mode 0 replays a HIP graph; 1 launches directly; 2
+// launches the kernel once per array; 3 runs each record through the
+// unbatched entry points).  Programs are element
+// expressions over input 0 (the element, x's dtype) and input 1 (the
+// record's scalar, x's dtype).
+class GBatch {
+ public:
+  GBatch(DType x_dtype, DType y_dtype, const std::vector<Expr> &progs, const std::vector<GBatchRecord> &records);
+  ~GBatch();
+  GBatch(const GBatch &) = delete;
+  GBatch &operator=(const GBatch &) = delete;
+
+  int64_t count() const { return (int64_t)records.size(); }
+  NArray x_at(int64_t k) const;  // copies of x_k, y_k, z_k
+  NArray y_at(int64_t k) const;
+  NArray z_at(int64_t k) const;
+  // copies a flat array of x's dtype and size (pads included) into x
+  void set_x(const NArray &flat);
+  void run(int mode = 0);
+  // (work-item elements, workgroups, queue 1 / static 0)
+  std::tuple<int64_t, int, int> tune();
+  std::vector<double> sums() const;
+  std::vector<double> maxes() const;
+
+  DType x_dtype, y_dtype;
+  std::vector<GBatchRecord> records;
+  std::vector<Shape> z_shapes;  // the regions' shapes (degenerate axes dropped)
+  std::vector<int64_t> offsets_xy, offsets_z;
+  NArray x, y, z;  // packed, flat
+  std::unique_ptr<DeviceDoubles> sums_dev, maxes_dev;
+
+ private:
+  void *handle_ = nullptr;
+  void prepare_run();
+  NArray piece(const NArray &flat, DType dt, int64_t off, const Shape &shape) const;
+};
+
 // Serialisation (n_array.cr:807-911): {"shape": [...], "elements": [...]}
 // as JSON and as YAML, elements in buffer order, each as its type's to_s.
 std::string to_json(const NArray &a);

@@ -668,4 +668,63 @@ PYBIND11_MODULE(_phase, m) {
       .def_property_readonly("x", [](const Batch &b) { return b.x; })
       .def_property_readonly("y", [](const Batch &b) { return b.y; })
       .def_property_readonly("z", [](const Batch &b) { return b.z; });
+
+  // progs: callables (v, s) -> expression over the element and the scalar;
+  // records: dicts with "shape", and optionally "map", "red" (program
+  // indices), "ops" ("sum", "max" or both, as a string or a list), "region"
+  // (range literals, as `view` takes them) and "scalar".
+  py::class_<GBatch>(m, "GBatch")
+      .def(py::init([](DType xdt, DType ydt, const py::list &progs, const py::list &records) {
+             std::vector<Expr> ps;
+             for (auto f : progs) {
+               if (py::isinstance<Expr>(f)) {
+                 ps.push_back(f.cast<Expr>());
+                 continue;
+               }
+               py::object r = f(py::cast(Expr::input(0, xdt)), py::cast(Expr::input(1, xdt)));
+               ps.push_back(as_expr(r));
+             }
+             std::vector<GBatchRecord> rs;
+             for (auto item : records) {
+               py::dict d = item.cast<py::dict>();
+               GBatchRecord g;
+               g.shape = d["shape"].cast<Shape>();
+               if (d.contains("map") && !d["map"].is_none()) g.map = d["map"].cast<int>();
+               if (d.contains("red") && !d["red"].is_none()) g.red = d["red"].cast<int>();
+               if (d.contains("ops")) {
+                 py::object o = d["ops"];
+                 std::vector<std::string> names;
+                 if (py::isinstance<py::str>(o)) names.push_back(o.cast<std::string>());
+                 else names = o.cast<std::vector<std::string>>();
+                 for (const std::string &n : names) {
+                   if (n == "sum") g.red_ops |= PH_RED_SUM;
+                   else if (n == "max") g.red_ops |= PH_RED_MAX;
+                   else throw ArgumentError("batch reduction " + n + " (sum or max)");
+                 }
+               }
+               if (d.contains("region") && !d["region"].is_none()) {
+                 g.slice = true;
+                 g.region = to_region(d["region"]);
+               }
+               if (d.contains("scalar")) g.scalar = d["scalar"].cast<double>();
+               rs.push_back(g);
+             }
+             return std::make_unique<GBatch>(xdt, ydt, ps, rs);
+           }),
+           py::arg("x_dtype"), py::arg("y_dtype"), py::arg("progs"), py::arg("records"))
+      .def("__len__", &GBatch::count)
+      .def("x_at", &GBatch::x_at)
+      .def("y_at", &GBatch::y_at)
+      .def("z_at", &GBatch::z_at)
+      .def("set_x", &GBatch::set_x)
+      .def("run", &GBatch::run, py::arg("mode") = 0)
+      .def("tune", &GBatch::tune)
+      .def("sums", &GBatch::sums)
+      .def("maxes", &GBatch::maxes)
+      .def_readonly("offsets_xy", &GBatch::offsets_xy)
+      .def_readonly("offsets_z", &GBatch::offsets_z)
+      .def_readonly("z_shapes", &GBatch::z_shapes)
+      .def_property_readonly("x", [](const GBatch &b) { return b.x; })
+      .def_property_readonly("y", [](const GBatch &b) { return b.y; })
+      .def_property_readonly("z", [](const GBatch &b) { return b.z; });
 }

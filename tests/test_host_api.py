@@ -300,3 +300,22 @@ def test_float_text_layout(host):
     rng = np.random.default_rng(3)
     for v in rng.standard_normal(2000) * 10.0 ** rng.integers(-30, 30, 2000):
         assert float(host.float_to_s(float(v)).replace("e+", "e")) == v  # reads back exactly
+
+
+def test_general_batch_programs_compile_for_gfx950(host):
+    """The general batch's persistent kernel is generated and compiled for
+    programs over Int32, Float32 and Float64 elements (map into Int32,
+    Float32, Float64 and Bool results; float results also as reductions);
+    other input layouts are refused."""
+    D = host.DType
+    for xd in (D.F32, D.F64, D.I32):
+        v, s = host.input(0, xd), host.input(1, xd)
+        progs = [v + s, v * 2.0 - s, v > s]
+        if xd == D.I32:
+            progs += [v.wadd(s), v // s]
+        for p in progs:
+            host.compile_check(p, [xd, xd], 1, 1, 5)
+    with pytest.raises(host.ArgumentError):
+        host.compile_check(host.input(0, D.F32), [D.F32], 1, 1, 5)
+    with pytest.raises(host.ArgumentError):
+        host.compile_check(host.input(0, D.BF16) + host.input(1, D.BF16), [D.BF16, D.BF16], 1, 1, 5)
