@@ -620,7 +620,8 @@ def path_ops(phase, ev, args):
     x = phase.NArray.random([n, n], D.I32, seed=1, dist="int31")
     y = phase.NArray.random([n, n], D.I32, seed=5, dist="int31")
     z = phase.NArray.random([n, n], D.I32, seed=6, dist="int31")
-    mask = x.process(lambda v: (v & 1) == 0).to_narr()
+    # the mask is an input of the two mask assignments: computed once, here
+    mask = x.process(lambda v: (v & 1) == 0).to_narr().materialize()
     res = {}
 
     def entry(name, kernel, fn, by, cpu=None):
