@@ -422,10 +422,12 @@ int ph_batch_destroy(void *handle);
  *   map >= 0    y_k = progs[map] applied to every element, as
  *               `x_k.map { |v| ... }` (src/n_array.cr:589-595); y has y_dtype
  *               (I32, F32, F64 or Bool), the programs' result type;
- *   red >= 0    v = progs[red] applied to every element (F32 or F64); with
- *               red_ops bit PH_RED_SUM sums[k] = v.sum, with PH_RED_MAX
+ *   red >= 0    v = progs[red] applied to every element (I32, F32 or F64);
+ *               with red_ops bit PH_RED_SUM sums[k] = v.sum, with PH_RED_MAX
  *               maxes[k] = v.max (Enumerable#sum / #max over `each`,
- *               src/multi_indexable.cr:18, :656-658);
+ *               src/multi_indexable.cr:18, :656-658); an Int32 sum is exact
+ *               and checked like Crystal's `+`: a running total outside
+ *               Int32 raises PH_FLAG_OVERFLOW;
  *   slice != 0  z_k = x_k.view(region).to_narr, the region given per axis as
  *               first, step (non-zero; a negative step runs the axis
  *               backwards) and count (src/multi_indexable.cr:881-887,
@@ -495,8 +497,8 @@ int ph_gbatch_destroy(void *handle);
  * kernels of a map whose input 0 reads the output's own buffer transposed
  * (rows == cols; input 0 has the output's dtype), kind 4 those of a copy
  * of in_dtypes[0] elements that reverses a buffer onto itself, kind 5 the
- * general batch's kernels with `prog` as map and (for a float result)
- * reduction program (two inputs of one dtype: the element and the scalar;
+ * general batch's kernels with `prog` as map and (for an I32, F32 or F64
+ * result) reduction program (two inputs of one dtype: the element and the scalar;
  * rows and cols unused).  It needs no GPU, so program lowering can be
  * checked on a build host.
  * ------------------------------------------------------------------------- */

@@ -22,7 +22,13 @@
 // ran which item.  Maxima carry (value, first index): the larger value wins,
 // equal values keep the lower index, and a NaN counts only at index 0, where
 // it sticks; that state combines in any order and gives exactly the result
-// of Enumerable#max's `>` loop (src/multi_indexable.cr:18).
+// of Enumerable#max's `>` loop (src/multi_indexable.cr:18).  Int32 values
+// are summed exactly, with Crystal's checked `+`: a step's elements are taken
+// in lexicographic order (vector m, then lane, then element), so a prefix scan
+// across the wavefront per vector gives every running total; an item keeps
+// its total and its lowest and highest running total, and the fold kernel
+// chains the items in order and raises the overflow flag when a running
+// total leaves Int32, as the sequential loop would.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -81,6 +87,7 @@ struct GArgs {
   const int *first; const int *rops;
   double *sums; double *maxes;
   int *counter;
+  long long *pint;
   int nitems, dynamic, n, pad_;
 };
 __device__ __forceinline__ unsigned fdiv(unsigned n, unsigned m, unsigned l) {
@@ -131,6 +138,7 @@ struct GArgsHost {
   double *sums;
   double *maxes;
   int *counter;
+  long long *pint;
   int nitems, dynamic, n, pad_;
 };
 
@@ -165,6 +173,8 @@ struct GBatch {
   int32_t *d_first = nullptr, *d_rops = nullptr, *d_counter = nullptr;
   double *d_psum = nullptr, *d_pmv = nullptr;
   unsigned *d_pmi = nullptr;
+  long long *d_pint = nullptr;  // Int32 sums: per item total, lowest and highest running total
+  bool any_isum = false;
   void *d_scalars = nullptr;  // per record, x dtype (the unbatched form's scalar operand)
   int64_t nitems = 0, items_cap = 0;
   int64_t chunk = 16384;
@@ -222,6 +232,7 @@ std::string item_source(const GBatch &B, int c, int vec, int msteps) {
   const char *XV = vec_type(B.xdt);
   const bool ysame = C.map >= 0 && B.ydt != PH_BOOL && ph::dtype_size(B.ydt) == ph::dtype_size(B.xdt);
   const bool red = C.red >= 0;
+  const bool ired = red && B.progs[(size_t)C.red].out_dtype == PH_I32;
   // DS / DM: the item sums / maximises (R.ops, dispatched once per item)
   std::string s = fmt("template <bool DS, bool DM>\n__device__ __forceinline__ void item_%d(const GArgs &A, const GRec &R, "
                       "const int lane, %s *sx) {\n", c, XS);
@@ -231,8 +242,11 @@ std::string item_source(const GBatch &B, int c, int vec, int msteps) {
     s += fmt("  %s *__restrict__ za = (%s *)A.z + R.zoff;\n  unsigned zlo = R.zlo;\n  const bool sl = R.zlo < R.zhi;\n",
              XS, XS);
   s += fmt("  const %s sc = (%s)R.scalar;\n  int fl = 0;\n", XC, XC);
-  if (red)
+  if (red && !ired)
     s += "  double gs = 0.0;\n  double mv = -__builtin_inf(); unsigned mi = 0xffffffffu;\n";
+  if (ired)
+    s += "  long long it = 0, ilo = 0, ihi = 0;  // total so far; lowest / highest running total\n"
+         "  int imx = -2147483647 - 1;\n";
   s += fmt("  for (unsigned base = R.lo; base < R.hi; base += %du) {\n", sub);
   s += fmt("    const unsigned end = min(R.hi, base + %du);\n", sub);
   // lanes past the end load the last whole vector inside the array (its
@@ -250,6 +264,7 @@ std::string item_source(const GBatch &B, int c, int vec, int msteps) {
   for (int j = 0; j < vec; ++j) s += fmt("%sv[m].%c", j ? ", " : "", "xyzw"[j]);
   s += "};\n";
   if (ysame) s += fmt("      %s yo[%d];\n", YS, vec);
+  if (ired) s += fmt("      long long iv[%d];\n", vec);
   s += fmt("#pragma unroll\n      for (int j = 0; j < %d; ++j) {\n", vec);
   s += "        const unsigned e = e0 + j;\n        const bool ok = e < end;\n";
   s += fmt("        const %s xe = (%s)xs_[j];\n", XC, XC);
@@ -267,12 +282,31 @@ std::string item_source(const GBatch &B, int c, int vec, int msteps) {
     std::string body;
     const std::string r = emit_prog(&B.progs[(size_t)C.red], {"xe", "sc"}, &body, "_r");
     s += "        const int flr_ = fl;\n" + body + "        if (!ok) fl = flr_;\n";
-    s += fmt("        const double rv = (double)(%s);\n", r.c_str());
-    s += "        if (DS) gs += ok ? rv : 0.0;\n";
-    s += "        if (DM && ok) {\n          if (rv != rv) { if (e == 0u) { mv = rv; mi = 0u; } }\n"
-         "          else if (rv > mv || (rv == mv && e < mi)) { mv = rv; mi = e; }\n        }\n";
+    if (ired) {
+      s += fmt("        iv[j] = ok ? (long long)(%s) : 0LL;\n", r.c_str());
+      s += fmt("        if (DM && ok) imx = imx > (%s) ? imx : (%s);\n", r.c_str(), r.c_str());
+    } else {
+      s += fmt("        const double rv = (double)(%s);\n", r.c_str());
+      s += "        if (DS) gs += ok ? rv : 0.0;\n";
+      s += "        if (DM && ok) {\n          if (rv != rv) { if (e == 0u) { mv = rv; mi = 0u; } }\n"
+           "          else if (rv > mv || (rv == mv && e < mi)) { mv = rv; mi = e; }\n        }\n";
+    }
   }
   s += "      }\n";
+  if (ired) {
+    // running totals of this vector's elements: the lane's own prefix sums
+    // plus the wavefront's exclusive prefix of lane totals plus the item's
+    // total so far (elements past the end add 0)
+    s += fmt("      if (DS) {\n        long long pf = 0, pmin = 0, pmax = 0;\n#pragma unroll\n        for (int j = 0; j < %d; ++j) {\n"
+             "          pf += iv[j];\n          pmin = pf < pmin ? pf : pmin;\n          pmax = pf > pmax ? pf : pmax;\n        }\n",
+             vec);
+    s += "        long long inc = pf;\n"
+         "        for (int d = 1; d < 64; d <<= 1) {\n          const long long t = __shfl_up(inc, d, 64);\n"
+         "          if (lane >= d) inc += t;\n        }\n"
+         "        const long long ex = it + inc - pf;\n"
+         "        ilo = ex + pmin < ilo ? ex + pmin : ilo;\n        ihi = ex + pmax > ihi ? ex + pmax : ihi;\n"
+         "        it += __shfl(inc, 63, 64);\n      }\n";
+  }
   if (ysame) {
     const char *YV = vec_type(B.ydt);
     s += fmt("      if (e0 + %du <= end) {\n        %s yw;\n", vec, YV);
@@ -300,7 +334,14 @@ std::string item_source(const GBatch &B, int c, int vec, int msteps) {
     s += "    zlo = zhi;\n    __builtin_amdgcn_wave_barrier();  // the next step overwrites the LDS copy\n    }\n";
   }
   s += "  }\n";
-  if (red) {
+  if (ired) {
+    s += "  if (DS) {\n    for (int o = 32; o > 0; o >>= 1) {\n"
+         "      const long long a2 = __shfl_xor(ilo, o, 64), b2 = __shfl_xor(ihi, o, 64);\n"
+         "      ilo = a2 < ilo ? a2 : ilo;\n      ihi = b2 > ihi ? b2 : ihi;\n    }\n"
+         "    if (lane == 0) { A.pint[3 * R.slot] = it; A.pint[3 * R.slot + 1] = ilo; A.pint[3 * R.slot + 2] = ihi; }\n  }\n";
+    s += "  if (DM) {\n    for (int o = 32; o > 0; o >>= 1) {\n      const int b2 = __shfl_xor(imx, o, 64);\n"
+         "      imx = b2 > imx ? b2 : imx;\n    }\n    if (lane == 0) A.pmv[R.slot] = (double)imx;\n  }\n";
+  } else if (red) {
     s += "  if (DS) {\n    for (int o = 32; o > 0; o >>= 1) gs += __shfl_xor(gs, o, 64);\n"
          "    if (lane == 0) A.psum[R.slot] = gs;\n  }\n";
     s += "  if (DM) {\n    for (int o = 32; o > 0; o >>= 1) {\n      const double v2 = __shfl_xor(mv, o, 64);\n"
@@ -346,9 +387,17 @@ std::string batch_source(const GBatch &B) {
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_gbatch_final(GArgs A) {\n";
   s += "  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;\n  if (k >= A.n) return;\n";
   s += "  const int ops = A.rops[k];\n  const int b = A.first[k], e = A.first[k + 1];\n";
-  s += "  if ((ops & 1) && A.sums) {\n    double t = 0.0;\n    for (int i = b; i < e; ++i) t += A.psum[i];\n"
+  s += "  if ((ops & 5) == 5 && A.sums) {  // Int32: checked running totals, items in order\n"
+       "    long long t = 0; bool ov = false;\n"
+       "    for (int i = b; i < e; ++i) {\n"
+       "      if (t + A.pint[3 * i + 1] < -2147483648LL || t + A.pint[3 * i + 2] > 2147483647LL) ov = true;\n"
+       "      t += A.pint[3 * i];\n    }\n"
+       "    A.sums[k] = (double)t;\n    if (ov) atomicOr(A.counter + 1, 1);\n"
+       "  } else if ((ops & 1) && A.sums) {\n    double t = 0.0;\n    for (int i = b; i < e; ++i) t += A.psum[i];\n"
        "    A.sums[k] = t;\n  }\n";
-  s += "  if ((ops & 2) && A.maxes && e > b) {\n    double v = A.pmv[b]; unsigned ix = A.pmi[b];\n"
+  s += "  if ((ops & 6) == 6 && A.maxes && e > b) {\n    double v = A.pmv[b];\n"
+       "    for (int i = b + 1; i < e; ++i) v = A.pmv[i] > v ? A.pmv[i] : v;\n    A.maxes[k] = v;\n"
+       "  } else if ((ops & 2) && A.maxes && e > b) {\n    double v = A.pmv[b]; unsigned ix = A.pmi[b];\n"
        "    for (int i = b + 1; i < e; ++i) g_max_comb(v, ix, A.pmv[i], A.pmi[i]);\n    A.maxes[k] = v;\n  }\n}\n";
   s += "extern \"C\" __global__ void ph_gbatch_reset(GArgs A) {\n"
        "  if (threadIdx.x == 0 && blockIdx.x == 0) { A.counter[0] = 0; A.counter[1] = 0; }\n}\n";
@@ -420,6 +469,8 @@ int build_items(GBatch *B, int64_t chunk) {
     (void)hipFree(B->d_psum);
     (void)hipFree(B->d_pmv);
     (void)hipFree(B->d_pmi);
+    (void)hipFree(B->d_pint);
+    B->d_pint = nullptr;
     B->d_recs = B->d_recs_am = nullptr;
     B->d_psum = B->d_pmv = nullptr;
     B->d_pmi = nullptr;
@@ -428,7 +479,8 @@ int build_items(GBatch *B, int64_t chunk) {
         hipMalloc(&B->d_recs_am, sizeof(GRec) * (size_t)ni) != hipSuccess ||
         hipMalloc(&B->d_psum, sizeof(double) * (size_t)ni) != hipSuccess ||
         hipMalloc(&B->d_pmv, sizeof(double) * (size_t)ni) != hipSuccess ||
-        hipMalloc(&B->d_pmi, sizeof(unsigned) * (size_t)ni) != hipSuccess)
+        hipMalloc(&B->d_pmi, sizeof(unsigned) * (size_t)ni) != hipSuccess ||
+        (B->any_isum && hipMalloc(&B->d_pint, 3 * sizeof(long long) * (size_t)ni) != hipSuccess))
       return ph::set_error(PH_E_OOM, "batch work items (%lld)", (long long)ni);
     B->items_cap = ni;
   }
@@ -498,6 +550,7 @@ int bind(GBatch *B, const void *x, void *y, void *z, double *sums, double *maxes
   a.sums = sums;
   a.maxes = maxes;
   a.counter = B->d_counter;
+  a.pint = B->d_pint;
   a.nitems = (int)B->nitems;
   a.dynamic = B->dynamic;
   a.n = B->n;
@@ -616,7 +669,7 @@ int gbatch_compile_check(const ph_prog *prog, int x_dtype) {
   B.progs.push_back(*prog);
   B.combos.push_back(Combo{0, -1});
   B.any_slice = true;
-  if (prog->out_dtype == PH_F32 || prog->out_dtype == PH_F64) {
+  if (prog->out_dtype == PH_F32 || prog->out_dtype == PH_F64 || prog->out_dtype == PH_I32) {
     B.combos.push_back(Combo{-1, 0});
     B.combos.push_back(Combo{0, 0});
     B.any_red = true;
@@ -714,9 +767,11 @@ int ph_gbatch_create(void **handle, int32_t x_dtype, int32_t y_dtype, int32_t n_
     const int ops = R.red >= 0 ? R.red_ops : 0;
     if (R.red >= 0) {
       const int odt = progs[R.red].out_dtype;
-      if (odt != PH_F32 && odt != PH_F64)
-        return fail(ph::set_error(PH_E_UNSUPPORTED, "batch record %d: reductions of dtype %d (F32 or F64)", k, odt));
+      if (odt != PH_F32 && odt != PH_F64 && odt != PH_I32)
+        return fail(ph::set_error(PH_E_UNSUPPORTED, "batch record %d: reductions of dtype %d (I32, F32 or F64)", k,
+                                  odt));
       if (ops < 1 || ops > 3) return fail(ph::set_error(PH_E_ARGUMENT, "batch record %d: red_ops %d", k, ops));
+      if (odt == PH_I32 && (ops & PH_RED_SUM)) B->any_isum = true;
     }
     const bool slice = R.slice && a.zn > 0;
     if (R.map >= 0 || R.red >= 0 || slice) {
@@ -743,13 +798,14 @@ int ph_gbatch_create(void **handle, int32_t x_dtype, int32_t y_dtype, int32_t n_
   }
   B->total = off;
   B->ztotal = zoff;
+  B->may_flag = B->may_flag || B->any_isum;  // a checked Int32 sum can overflow
   // per-record reduction ops and scalars (the unbatched form's operand)
   std::vector<int32_t> rops((size_t)n);
   const size_t es = ph::dtype_size(x_dtype);
   std::vector<char> sc((size_t)std::max(n, 1) * es);
   for (int k = 0; k < n; ++k) {
     const ph_batch_rec &R = B->recs[(size_t)k];
-    rops[(size_t)k] = R.red >= 0 ? R.red_ops : 0;
+    rops[(size_t)k] = R.red >= 0 ? R.red_ops | (B->progs[(size_t)R.red].out_dtype == PH_I32 ? 4 : 0) : 0;
     if (x_dtype == PH_F32) {
       const float v = (float)R.scalar;
       std::memcpy(&sc[(size_t)k * es], &v, es);
@@ -911,6 +967,7 @@ int ph_gbatch_destroy(void *handle) {
   (void)hipFree(B->d_psum);
   (void)hipFree(B->d_pmv);
   (void)hipFree(B->d_pmi);
+  (void)hipFree(B->d_pint);
   (void)hipFree(B->d_scalars);
   delete B;
   return PH_OK;

@@ -241,3 +241,44 @@ def test_c5_workload_as_general_batch(ph):
         xk = xh[of:of + n].astype(np.float64)
         ex = O.sum_exact(xk)
         assert abs(sums[k] - ex) <= n * 2.0 ** -52 * float(np.abs(xk).sum()), k
+
+
+def test_int32_reductions_exact_and_checked(ph):
+    """Int32 sums are exact and checked like Crystal's `+` (a running total
+    outside Int32 raises OverflowError, even when the total ends inside it,
+    across work items too); Int32 maxima exact."""
+    rng = np.random.default_rng(211)
+    progs = [lambda v, s: v, lambda v, s: v.wmul(s) - v]
+    shapes = [[int(rng.integers(1, 60))] * int(rng.integers(1, 4)) for _ in range(150)] + [[70000], [3, 40000]]
+    recs = [{"shape": sh, "red": k % 2, "ops": ["sum", "max"], "scalar": float(rng.integers(-3, 4))}
+            for k, sh in enumerate(shapes)]
+    b = ph.GBatch(ph.DType.I32, ph.DType.I32, progs, recs)
+    h = rng.integers(-20000, 20000, b.x.size).astype(np.int32)
+    b.set_x(ph.NArray.from_numpy(h))
+    for mode in (0, 2):
+        b.run(mode)
+        sums, maxes = b.sums(), b.maxes()
+        for k, rec in enumerate(recs):
+            n = int(np.prod(rec["shape"]))
+            xk = h[b.offsets_xy[k]:b.offsets_xy[k] + n].astype(np.int64)
+            s = int(rec["scalar"])
+            v = xk if rec["red"] == 0 else (xk * s).astype(np.int32).astype(np.int64) - xk
+            assert sums[k] == float(v.sum()), (mode, k)
+            assert maxes[k] == float(v.max()), (mode, k)
+    big = 2 ** 31 - 1
+    cases = [([big, -1, 1], False), ([big, 1, -5], True), ([2 ** 30, 2 ** 30, -2 ** 30, -2 ** 30], True),
+             ([-2 ** 31, 0, -1], True), ([-2 ** 31, 5, -5], False), ([60000] * 40000, True),
+             ([60000] * 35000 + [-60000] * 5000, False)]
+    for vals, overflow in cases:
+        recs = [{"shape": [len(vals)], "red": 0, "ops": "sum"}]
+        b = ph.GBatch(ph.DType.I32, ph.DType.I32, [lambda v, s: v], recs)
+        h = np.zeros(b.x.size, np.int32)
+        h[:len(vals)] = vals
+        b.set_x(ph.NArray.from_numpy(h))
+        for mode in (0, 3):
+            if overflow:
+                with pytest.raises(ph.OverflowError):
+                    b.run(mode)
+            else:
+                b.run(mode)
+                assert b.sums()[0] == float(sum(vals)), (vals[:3], mode)
