@@ -325,6 +325,22 @@ def main():
     # per-kernel durations (HIP events around back-to-back launches)
     g = view.to_narr()
     ms_gather = time_loop(ev, lambda: view.to_narr(), 20, phase)
+    # the gather re-reads one third of x (about 134 MB of lines), which the
+    # 256 MB MALL may partly hold between back-to-back launches: also time
+    # each launch alone after reading a 512 MiB array (a MALL flush; a
+    # written one would leave dirty lines whose write-back the gather pays)
+    flush = phase.NArray.fill([1 << 27], 0).materialize()
+    fa, fb = ev.new(), ev.new()
+    tot = 0.0
+    for _ in range(10):
+        flush.sum()
+        ev.record(fa)
+        view.to_narr()
+        ev.record(fb)
+        ev.sync()
+        tot += ev.ms(fa, fb)
+    ms_gather_flushed = tot / 10
+    del flush
 
     # fill + scatter: 8 back-to-back launches, each into a fresh pending fill
     zs = [phase.NArray.fill([n, n], 0) for _ in range(8)]
@@ -349,7 +365,11 @@ def main():
     extra = {"gather": {"kernel": "k_copy_rows<uint32_t,8,int,DCONT> (strided gather)", "achieved_gbs": round(achieved_g, 1),
                         "frac": round(achieved_g / PEAK_HBM_GBS, 4), "ms_per_launch": round(ms_gather, 4),
                         "algorithmic_bytes": gather_bytes,
-                        "traffic": traffic.get("k_copy_rows") if traffic else None}}
+                        "traffic": traffic.get("k_copy_rows") if traffic else None,
+                        "mall_flushed": {"ms_per_launch": round(ms_gather_flushed, 4),
+                                         "achieved_gbs": round(gather_bytes / ms_gather_flushed / 1e6, 1),
+                                         "note": "each launch alone, after 512 MiB read (event timing "
+                                                 "includes one launch's ramp-up)"}}}
     del zs, mvs, g
 
     # plain contiguous copy of 256 MiB (Int32): the practical copy ceiling of
