@@ -285,6 +285,46 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins,
            const ph_view_desc *out_desc, int32_t *flags_out);
 
 /* ---------------------------------------------------------------------------
+ * Buffer element index of every coordinate of a view, lexicographic:
+ *     out[k] = d(k)      (device int64 array of desc_count(d) entries)
+ * ComposedTransform#apply (src/view_util/transforms.cr:28-101) for the part
+ * of a view chain that one descriptor folds.  A chain longer than a
+ * descriptor holds composes these arrays: the index array of its first
+ * descriptor, gathered (ph_strided_copy of 8-byte elements) through the
+ * descriptors of the rest, is the chain's own index array.
+ * ------------------------------------------------------------------------- */
+int ph_view_index(int64_t *out, const ph_view_desc *d);
+
+/* ---------------------------------------------------------------------------
+ * ph_map whose operands may be addressed through index arrays (views whose
+ * transform chain is longer than one descriptor holds, src/view_util/
+ * transforms.cr:28-101).  For element k (lexicographic over out_desc's
+ * shape):
+ *   input j reads  ins[j][in_index[j][k]]  when in_index && in_index[j],
+ *                  ins[j][in_descs[j](k)]  otherwise;
+ *   the result goes to out[out_index[k]] when out_index, else
+ *   out[out_desc(k)].
+ * Index arrays are device int64 arrays of desc_count(out_desc) entries.  An
+ * indexed operand's descriptor gives its dtype, buffer extent and logical
+ * shape; its offset, strides and stages are ignored.  prog == NULL with
+ * n_in == 1 copies input 0 (out's dtype or one of its size, bits unchanged).
+ * Every index must lie in [0, extent) (PH_E_INDEX, nothing written) and
+ * out_index must name each element at most once (PH_E_ARGUMENT, nothing
+ * written; a MutableView chain is one-to-one).  Inputs on the output's
+ * buffer are read in the reference's element order, as in ph_map: element
+ * k sees the writes of the elements before it.
+ * Replaces NArray#unsafe_set_chunk(region, src) (src/n_array.cr:484-492)
+ * and NArray#[]=(mask, value) (:523-536) where the source, the mask or the
+ * destination (a MutableView, src/mutable_view.cr:16-18) is such a view of
+ * the array being written.  Scratch: 8 bytes per element of out's buffer
+ * when out_index is given (the inverse of the write map).
+ * ------------------------------------------------------------------------- */
+int ph_map_indexed(const ph_prog *prog, int32_t n_in, const void *const *ins,
+                   const ph_view_desc *in_descs, const int64_t *const *in_index,
+                   void *out, const ph_view_desc *out_desc,
+                   const int64_t *out_index, int32_t *flags_out);
+
+/* ---------------------------------------------------------------------------
  * Fused map + reduction of prog's value over a view of rank >= 1.
  * keep_axis = -1: one result (Enumerable#sum / #max over `each`,
  *   src/multi_indexable.cr:18, :656-658).
@@ -499,8 +539,10 @@ int ph_gbatch_destroy(void *handle);
  * of in_dtypes[0] elements that reverses a buffer onto itself, kind 5 the
  * general batch's kernels with `prog` as map and (for an I32, F32 or F64
  * result) reduction program (two inputs of one dtype: the element and the scalar;
- * rows and cols unused).  It needs no GPU, so program lowering can be
- * checked on a build host.
+ * rows and cols unused), kind 6 those of ph_map_indexed with input 0 (of
+ * the output's dtype) read from the output's buffer through an index array
+ * and the output written through one.  It needs no GPU, so program
+ * lowering can be checked on a build host.
  * ------------------------------------------------------------------------- */
 int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes,
                      int64_t rows, int64_t cols, int32_t kind);

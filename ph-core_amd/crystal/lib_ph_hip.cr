@@ -89,6 +89,10 @@ lib LibPhHip
                                      src : Void*, sd : ViewDesc*, fill : Void*) : Int32
   fun map = ph_map(prog : Prog*, n_in : Int32, ins : Void**, in_descs : ViewDesc*, out : Void*,
                    out_desc : ViewDesc*, flags_out : Int32*) : Int32
+  fun view_index = ph_view_index(out : Int64*, d : ViewDesc*) : Int32
+  fun map_indexed = ph_map_indexed(prog : Prog*, n_in : Int32, ins : Void**, in_descs : ViewDesc*,
+                                   in_index : Int64**, out : Void*, out_desc : ViewDesc*,
+                                   out_index : Int64*, flags_out : Int32*) : Int32
   fun reduce = ph_reduce(prog : Prog*, n_in : Int32, ins : Void**, in_descs : ViewDesc*,
                          keep_axis : Int32, ops : Int32, acc : Int32, out_sum : Float64*,
                          out_max : Float64*, flags_out : Int32*) : Int32

@@ -26,7 +26,8 @@ int device();
 // Lazily grown device scratch, valid until the next call that asks for more;
 // all work on it is ordered by the single library stream.  Slots: 0-1
 // gathered map and reduction inputs, 2 reduction and heat partials, 3 flag
-// words, 4-7 the buffers of maps whose inputs share the output's buffer.
+// words, 4-7 the buffers of maps whose inputs share the output's buffer,
+// 8 the inverse of an indexed map's write index.
 int scratch(size_t bytes, void **out, int slot = 0);
 
 // ph_strided_copy when source and destination share a buffer: the copy in

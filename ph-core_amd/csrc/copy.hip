@@ -161,6 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_copy_general(E *__restrict__ dst, ph
   }
 }
 
+// Buffer index of every coordinate of a view, lexicographic: out[g] = d(g).
+__global__ __launch_bounds__(kBlock) void k_view_index(int64_t *__restrict__ out, ph_view_desc d, int64_t total) {
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += (int64_t)gridDim.x * kBlock) {
+    int64_t coord[PH_MAX_RANK];
+    int64_t r = g;
+    for (int i = d.view.rank - 1; i >= 0; --i) {
+      const int64_t n = d.view.shape[i];
+      const int64_t q = r / n;
+      coord[i] = r - q * n;
+      r = q;
+    }
+    out[g] = map_index(d, coord);
+  }
+}
+
 // Transposing copy through LDS: the source is contiguous along axis A and the
 // destination along another axis B (a permuted view materialised, or a
 // contiguous array scattered into a permuted MutableView).  A workgroup owns
@@ -815,6 +830,21 @@ int ph_strided_copy(void *dst, const ph_view_desc *dd, const void *src, const ph
       return ph::set_error(PH_E_ARGUMENT, "copy source overlaps the destination through another base pointer");
   }
   return dispatch_copy(dst, dd, src, sd, nullptr);
+}
+
+int ph_view_index(int64_t *out, const ph_view_desc *d) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  int rc = ph::validate_desc(d, "view_index");
+  if (rc) return rc;
+  const int64_t total = ph::desc_count(d);
+  if (total == 0) return PH_OK;
+  if (!out) return ph::set_error(PH_E_ARGUMENT, "ph_view_index: null output");
+  rc = ph::launch_gate("view_index");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_view_index, dim3((unsigned)grid_for((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     ph::stream(), out, *d, total);
+  return ph::check_launch("k_view_index");
 }
 
 int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape, int32_t rank,

@@ -1745,7 +1745,10 @@ const char *raw_type(size_t esize) {
 }
 
 const char *kOrdPrelude = R"PH(
-struct OrdArgs { const void *in[8]; void *out; const void *tp; void *tn; const i64 *q0; i64 *q1; int *flags; i64 iter; i64 l0; i64 l1; };
+struct OrdArgs {
+  const void *in[8]; void *out; const void *tp; void *tn; const i64 *q0; i64 *q1; int *flags; i64 iter; i64 l0; i64 l1;
+  const i64 *ix[8]; const i64 *wx; i64 *inv; i64 *mm; i64 ibase; i64 ispan;
+};
 __device__ __forceinline__ bool ph_same(bool a, bool b) { return a == b; }
 __device__ __forceinline__ bool ph_same(int a, int b) { return a == b; }
 __device__ __forceinline__ bool ph_same(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
@@ -1763,8 +1766,11 @@ __device__ __forceinline__ void ph_ctl(int *w, int ctl) {
 )PH";
 
 // What the generated kernels need to know: the program (null: a copy of
-// input 0 of `esize`-byte elements), the operands and which inputs read the
-// output's buffer.
+// input 0 of `esize`-byte elements), the operands, which inputs read the
+// output's buffer, and which operands are addressed through index arrays
+// (element L of input j at ix[j][L], the output at wx[L]; W^-1 is then a
+// table over the addresses the output spans, inv[a - ibase] = the element
+// writing a, or -1).
 struct OrdSpec {
   const ph_prog *prog = nullptr;
   int n_in = 1;
@@ -1773,6 +1779,9 @@ struct OrdSpec {
   const ph_view_desc *out = nullptr;
   unsigned hazard = 0;
   size_t esize = 0;
+  unsigned indexed = 0;
+  bool out_indexed = false;
+  bool any_index() const { return indexed != 0 || out_indexed; }
 };
 
 std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
@@ -1790,6 +1799,24 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
                 : fmt("ph_ld((const %s *)%s, %s)", in_st(j).c_str(), ptr.c_str(), idx.c_str());
   };
   auto vtype = [&](int j) { return copy ? std::string(raw_type(S.esize)) : std::string(ctype(compute_dt(S.in_dt[j]))); };
+  // a_j: the address input j reads for element Lv (coordinates c0.. of Lv
+  // in scope for a descriptor)
+  auto in_addr = [&](int j, const std::string &Lv) {
+    if (S.indexed >> j & 1) return fmt("      const i64 a_%d = A.ix[%d][%s];\n", j, j, Lv.c_str());
+    return emit_addr(*S.in[j], fmt("a_%d", j), "");
+  };
+  // ok_j / lp_j: whether a_j is written by some element, and which
+  auto inverse = [&](int j) {
+    if (S.out_indexed)
+      return fmt("      const i64 lp_%d = (a_%d >= A.ibase && a_%d - A.ibase < A.ispan) ? A.inv[a_%d - A.ibase] : -1; "
+                 "const bool ok_%d = lp_%d >= 0;\n", j, j, j, j, j, j);
+    return emit_inverse(w, fmt("a_%d", j), fmt("_%d", j));
+  };
+  // w_: the address element Lv writes
+  auto out_addr = [&](const std::string &Lv) {
+    if (S.out_indexed) return fmt("      const i64 w_ = A.wx[%s];\n", Lv.c_str());
+    return emit_addr(*S.out, "w_", "");
+  };
   // element program over x0.. (a copy: x0 itself); defines `res`
   auto body = [&](std::string *code) {
     if (copy) {
@@ -1811,7 +1838,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
   for (int j = 0; j < S.n_in; ++j) {
     if (!(S.hazard >> j & 1)) continue;
     const std::string sx = fmt("_%d", j);
-    s += emit_addr(*S.in[j], "a" + sx, "") + emit_inverse(w, "a" + sx, sx);
+    s += in_addr(j, "L") + inverse(j);
     s += fmt("      if (ok%s && lp%s != L) ctl |= 1; if (ok%s && lp%s < L) ctl |= 2;\n", sx.c_str(), sx.c_str(),
              sx.c_str(), sx.c_str());
   }
@@ -1822,9 +1849,9 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
   s += loop + emit_coords(S.out->view, "L", "");
   for (int j = 0; j < S.n_in; ++j) {
     const std::string sx = fmt("_%d", j);
-    s += emit_addr(*S.in[j], "a" + sx, "");
+    s += in_addr(j, "L");
     if (S.hazard >> j & 1) {
-      s += emit_inverse(w, "a" + sx, sx);
+      s += inverse(j);
       s += fmt("      const bool dep%s = ok%s && lp%s < L; if (dep%s) ctl |= 2;\n", sx.c_str(), sx.c_str(), sx.c_str(),
                sx.c_str());
       s += fmt("      const %s x%d = (A.iter > 0 && dep%s) ? %s : %s;\n", vtype(j).c_str(), j, sx.c_str(),
@@ -1845,7 +1872,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
 
   // write: out[W(L)] = tp[L]
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_write(OrdArgs A) {\n";
-  s += loop + emit_coords(S.out->view, "L", "") + emit_addr(*S.out, "w_", "");
+  s += loop + emit_coords(S.out->view, "L", "") + out_addr("L");
   s += fmt("      ((%s *)A.out)[w_] = ((const %s *)A.tp)[L];\n    }\n}\n", OST.c_str(), OST.c_str());
 
   // seq: the loop itself, in order, over [l0, l1), one thread
@@ -1854,11 +1881,11 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
   s += "    for (i64 L = A.l0; L < A.l1; ++L) {\n" + emit_coords(S.out->view, "L", "");
   for (int j = 0; j < S.n_in; ++j) {
     const std::string sx = fmt("_%d", j);
-    s += emit_addr(*S.in[j], "a" + sx, "");
+    s += in_addr(j, "L");
     s += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), "a" + sx).c_str());
   }
   body(&s);
-  s += emit_addr(*S.out, "w_", "");
+  s += out_addr("L");
   if (copy) s += fmt("      ((%s *)A.out)[w_] = res;\n", OST.c_str());
   else s += fmt("      ph_stp((%s *)A.out, w_, res);\n", OST.c_str());
   s += "    }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
@@ -1866,7 +1893,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
   if (copy) {
     // chain links: q1[L] = the earlier element whose write L reads, or -1
     s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_link(OrdArgs A) {\n";
-    s += loop + emit_coords(S.out->view, "L", "") + emit_addr(*S.in[0], "a_0", "") + emit_inverse(w, "a_0", "_0");
+    s += loop + emit_coords(S.out->view, "L", "") + in_addr(0, "L") + inverse(0);
     s += "      A.q1[L] = (ok_0 && lp_0 < L) ? lp_0 : -1;\n    }\n}\n";
     // one pointer-jumping pass: q1[L] = q0[q0[L]] where both are links
     s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_jump(OrdArgs A) {\n  int ctl = 0;\n";
@@ -1876,8 +1903,55 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
     // gather: tn[L] = the old value at the read address of L's root
     s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_gather(OrdArgs A) {\n";
     s += loop + "      const i64 q = A.iter ? A.q0[L] : -1; const i64 R = q >= 0 ? q : L;\n";
-    s += emit_coords(S.out->view, "R", "") + emit_addr(*S.in[0], "a_0", "");
+    s += emit_coords(S.out->view, "R", "") + in_addr(0, "R");
     s += fmt("      ((%s *)A.tn)[L] = ((const %s *)A.in[0])[a_0];\n    }\n}\n", OST.c_str(), OST.c_str());
+  }
+  if (S.any_index()) {
+    // check: every index inside its buffer, and the span the writes cover
+    // (mm preset to {extent, -1} by ph_ord_mm_init)
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_check(OrdArgs A) {\n  int ctl = 0;\n";
+    s += "  i64 lo = 0x7fffffffffffffffLL, hi = -1;\n" + loop;
+    for (int j = 0; j < S.n_in; ++j)
+      if (S.indexed >> j & 1)
+        s += fmt("      { const i64 a = A.ix[%d][L]; if (a < 0 || a >= %s) ctl |= 1; }\n", j,
+                 lit_i64(S.in[j]->extent).c_str());
+    if (S.out_indexed)
+      s += fmt("      { const i64 a = A.wx[L]; if (a < 0 || a >= %s) ctl |= 1; lo = a < lo ? a : lo; hi = a > hi ? a : hi; }\n",
+               lit_i64(S.out->extent).c_str());
+    s += "    }\n  ph_ctl(A.flags + 1, ctl);\n";
+    if (S.out_indexed) {
+      s += "  for (int o = 32; o > 0; o >>= 1) {\n";
+      s += "    const i64 l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);\n";
+      s += "    lo = l2 < lo ? l2 : lo; hi = h2 > hi ? h2 : hi;\n  }\n";
+      s += "  if ((threadIdx.x & 63) == 0 && hi >= 0) {\n";
+      s += "    atomicMin((unsigned long long *)A.mm, (unsigned long long)lo);\n";
+      s += "    atomicMax((unsigned long long *)(A.mm + 1), (unsigned long long)hi);\n  }\n";
+    }
+    s += "}\n";
+    s += "extern \"C\" __global__ void ph_ord_mm_init(OrdArgs A) {\n";
+    s += "  if (blockIdx.x == 0 && threadIdx.x == 0) { A.mm[0] = " + lit_i64(S.out->extent) + "; A.mm[1] = 0; }\n}\n";
+    // direct: one parallel pass, for maps in which no element reads another's output
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_direct(OrdArgs A) {\n  int fl = 0;\n";
+    s += loop + emit_coords(S.out->view, "L", "");
+    for (int j = 0; j < S.n_in; ++j) {
+      s += in_addr(j, "L");
+      s += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), fmt("a_%d", j)).c_str());
+    }
+    body(&s);
+    s += out_addr("L");
+    if (copy) s += fmt("      ((%s *)A.out)[w_] = res;\n", OST.c_str());
+    else s += fmt("      ph_stp((%s *)A.out, w_, res);\n", OST.c_str());
+    s += "    }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
+  }
+  if (S.out_indexed) {
+    // W^-1 as a table: cleared, filled, then checked (an element named
+    // twice leaves one of its writers without its entry)
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_inv_clear(OrdArgs A) {\n";
+    s += "  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < A.ispan; i += (i64)gridDim.x * 256) A.inv[i] = -1;\n}\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_inv_fill(OrdArgs A) {\n" + loop;
+    s += "      A.inv[A.wx[L] - A.ibase] = L;\n    }\n}\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_inv_check(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    s += "      if (A.inv[A.wx[L] - A.ibase] != L) ctl |= 1;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
   }
   return s;
 }
@@ -1885,6 +1959,8 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
 struct OrdKernels {
   hipFunction_t classify = nullptr, relax = nullptr, write = nullptr, seq = nullptr;
   hipFunction_t link = nullptr, jump = nullptr, gather = nullptr;
+  hipFunction_t check = nullptr, direct = nullptr, mm_init = nullptr, inv_clear = nullptr, inv_fill = nullptr,
+                inv_check = nullptr;
 };
 
 std::map<std::string, OrdKernels> &ord_cache() {
@@ -1896,6 +1972,8 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   std::string key = S.prog ? "O" + prog_key(S.prog) : fmt("C%zu", S.esize);
   key_add(&key, S.in_dt, sizeof(int) * (size_t)S.n_in);
   key_add(&key, &S.hazard, sizeof(S.hazard));
+  key_add(&key, &S.indexed, sizeof(S.indexed));
+  key_add(&key, &S.out_indexed, sizeof(S.out_indexed));
   key_add(&key, S.out, sizeof(ph_view_desc));
   for (int j = 0; j < S.n_in; ++j) key_add(&key, S.in[j], sizeof(ph_view_desc));
   auto hit = ord_cache().find(key);
@@ -1911,6 +1989,12 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_link", &K->link);
   if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_jump", &K->jump);
   if (!rc && !S.prog) rc = ph::jit_kernel2(src, "ph_ord_gather", &K->gather);
+  if (!rc && S.any_index()) rc = ph::jit_kernel2(src, "ph_ord_check", &K->check);
+  if (!rc && S.any_index()) rc = ph::jit_kernel2(src, "ph_ord_direct", &K->direct);
+  if (!rc && S.any_index()) rc = ph::jit_kernel2(src, "ph_ord_mm_init", &K->mm_init);
+  if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_clear", &K->inv_clear);
+  if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_fill", &K->inv_fill);
+  if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_check", &K->inv_check);
   if (rc) return rc;
   ord_cache()[key] = *K;
   return PH_OK;
@@ -1925,6 +2009,11 @@ struct OrdArgsHost {
   int64_t *q1;
   int *flags;
   int64_t iter, l0, l1;
+  const int64_t *ix[PH_MAX_INPUTS];
+  const int64_t *wx;
+  int64_t *inv;
+  int64_t *mm;  // the write index's least and greatest address
+  int64_t ibase, ispan;  // the addresses inv covers: [ibase, ibase + ispan)
 };
 
 int ord_launch(hipFunction_t fn, OrdArgsHost &A, int64_t n, bool one_thread, const char *what) {
@@ -1965,29 +2054,70 @@ int ord_sequential(const OrdKernels &K, OrdArgsHost &A, int64_t n, int32_t *flag
   return PH_OK;
 }
 
+// One parallel pass of an indexed map in which no element reads another's
+// output; its flags as raised.
+int ord_direct(const OrdKernels &K, OrdArgsHost &A, int64_t n, int32_t *flags_out) {
+  int words[2] = {0, 0};
+  int rc = ord_pass(K.direct, A, n, words, "indexed map");
+  if (rc) return rc;
+  if (flags_out) *flags_out = words[0];
+  return PH_OK;
+}
+
 // Runs S in the reference's order.  *done false: no element reads another's
-// output, and the caller's parallel kernels give the same result.
-int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *flags_out, bool *done) {
+// output, and the caller's parallel kernels give the same result.  ix / wx:
+// the index arrays of the operands S marks as indexed.
+int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *flags_out, bool *done,
+                const int64_t *const *ix = nullptr, const int64_t *wx = nullptr) {
   *done = true;
   if (flags_out) *flags_out = 0;
   const int64_t n = ph::desc_count(S.out);
-  const WriteInverse w = write_inverse(*S.out);
+  const WriteInverse w = S.out_indexed ? WriteInverse() : write_inverse(*S.out);
   OrdKernels K;
   int rc = ord_kernels(S, w, &K);
   if (rc) return rc;
   OrdArgsHost A;
   std::memset(&A, 0, sizeof(A));
-  for (int j = 0; j < S.n_in; ++j) A.in[j] = ins[j];
+  for (int j = 0; j < S.n_in; ++j) {
+    A.in[j] = ins[j];
+    A.ix[j] = (S.indexed >> j & 1) ? ix[j] : nullptr;
+  }
   A.out = out;
+  A.wx = S.out_indexed ? wx : nullptr;
   void *fw;
   rc = ph::scratch(64, &fw, 3);
   if (rc) return rc;
   A.flags = (int *)fw;
-  if (!w.ok) return ord_sequential(K, A, n, flags_out);  // an output that does not nest
+  A.mm = (int64_t *)fw + 2;  // after the two flag words
   int words[2] = {0, 0};
+  if (S.any_index()) {
+    rc = ord_launch(K.mm_init, A, 1, true, "indexed map (index bounds)");
+    if (!rc) rc = ord_pass(K.check, A, n, words, "indexed map (index bounds)");
+    if (rc) return rc;
+    if (words[1] & kCtlConflict) return ph::set_error(PH_E_INDEX, "an index array addresses outside its buffer");
+    if (S.out_indexed) {
+      int64_t mm[2];
+      PH_HIP(hipMemcpyAsync(mm, A.mm, sizeof(mm), hipMemcpyDeviceToHost, ph::stream()));
+      PH_HIP(hipStreamSynchronize(ph::stream()));
+      A.ibase = mm[0];
+      A.ispan = mm[1] - mm[0] + 1;
+      void *inv;
+      rc = ph::scratch((size_t)A.ispan * sizeof(int64_t), &inv, 8);
+      if (rc) return rc;
+      A.inv = (int64_t *)inv;
+      rc = ord_launch(K.inv_clear, A, A.ispan, false, "indexed map (inverse)");
+      if (!rc) rc = ord_launch(K.inv_fill, A, n, false, "indexed map (inverse)");
+      if (!rc) rc = ord_pass(K.inv_check, A, n, words, "indexed map (inverse)");
+      if (rc) return rc;
+      if (words[1] & kCtlConflict) return ph::set_error(PH_E_ARGUMENT, "the output index names an element twice");
+    }
+    if (!S.hazard) return ord_direct(K, A, n, flags_out);
+  }
+  if (!S.out_indexed && !w.ok) return ord_sequential(K, A, n, flags_out);  // an output that does not nest
   rc = ord_pass(K.classify, A, n, words, "ordered map (classify)");
   if (rc) return rc;
   if (!(words[1] & kCtlConflict)) {
+    if (S.any_index()) return ord_direct(K, A, n, flags_out);
     *done = false;
     return PH_OK;
   }
@@ -2116,6 +2246,35 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
       return ph::set_error(PH_E_ARGUMENT, "kind 5 takes two inputs of one dtype (I32, F32 or F64)");
     return ph::gbatch_compile_check(prog, in_dt[0]);
   }
+  if (kind == 6) {
+    // the kernels of an indexed map: input 0 reads the output's buffer
+    // through an index array, and the output is written through one
+    if (in_dt[0] != prog->out_dtype) return ph::set_error(PH_E_ARGUMENT, "kind 6: input 0 must have the output's dtype");
+    ph_view_desc d[PH_MAX_INPUTS + 1];
+    OrdSpec S;
+    S.prog = prog;
+    S.n_in = n_in;
+    for (int j = 0; j <= n_in; ++j) {
+      std::memset(&d[j], 0, sizeof(d[j]));
+      d[j].dtype = j < n_in ? in_dt[j] : prog->out_dtype;
+      d[j].extent = rows * cols;
+      d[j].view.rank = 2;
+      d[j].view.shape[0] = rows;
+      d[j].view.shape[1] = cols;
+      d[j].view.stride[0] = cols;
+      d[j].view.stride[1] = 1;
+      if (j < n_in) {
+        S.in_dt[j] = in_dt[j];
+        S.in[j] = &d[j];
+      }
+    }
+    S.out = &d[n_in];
+    S.hazard = 1;
+    S.esize = ph::dtype_size(prog->out_dtype);
+    S.indexed = 1;
+    S.out_indexed = true;
+    return ph::jit_compile_only(ord_source(S, WriteInverse()));
+  }
   if (kind == 3 || kind == 4) {
     // the kernels of a map whose input 0 reads the output's buffer
     // transposed (3), or of a copy reversing a buffer onto itself (4)
@@ -2223,6 +2382,99 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   P.tr = 64;
   P.ntr = (rows + 63) / 64;
   return ph::jit_compile_only(red_source(prog, n_in, in_dt, V.rs, V.cs, P));
+}
+
+int ph_map_indexed(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,
+                   const int64_t *const *in_index, void *out, const ph_view_desc *out_desc,
+                   const int64_t *out_index, int32_t *flags_out) {
+  PH_GUARD();
+  PH_REQUIRE_INIT();
+  if (flags_out) *flags_out = 0;
+  if (n_in < 1 || n_in > PH_MAX_INPUTS) return ph::set_error(PH_E_ARGUMENT, "%d inputs (1..%d)", n_in, PH_MAX_INPUTS);
+  if (!ins || !in_descs || !out_desc) return ph::set_error(PH_E_ARGUMENT, "ph_map_indexed: null argument");
+  if (!prog && n_in != 1) return ph::set_error(PH_E_ARGUMENT, "a copy (no program) takes one input");
+  // the descriptors the kernels see: an indexed operand keeps its dtype,
+  // buffer extent and logical shape
+  auto plain = [](ph_view_desc *d) {
+    const int r = d->view.rank < 0 ? 0 : std::min(d->view.rank, PH_MAX_RANK);
+    d->nstage = 0;
+    d->view.offset = 0;
+    d->view.pad_ = 0;
+    for (int i = 0; i < PH_MAX_RANK; ++i) {
+      d->view.stride[i] = 0;
+      if (i >= r) d->view.shape[i] = 0;
+    }
+    std::memset(d->stage, 0, sizeof(d->stage));
+  };
+  ph_view_desc din[PH_MAX_INPUTS], dout = *out_desc;
+  unsigned indexed = 0;
+  for (int j = 0; j < n_in; ++j) {
+    din[j] = in_descs[j];
+    if (in_index && in_index[j]) {
+      indexed |= 1u << j;
+      plain(&din[j]);
+    }
+  }
+  if (out_index) plain(&dout);
+  if (!indexed && !out_index)
+    return prog ? ph_map(prog, n_in, ins, in_descs, out, out_desc, flags_out)
+                : ph_strided_copy(out, out_desc, ins[0], in_descs, 0);
+  int in_dt[PH_MAX_INPUTS];
+  int rc;
+  if (prog) {
+    rc = check_inputs(prog, n_in, ins, din, in_dt, true);
+  } else {
+    rc = ph::validate_desc(&din[0], "input");
+    if (!rc && !ins[0]) rc = ph::set_error(PH_E_ARGUMENT, "input 0: null pointer");
+    in_dt[0] = din[0].dtype;
+  }
+  if (rc) return rc;
+  rc = ph::validate_desc(&dout, "output");
+  if (rc) return rc;
+  if (dout.nstage != 0) return ph::set_error(PH_E_UNSUPPORTED, "map output must be affine or indexed");
+  if (prog ? dout.dtype != prog->out_dtype : ph::dtype_size(dout.dtype) != ph::dtype_size(din[0].dtype))
+    return ph::set_error(PH_E_ARGUMENT, "output dtype %d does not take the %s", dout.dtype,
+                         prog ? "program's result" : "input's elements");
+  if (dout.view.rank != din[0].view.rank) return ph::set_error(PH_E_SHAPE, "output rank differs from inputs");
+  for (int i = 0; i < dout.view.rank; ++i)
+    if (dout.view.shape[i] != din[0].view.shape[i])
+      return ph::set_error(PH_E_SHAPE, "output shape differs on axis %d", i);
+  const int64_t n = ph::desc_count(&dout);
+  if (prog && prog_uses_index(prog) && n > INT32_MAX)
+    return ph::set_error(PH_E_OVERFLOW, "element index of %lld elements exceeds Int32", (long long)n);
+  if (n == 0) return PH_OK;
+  if (!out) return ph::set_error(PH_E_ARGUMENT, "output: null pointer");
+  // inputs on the output's buffer (any through an index array, or through
+  // a descriptor when the output is indexed) read it in the loop's order
+  unsigned hz = 0;
+  const size_t oes = ph::dtype_size(dout.dtype);
+  for (int j = 0; j < n_in; ++j) {
+    const char *ib = (const char *)ins[j], *ob = (const char *)out;
+    if (ib != ob) {
+      const size_t ies = ph::dtype_size(din[j].dtype);
+      if (!(ib + (size_t)din[j].extent * ies <= ob || ob + (size_t)dout.extent * oes <= ib))
+        return ph::set_error(PH_E_ARGUMENT, "input %d overlaps the output through another base pointer", j);
+      continue;
+    }
+    if (!(indexed >> j & 1) && !out_index && same_map(din[j], dout)) continue;
+    if (prog ? din[j].dtype != dout.dtype : ph::dtype_size(din[j].dtype) != oes)
+      return ph::set_error(PH_E_ARGUMENT, "input %d reads the output's buffer as another element type", j);
+    hz |= 1u << j;
+  }
+  OrdSpec S;
+  S.prog = prog;
+  S.n_in = n_in;
+  for (int j = 0; j < n_in; ++j) {
+    S.in_dt[j] = in_dt[j];
+    S.in[j] = &din[j];
+  }
+  S.out = &dout;
+  S.hazard = hz;
+  S.esize = oes;
+  S.indexed = indexed;
+  S.out_indexed = out_index != nullptr;
+  bool done = false;
+  return run_ordered(S, ins, out, flags_out, &done, in_index, out_index);
 }
 
 int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_view_desc *in_descs,

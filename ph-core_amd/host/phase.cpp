@@ -1477,6 +1477,71 @@ static NArray materialize_operand(const Operand &o) {
   return out;
 }
 
+static void apply_transform(View &v, const Transform &t);
+
+// Buffer indices of the elements of a view descriptor, lexicographic, as
+// int64 values in an array of 8-byte elements.
+static NArray index_of_desc(const ph_view_desc &d) {
+  NArray r = NArray::empty(Shape(d.view.shape, d.view.shape + d.view.rank), DType::F64);
+  if (r.size() > 0) check(ph_view_index((int64_t *)r.storage->dev_write(), &d), "view index");
+  return r;
+}
+
+// The same for a chain past one descriptor (then restricted to `region`):
+// the index array of its descriptor, gathered through the rest of the chain
+// (8-byte copies move the indices unchanged).
+static NArray index_of_chain(const DeepChain &c, const IndexRegion *region = nullptr) {
+  NArray mid = index_of_desc(c.desc);
+  View v = mid.view();
+  for (size_t i = c.tail.size(); i-- > 0;) apply_transform(v, c.tail[i]);
+  if (region) v.restrict_to(*region);
+  return v.to_narr();
+}
+
+// dst's elements (through descriptor dd, or at the buffer indices in `wi`)
+// <- src, element k of one from element k of the other, both lexicographic,
+// with the source read element by element in the reference's loop order
+// (src/n_array.cr:484-492 pulls `src.each` as it writes), so it sees the
+// writes made before it.  Used where the source reads dst's own buffer and
+// either side is a view past one descriptor.  False (nothing written) for a
+// ProcView or conversion into BFloat16 storage, which only a gathered copy
+// can round.
+static bool write_in_order(const std::shared_ptr<Storage> &dst, const ph_view_desc &dd, const NArray *wi,
+                           const Operand &src) {
+  const std::shared_ptr<Storage> &ss = src.deep ? src.deep->storage : src.storage;
+  const DType sdt = ss->dtype, ddt = dst->dtype;
+  const bool copy = !src.body && dtype_size(sdt) == dtype_size(ddt) && compute_dtype(sdt) == compute_dtype(ddt);
+  ph_prog prog{};
+  if (!copy) {
+    Expr e = Expr::input(0, sdt);
+    if (src.body) e = src.body->substitute({e});
+    if (e.dtype() != compute_dtype(ddt))
+      throw ArgumentError(std::string("cannot store ") + dtype_name(e.dtype()) + " values into a " + dtype_name(ddt) +
+                          " array");
+    if (ddt == DType::BF16) return false;
+    prog = e.compile(1);
+  }
+  NArray ri = src.deep ? index_of_chain(*src.deep) : index_of_desc(src.desc);
+  NArray wtmp;
+  if (!wi && dd.nstage != 0) {
+    wtmp = index_of_desc(dd);
+    wi = &wtmp;
+  }
+  // indexed operands: their descriptors give dtype, extent and shape
+  ph_view_desc sd = src.deep ? src.deep->desc : src.desc;
+  sd.dtype = (int32_t)sdt;
+  sd.view.rank = dd.view.rank;
+  for (int i = 0; i < dd.view.rank; ++i) sd.view.shape[i] = dd.view.shape[i];
+  const void *ip = ss->dev_read();
+  const int64_t *rip = (const int64_t *)ri.storage->dev_read();
+  const int64_t *wip = wi ? (const int64_t *)wi->storage->dev_read() : nullptr;
+  void *op = dst->dev_write();
+  int32_t flags = 0;
+  check(ph_map_indexed(copy ? nullptr : &prog, 1, &ip, &sd, &rip, op, &dd, wip, &flags), "ordered write");
+  check_flags(flags, "elementwise");
+  return true;
+}
+
 NArray NArray::unsafe_fetch_chunk(const IndexRegion &region) const {
   View v = view();
   v.restrict_to(region);
@@ -1744,6 +1809,7 @@ static void scatter_into(const std::shared_ptr<Storage> &dst, const ph_view_desc
   for (int i = 0; i < dd.view.rank; ++i) n *= dd.view.shape[i];
   if (n == 0) return;
   Shape dshape(dd.view.shape, dd.view.shape + dd.view.rank);
+  if (src_in.deep && src_in.deep->storage == dst && write_in_order(dst, dd, nullptr, src_in)) return;
   Operand src = src_in;
   std::optional<ph_view_desc> fd = fit_src_desc(src.desc, dshape);
   if (!fd) {  // the source reshaped to dst's shape needs one stage more: gather it first
@@ -1889,7 +1955,14 @@ Operand View::operand() const {
     mid.shape = mid_shape_;
     View v = materialize_operand(mid).view();
     replay_tail(v);
-    return v.operand();
+    Operand o = v.operand();
+    auto c = std::make_shared<DeepChain>();
+    c->storage = storage;
+    c->desc = desc_;
+    c->mid_shape = mid_shape_;
+    c->tail.assign(transforms.begin(), transforms.begin() + (std::ptrdiff_t)tail_);
+    o.deep = c;
+    return o;
   }
   Operand o;
   o.storage = storage;
@@ -2087,6 +2160,18 @@ void MutableView::deep_write(const std::function<void(MutableView &)> &fn) {
 
 void MutableView::unsafe_set_chunk(const IndexRegion &region, const Operand &src) {
   if (deep()) {
+    if (src.storage == storage || (src.deep && src.deep->storage == storage)) {
+      // the source reads the buffer written: written through the chain's
+      // index array, element by element in the loop's order
+      DeepChain wc;
+      wc.storage = storage;
+      wc.desc = desc_;
+      wc.mid_shape = mid_shape_;
+      wc.tail.assign(transforms.begin(), transforms.begin() + (std::ptrdiff_t)tail_);
+      NArray wi = index_of_chain(wc, &region);
+      ph_view_desc dd = contiguous_desc(storage->dtype, region.shape(), std::max<int64_t>(storage->elements, 1));
+      if (wi.size() == 0 || write_in_order(storage, dd, &wi, src)) return;
+    }
     deep_write([&](MutableView &t) { t.unsafe_set_chunk(region, src); });
     return;
   }
@@ -2155,9 +2240,13 @@ Scalar ProcView::get(const Coord &coord) const {
 // NArray of the result type), or into a new NArray when out is null.
 namespace {
 // A fused map's kernel inputs (distinct storage and descriptor pairs) and
-// its body over them.
+// its body over them.  An input read through a chain past one descriptor
+// from the buffer the map writes has its index array in `index` (and its
+// chain in `chain`); the others have empty entries there.
 struct MapPlan {
   std::vector<Operand> inputs;
+  std::vector<NArray> index;
+  std::vector<const DeepChain *> chain;
   Expr body;
 };
 
@@ -2216,21 +2305,46 @@ MapPlan plan_map(const std::vector<Operand> &ops, const std::function<Expr(const
       }
     }
     int idx = -1;
+    if (o.deep && dst && o.deep->storage.get() == dst) {
+      // a view past one descriptor of the buffer written: read through its
+      // index array, element by element in the loop's order
+      for (size_t k = 0; k < p.inputs.size(); ++k)
+        if (p.chain[k] == o.deep.get()) idx = (int)k;
+      if (idx < 0) {
+        if (p.inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
+        Operand in;
+        in.storage = o.deep->storage;
+        in.desc = o.deep->desc;
+        in.desc.view.rank = (int32_t)o.shape.size();
+        for (size_t i = 0; i < o.shape.size(); ++i) in.desc.view.shape[i] = o.shape[i];
+        in.shape = o.shape;
+        p.inputs.push_back(in);
+        p.index.push_back(index_of_chain(*o.deep));
+        p.chain.push_back(o.deep.get());
+        idx = (int)p.inputs.size() - 1;
+      }
+      Expr e = Expr::input(idx, o.deep->storage->dtype);
+      return o.body ? o.body->substitute({e}) : e;
+    }
     for (size_t k = 0; k < p.inputs.size(); ++k)
-      if (p.inputs[k].storage == o.storage && std::memcmp(&p.inputs[k].desc, &o.desc, sizeof(ph_view_desc)) == 0)
+      if (!p.chain[k] && p.inputs[k].storage == o.storage &&
+          std::memcmp(&p.inputs[k].desc, &o.desc, sizeof(ph_view_desc)) == 0)
         idx = (int)k;
     if (idx < 0) {
       if (p.inputs.size() >= PH_MAX_INPUTS) throw ArgumentError("more than 8 distinct operands in one map");
       Operand in = o;
       in.body.reset();
+      in.deep.reset();  // the gathered copy is what a map not writing its source reads
       int staged = 0;
-      for (const Operand &q : p.inputs) staged += q.desc.nstage > 0 ? 1 : 0;
+      for (size_t k = 0; k < p.inputs.size(); ++k) staged += !p.chain[k] && p.inputs[k].desc.nstage > 0 ? 1 : 0;
       if (in.desc.nstage > 0 && staged >= 2) {
         // ph_map takes at most two reshaped/tiled operands: gather this one
         NArray g = materialize_operand(in);
         in = g.operand();
       }
       p.inputs.push_back(in);
+      p.index.emplace_back();
+      p.chain.push_back(nullptr);
       idx = (int)p.inputs.size() - 1;
     }
     Expr e = Expr::input(idx, (DType)o.desc.dtype);
@@ -2266,13 +2380,22 @@ void launch_map(const MapPlan &p, const std::shared_ptr<Storage> &dst, const ph_
   ph_prog prog = p.body.compile((int)p.inputs.size());
   std::vector<const void *> ptrs;
   std::vector<ph_view_desc> descs;
-  for (const Operand &o : p.inputs) {
-    ptrs.push_back(o.storage->dev_read());
-    descs.push_back(o.desc);
+  std::vector<const int64_t *> index;
+  bool indexed = false;
+  for (size_t k = 0; k < p.inputs.size(); ++k) {
+    ptrs.push_back(p.inputs[k].storage->dev_read());
+    descs.push_back(p.inputs[k].desc);
+    index.push_back(p.chain[k] ? (const int64_t *)p.index[k].storage->dev_read() : nullptr);
+    indexed = indexed || p.chain[k];
   }
   void *op = dst->dev_write();
   int32_t flags = 0;
-  check(ph_map(&prog, (int32_t)p.inputs.size(), ptrs.data(), descs.data(), op, &od, &flags), "map");
+  if (indexed)
+    check(ph_map_indexed(&prog, (int32_t)p.inputs.size(), ptrs.data(), descs.data(), index.data(), op, &od, nullptr,
+                         &flags),
+          "map");
+  else
+    check(ph_map(&prog, (int32_t)p.inputs.size(), ptrs.data(), descs.data(), op, &od, &flags), "map");
   check_flags(flags, "elementwise");
 }
 }  // namespace

@@ -280,13 +280,29 @@ class View;
 class MutableView;
 class ProcView;
 
+// A view whose transform chain is longer than one descriptor holds: the
+// storage it reads, the descriptor of the chain's oldest transforms (a view
+// of shape mid_shape) and the newer ones (newest first).  Enough to rebuild
+// the buffer index of each of its elements (ph_view_index, then gathers
+// through the newer transforms).
+struct DeepChain {
+  std::shared_ptr<Storage> storage;
+  ph_view_desc desc;
+  Shape mid_shape;
+  std::vector<Transform> tail;
+};
+
 // An operand of a fused elementwise kernel: storage + descriptor (+ the
-// ProcView block applied to it, as an expression over input 0).
+// ProcView block applied to it, as an expression over input 0).  An operand
+// of a view past one descriptor is a gathered copy of it (read when the view
+// is used) and keeps the chain in `deep`: a write to the storage the chain
+// reads then reads it element by element in the loop's order instead.
 struct Operand {
   std::shared_ptr<Storage> storage;
   ph_view_desc desc;
   Shape shape;
   std::optional<Expr> body;
+  std::shared_ptr<const DeepChain> deep;
   DType value_dtype() const;
 };
 

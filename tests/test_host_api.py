@@ -266,6 +266,22 @@ def test_ordered_map_programs_compile_for_gfx950(host):
         host.compile_check(host.placeholder(d), [d], 30, 70, 4)
 
 
+def test_indexed_map_programs_compile_for_gfx950(host):
+    """The kernels of a map whose source or destination is a view chain
+    longer than one descriptor (operands addressed through index arrays:
+    bounds check, write-inverse table, direct pass, and the ordered passes
+    reading the inverse table) are generated and compiled."""
+    for dt, fn in BLOCKS[:3] + BLOCKS[8:10]:
+        d = getattr(host.DType, DT[dt][0])
+        e = fn(host.placeholder(d))
+        if e.dtype != d:
+            continue
+        host.compile_check(e, [d], 24, 16, 6)
+    m = host.input(1, host.DType.Bool)
+    sel = host.select(m, host.input(0, host.DType.I32) * 2, host.input(2, host.DType.I32))
+    host.compile_check(sel, [host.DType.I32, host.DType.Bool, host.DType.I32], 40, 40, 6)
+
+
 def test_element_index_programs(host):
     """A block that reads the element's lexicographic index (map_with_index,
     n_array.cr:605-611) compiles into the map kernel in its row-tiled and its
