@@ -1368,12 +1368,21 @@ int or_sum_i32(const int32_t *x, int64_t n, int64_t *out) {
 // Correctly rounded sum (Shewchuk's exact partials, the published algorithm
 // behind Python's math.fsum, with its half-way rounding correction).  This is
 // the FP64 reference a compensated GPU sum is measured against; it is not
-// ph-core's (sequential) result, whose own error grows with n.
+// ph-core's (sequential) result, whose own error grows with n.  Non-finite
+// terms are summed apart, as math.fsum does: any NaN gives NaN, infinities
+// of one sign that infinity, of both signs NaN (where math.fsum raises).
 double or_sum_exact(const double *x, int64_t n) {
   std::vector<double> p;
   p.reserve(64);
+  double special = 0.0;
+  bool nonfinite = false;
   for (int64_t k = 0; k < n; ++k) {
     double v = x[k];
+    if (!std::isfinite(v)) {
+      special += v;
+      nonfinite = true;
+      continue;
+    }
     size_t i = 0;
     for (size_t j = 0; j < p.size(); ++j) {
       double y = p[j];
@@ -1386,6 +1395,7 @@ double or_sum_exact(const double *x, int64_t n) {
     p.resize(i);
     p.push_back(v);
   }
+  if (nonfinite) return special;
   if (p.empty()) return 0.0;
   size_t m = p.size();
   double hi = p[--m];

@@ -940,7 +940,18 @@ struct RedPlan {
   const char *acc_t;   // float / double / i64
   bool vec_ok[PH_MAX_INPUTS];  // whole-vector loads of the lane's VEC columns
   bool colex = false;  // the view's lexicographic order runs down the columns (a swapped transposed view)
+  int64_t ntc_fin = 0;  // column-tile states per row left for the finish after the pre-folds
+  int64_t ntr_fin = 0;  // row-tile states per column, likewise
 };
+
+// Column-tile states per row that the finish folds on one thread; rows with
+// more are pre-folded in ordered runs of this many (ph_red_prefold), and so
+// are the row-tile states of columns with more.
+constexpr int64_t kRowFold = 64;
+int64_t prefolded(int64_t ntc) {
+  while (ntc > kRowFold) ntc = (ntc + kRowFold - 1) / kRowFold;
+  return ntc;
+}
 
 // Statements evaluating the program at element (R, C) of the 2-D
 // arrangement; returns the value as a double expression.
@@ -1245,6 +1256,21 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
   }
   s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
   }
+  if ((P.do_rows && P.ntc > kRowFold) || (P.do_cols && P.ntr > kRowFold)) {
+    // pre-fold: state t of series i (a row over its column tiles, or a column
+    // over its row tiles) <- the states [t * 64, t * 64 + 64) of the level
+    // below, in order (a long series would otherwise be folded on one or four
+    // threads of the finish)
+    s += "struct FoldArgs { const void *src; void *dst; i64 rows; i64 nin; i64 nout; };\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_red_prefold(FoldArgs F) {\n";
+    s += fmt("  typedef %s S;\n", P.state);
+    s += "  const i64 g = (i64)blockIdx.x * 256 + threadIdx.x;\n  if (g >= F.rows * F.nout) return;\n";
+    s += "  const i64 i = g % F.rows, t = g / F.rows;\n";
+    s += fmt("  const i64 t0 = t * %s, t1 = (t0 + %s < F.nin) ? t0 + %s : F.nin;\n", lit_i64(kRowFold).c_str(),
+             lit_i64(kRowFold).c_str(), lit_i64(kRowFold).c_str());
+    s += "  S a; st_init(a);\n#pragma unroll 8\n  for (i64 u = t0; u < t1; ++u) a = st_comb(a, ((const S *)F.src)[u * F.rows + i]);\n";
+    s += "  ((S *)F.dst)[t * F.rows + i] = a;\n}\n";
+  }
   // --- finish.  Column blocks: 64 columns x 4 parts; part p folds row tiles
   // [p*ntr/4, (p+1)*ntr/4) in order, then the 4 parts are folded in order.
   // Row blocks: one row per thread (its column tiles in order), then the
@@ -1259,8 +1285,8 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     s += fmt("  if ((i64)blockIdx.x < %s) {\n", lit_i64(ncb).c_str());
     s += "    const int lane = threadIdx.x & 63; const int part = threadIdx.x >> 6;\n";
     s += "    const i64 c = (i64)blockIdx.x * 64 + lane;\n";
-    s += fmt("    const i64 t0 = part * %s / 4, t1 = (part + 1) * %s / 4;\n", lit_i64(P.ntr).c_str(),
-             lit_i64(P.ntr).c_str());
+    s += fmt("    const i64 t0 = part * %s / 4, t1 = (part + 1) * %s / 4;\n", lit_i64(P.ntr_fin).c_str(),
+             lit_i64(P.ntr_fin).c_str());
     s += "    S a; st_init(a);\n";
     if (fix) {
       // the column's first element, loaded before the fold so its latency overlaps it
@@ -1311,7 +1337,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       }
     }
     s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
-             lit_i64(P.rows).c_str(), lit_i64(P.ntc).c_str(), lit_i64(P.rows).c_str());
+             lit_i64(P.rows).c_str(), lit_i64(P.ntc_fin).c_str(), lit_i64(P.rows).c_str());
     s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n";
     if (fix) {
       // first element and, when the row's maximum is a zero, the first zero
@@ -1395,9 +1421,49 @@ struct View2D {
 };
 
 // Runs the fused reduction over a 2-D arrangement (inputs affine).
+// A split of n elements into rows of `cols` that the tiles stream well: a
+// multiple of 8 dividing n, nearest 16384 within [4096, 65536] (then
+// [1024, 2^20]); 0 when n has none.
+int64_t stream_split(int64_t n) {
+  int64_t best = 0;
+  for (int64_t lim : {int64_t(1) << 16, int64_t(1) << 20}) {
+    const int64_t lo = lim == (int64_t(1) << 16) ? 4096 : 1024;
+    for (int64_t c = lo; c <= lim && c <= n; c += 8)
+      if (n % c == 0 && (best == 0 || std::llabs(c - 16384) < std::llabs(best - 16384))) best = c;
+    if (best) break;
+  }
+  return best;
+}
+
 int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const int *in_dt,
-                 const View2D &V, int acc, double *full_sum, double *full_max, double *row_sum,
+                 const View2D &V0, int acc, double *full_sum, double *full_max, double *row_sum,
                  double *row_max, double *col_sum, double *col_max, int32_t *flags_out, bool colex) {
+  View2D V = V0;
+  // Only the whole view's result, over elements that lie contiguously in
+  // lexicographic order (a rank-1 array, a whole row-major array): any
+  // rows x cols split of the same run reads the same sequence, so the one
+  // that streams best is taken (tall tiles, short rows).  Int32 running
+  // totals and the maximum's first-zero key see the elements in the same
+  // order; only the grouping of the float partial sums changes.
+  if ((full_sum || full_max) && !row_sum && !row_max && !col_sum && !col_max && !colex) {
+    bool contig = true;
+    for (int j = 0; j < n_in; ++j) {
+      // element (r, c) at off + r * rs + c * cs must be off + r * cols + c
+      const bool cols_ok = V.cols == 1 || V.cs[j] == 1;
+      const bool rows_ok = V.rows == 1 || V.rs[j] == V.cols;
+      contig = contig && cols_ok && rows_ok;
+    }
+    const int64_t n = V.rows * V.cols;
+    const int64_t c = contig ? stream_split(n) : 0;
+    if (c && c != V.cols) {
+      V.rows = n / c;
+      V.cols = c;
+      for (int j = 0; j < n_in; ++j) {
+        V.rs[j] = c;
+        V.cs[j] = 1;
+      }
+    }
+  }
   RedPlan P;
   P.colex = colex;
   P.rows = V.rows;
@@ -1426,6 +1492,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
                   ((uintptr_t)ins[j] % align) == 0;
   }
   P.ntc = (P.cols + 256 * P.vec - 1) / (256 * P.vec);
+  P.ntc_fin = prefolded(P.ntc);
   // rows per tile: aim for ~512 workgroups (fewer, taller tiles write fewer
   // column partial states; measured best on 16384^2), a multiple of 16 (the
   // float tile folds rows 8 at a time), at most 1024 rows (256 for integer
@@ -1440,6 +1507,7 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   if (tr < 1) tr = 1;
   P.tr = (int)tr;
   P.ntr = (P.rows + P.tr - 1) / P.tr;
+  P.ntr_fin = prefolded(P.ntr);
   std::string key = "R" + prog_key(prog);
   key_add(&key, in_dt, sizeof(int) * (size_t)n_in);
   key_add(&key, V.rs, sizeof(int64_t) * (size_t)n_in);
@@ -1448,29 +1516,40 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   const int64_t plan[] = {P.rows, P.cols, P.tr, P.ntr, P.ntc, P.vec, P.do_rows, P.do_cols, P.do_full, P.colex};
   key_add(&key, plan, sizeof(plan));
   key += P.state;
-  hipFunction_t ktile, kfin;
+  hipFunction_t ktile, kfin, kfold = nullptr;
+  const bool fold_rows = P.do_rows && P.ntc > kRowFold, fold_cols = P.do_cols && P.ntr > kRowFold;
+  const bool prefold = fold_rows || fold_cols;
   int rc;
   auto hit = fused_cache().find(key);
   if (hit != fused_cache().end()) {
     ktile = hit->second.k[0];
     kfin = hit->second.k[1];
+    kfold = hit->second.k[2];
   } else {
     std::string src = red_source(prog, n_in, in_dt, V.rs, V.cs, P);
     rc = ph::jit_kernel(src, "ph_red_tile", &ktile);
     if (rc) return rc;
     rc = ph::jit_kernel(src, "ph_red_finish", &kfin);
     if (rc) return rc;
-    fused_cache()[key] = FusedKernels{{ktile, kfin, nullptr}, 0};
+    if (prefold) {
+      rc = ph::jit_kernel(src, "ph_red_prefold", &kfold);
+      if (rc) return rc;
+    }
+    fused_cache()[key] = FusedKernels{{ktile, kfin, kfold}, 0};
   }
   const bool may_flag = prog_may_flag(prog) || (std::string(P.state) == "SI" && (full_sum || row_sum || col_sum));
-  // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [row blocks of 256] (sized [rows])
+  // scratch: rowpart [ntc][rows], colpart [ntr][cols], rowfinal [row blocks of 256] (sized [rows]),
+  // the first pre-fold levels [ntc / 64][rows] and [ntr / 64][cols]
   const size_t rowpart_b = P.do_rows ? (size_t)(P.ntc * P.rows) * P.state_bytes : 0;
+  const size_t fold_b = fold_rows ? (size_t)((P.ntc + kRowFold - 1) / kRowFold * P.rows) * P.state_bytes : 0;
+  const size_t cfold_b = fold_cols ? (size_t)((P.ntr + kRowFold - 1) / kRowFold * P.cols) * P.state_bytes : 0;
   const size_t colpart_b = P.do_cols ? (size_t)(P.ntr * P.cols) * P.state_bytes : 0;
   const size_t rowfinal_b = P.do_rows ? (size_t)P.rows * P.state_bytes : 0;
   const size_t rowzero_b = P.do_full ? (size_t)((P.rows + 255) / 256) * sizeof(int64_t) : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   void *base;
-  rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + al(rowzero_b) + 256, &base, 2);
+  rc = ph::scratch(al(rowpart_b) + al(colpart_b) + al(rowfinal_b) + al(rowzero_b) + al(fold_b) + al(cfold_b) + 256,
+                   &base, 2);
   if (rc) return rc;
   int *dflags;
   rc = get_flags_buffer(&dflags, may_flag);
@@ -1500,6 +1579,39 @@ int run_reduce2d(const ph_prog *prog, int n_in, const void *const *ins, const in
   if (rc) return rc;
   PH_HIP(hipModuleLaunchKernel(ktile, (unsigned)(P.ntr * P.ntc), 1, 1, 256, 1, 1, 0, ph::stream(),
                                nullptr, cfg));
+  // ordered pre-folds of long series of tile states, 64 to 1, ping-pong
+  // between the tile kernel's buffer and a fold buffer, until the finish's
+  // count is left; returns the buffer holding the last level
+  auto fold = [&](void *states, char *spare, int64_t series, int64_t n) -> void * {
+    struct {
+      const void *src;
+      void *dst;
+      int64_t rows, nin, nout;
+    } F;
+    char *lv[2] = {(char *)states, spare};
+    int cur = 0;
+    while (n > kRowFold) {
+      const int64_t nout = (n + kRowFold - 1) / kRowFold;
+      F.src = lv[cur];
+      F.dst = lv[cur ^ 1];
+      F.rows = series;
+      F.nin = n;
+      F.nout = nout;
+      size_t fsz = sizeof(F);
+      void *fcfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &F, HIP_LAUNCH_PARAM_BUFFER_SIZE, &fsz, HIP_LAUNCH_PARAM_END};
+      if (hipModuleLaunchKernel(kfold, (unsigned)((series * nout + 255) / 256), 1, 1, 256, 1, 1, 0, ph::stream(),
+                                nullptr, fcfg) != hipSuccess)
+        return nullptr;
+      n = nout;
+      cur ^= 1;
+    }
+    return lv[cur];
+  };
+  char *spare = (char *)A.rowzero + al(rowzero_b);
+  if (fold_rows && !(A.rowpart = fold(A.rowpart, spare, P.rows, P.ntc)))
+    return ph::set_error(PH_E_HIP, "reduction pre-fold launch failed");
+  if (fold_cols && !(A.colpart = fold(A.colpart, spare + al(fold_b), P.cols, P.ntr)))
+    return ph::set_error(PH_E_HIP, "reduction pre-fold launch failed");
   // finish: column blocks of 64 columns, then row blocks of 256 rows
   const int64_t nfin = (P.do_cols ? (P.cols + 63) / 64 : 0) + (P.do_rows ? (P.rows + 255) / 256 : 0);
   if (nfin > 0)
@@ -2379,8 +2491,10 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
   P.vec = red_vec(std::string(P.state) == "SI", n_in, in_dt);
   for (int j = 0; j < PH_MAX_INPUTS; ++j) P.vec_ok[j] = j < n_in && cols % P.vec == 0;
   P.ntc = (cols + 256 * P.vec - 1) / (256 * P.vec);
+  P.ntc_fin = prefolded(P.ntc);
   P.tr = 64;
   P.ntr = (rows + 63) / 64;
+  P.ntr_fin = prefolded(P.ntr);
   return ph::jit_compile_only(red_source(prog, n_in, in_dt, V.rs, V.cs, P));
 }
 

@@ -266,6 +266,16 @@ def test_ordered_map_programs_compile_for_gfx950(host):
         host.compile_check(host.placeholder(d), [d], 30, 70, 4)
 
 
+def test_reduction_prefold_compiles_for_gfx950(host):
+    """Rows of more than 64 column tiles (2 x 2^20) and columns of more than
+    64 row tiles (2^17 x 64) get the ordered pre-fold kernel besides the
+    tile and the finish, for each state type."""
+    for d in (host.DType.F32, host.DType.F64, host.DType.I32, host.DType.BF16):
+        e = host.placeholder(d) * 2
+        host.compile_check(e, [d], 2, 1 << 20, 1)
+        host.compile_check(e, [d], 1 << 17, 64, 1)
+
+
 def test_indexed_map_programs_compile_for_gfx950(host):
     """The kernels of a map whose source or destination is a view chain
     longer than one descriptor (operands addressed through index arrays:

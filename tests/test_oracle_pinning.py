@@ -491,6 +491,14 @@ def test_reductions_vs_fsum():
     # ill-conditioned: exact cancellation
     x = np.array([1e100, 1.0, -1e100, 1e-100], dtype=np.float64)
     assert O.sum_exact(x) == math.fsum(x.tolist()) == 1.0
+    # non-finite terms, as math.fsum treats them (a NaN anywhere in a long
+    # run returns at once instead of growing the partials)
+    y = rng.standard_normal(200003)
+    y[7] = np.nan
+    assert math.isnan(O.sum_exact(y)) and math.isnan(math.fsum(y.tolist()))
+    for vals in ([math.inf, 1.0, 2.0], [-math.inf, -math.inf, 5.0], [1.0, math.inf, math.inf]):
+        assert O.sum_exact(np.array(vals)) == math.fsum(vals)
+    assert math.isnan(O.sum_exact(np.array([math.inf, 3.0, -math.inf])))  # math.fsum raises here
     # Int32 checked sum: a prefix that overflows raises even if the total fits
     v, rc = O.sum_i32(np.array([2 ** 31 - 1, 1, -1], np.int32))
     assert rc == O.E_OVERFLOW and v == 2 ** 31 - 1
