@@ -783,6 +783,7 @@ def batched_workload(phase, ev, args):
         "speedup_vs_unbatched": round(gms_unbatched / gms_graph, 2) if gms_unbatched else None,
         "pass_gbs": round(pass_bytes / gms_graph / 1e6, 1),
         "pass_frac": round(pass_bytes / gms_graph / 1e6 / PEAK_HBM_GBS, 4),
+        "traffic": (load_traffic() or {}).get("ph_gbatch"),
         "note": "records carry their own programs and regions (ph_gbatch); unbatched launches the same "
                 "kernel once per array",
     }
