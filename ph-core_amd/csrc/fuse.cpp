@@ -187,6 +187,46 @@ bool prog_uses_index(const ph_prog *p) {
   return false;
 }
 
+// Whether the value the reduction's maxima take can never be a signalling
+// NaN: it is widened from Float32 to Float64 on the way (a conversion), or
+// the program's result comes out of a floating-point operation that the
+// compiler cannot fold into a pass-through of its operand (no identity
+// constant: x * 1, x / 1, x - 0, x + -0 are folded).  Loads, constants,
+// selects and sign changes can pass a signalling NaN through unchanged.
+bool max_operand_quiet(const ph_prog *p, bool widened) {
+  if (widened) return true;
+  int def[PH_NREG];
+  for (int r = 0; r < PH_NREG; ++r) def[r] = -1;
+  for (int k = 0; k < p->n_insn; ++k)
+    if (p->insn[k].dst < PH_NREG) def[p->insn[k].dst] = k;
+  auto const_val = [&](int reg, int before, double *v) {
+    int d = -1;
+    for (int k = 0; k < before; ++k)
+      if (p->insn[k].dst == reg) d = k;
+    if (d < 0 || p->insn[d].op != PH_OP_CONST) return false;
+    const ph_insn &c = p->insn[d];
+    *v = c.dt == PH_I32 ? (double)c.imm.i : c.imm.f;
+    return true;
+  };
+  const int k = p->out_reg < PH_NREG ? def[p->out_reg] : -1;
+  if (k < 0) return false;
+  const ph_insn &in = p->insn[k];
+  if (in.dt != PH_F32 && in.dt != PH_F64) return false;
+  double va = 0, vb = 0;
+  const bool ca = const_val(in.a, k, &va), cb = const_val(in.b, k, &vb);
+  switch (in.op) {
+    case PH_OP_ADD: return !(ca && va == 0.0 && std::signbit(va)) && !(cb && vb == 0.0 && std::signbit(vb));
+    case PH_OP_SUB: return !(cb && vb == 0.0 && !std::signbit(vb)) && !(ca && va == 0.0 && std::signbit(va));
+    case PH_OP_MUL: return !(ca && std::fabs(va) == 1.0) && !(cb && std::fabs(vb) == 1.0);
+    case PH_OP_DIV: return !(cb && std::fabs(vb) == 1.0);
+    case PH_OP_FLOORDIV:
+    case PH_OP_MOD:
+    case PH_OP_SQRT: return true;
+    case PH_OP_CAST: return in.c != in.dt;
+    default: return false;
+  }
+}
+
 // Emits the statements computing the program for element `e`; inputs are
 // the expressions `inexpr[j]` (already of their compute type).  Returns the
 // name of the result variable.
@@ -801,8 +841,18 @@ __device__ __forceinline__ float ph_bf(unsigned w, int h) { return __uint_as_flo
 __device__ __forceinline__ bool ph_bb(unsigned w, int b) { return ((w >> (8 * b)) & 0xffu) != 0; }
 __device__ __forceinline__ float ph_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double ph_fmax(double a, double b) { return __builtin_fmax(a, b); }
-template <typename F, bool MAX> __device__ __forceinline__ F ph_comb2(F a, F b) {
-  if (MAX) return ph_fmax(a, b);
+// a signalling NaN made quiet (any other value unchanged)
+__device__ __forceinline__ float ph_quiet(float x) { return __builtin_canonicalizef(x); }
+__device__ __forceinline__ double ph_quiet(double x) { return __builtin_canonicalize(x); }
+// a = max(a, b) by the instruction itself, for operands that are never
+// signalling NaNs (a quiet NaN operand yields the other one, as fmax)
+__device__ __forceinline__ void ph_amax(double &a, double b) { __asm__ volatile("v_max_f64 %0, %0, %1" : "+v"(a) : "v"(b)); }
+__device__ __forceinline__ void ph_amax(float &a, float b) { __asm__ volatile("v_max_f32 %0, %0, %1" : "+v"(a) : "v"(b)); }
+// max of two values that are never signalling NaNs, by the instruction itself
+__device__ __forceinline__ double ph_amax2(double a, double b) { double r; __asm__("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b)); return r; }
+__device__ __forceinline__ float ph_amax2(float a, float b) { float r; __asm__("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b)); return r; }
+template <typename F, bool MAX, bool RAW = false> __device__ __forceinline__ F ph_comb2(F a, F b) {
+  if (MAX) return RAW ? ph_amax2(a, b) : ph_fmax(a, b);
   return a + b;
 }
 // Cross-lane moves.  v_permlane32_swap(a, b) exchanges lanes 32-63 of a with
@@ -850,21 +900,23 @@ template <int C> __device__ __forceinline__ double ph_dpp(double x) {
 // (lane bits 5, 4, 3) lane l holds 8 lanes' totals for row (l >> 3) & 7, and
 // a fixed three-level tree inside each 8-lane group (half mirror, then lane
 // bits 1 and 0) completes the row over all 64 lanes.
-template <typename F, bool MAX> __device__ __forceinline__ F tfold8(F (&v)[8]) {
+// (RAW: maxima by the instruction itself, for values that are never
+// signalling NaNs; values moved across lanes would otherwise be re-quieted)
+template <typename F, bool MAX, bool RAW = false> __device__ __forceinline__ F tfold8(F (&v)[8]) {
   const int lane = threadIdx.x & 63;
   F a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], b0 = v[4], b1 = v[5], b2 = v[6], b3 = v[7];
   ph_swap32(a0, b0); ph_swap32(a1, b1); ph_swap32(a2, b2); ph_swap32(a3, b3);
   // lanes 0-31: rows 0-3, lanes 32-63: rows 4-7
-  F w0 = ph_comb2<F, MAX>(a0, b0), w1 = ph_comb2<F, MAX>(a1, b1);
-  F w2 = ph_comb2<F, MAX>(a2, b2), w3 = ph_comb2<F, MAX>(a3, b3);
+  F w0 = ph_comb2<F, MAX, RAW>(a0, b0), w1 = ph_comb2<F, MAX, RAW>(a1, b1);
+  F w2 = ph_comb2<F, MAX, RAW>(a2, b2), w3 = ph_comb2<F, MAX, RAW>(a3, b3);
   ph_swap16(w0, w2); ph_swap16(w1, w3);
-  const F y0 = ph_comb2<F, MAX>(w0, w2), y1 = ph_comb2<F, MAX>(w1, w3);
+  const F y0 = ph_comb2<F, MAX, RAW>(w0, w2), y1 = ph_comb2<F, MAX, RAW>(w1, w3);
   const bool b8 = (lane & 8) != 0;
   const F keep = b8 ? y1 : y0, send = b8 ? y0 : y1;
-  F x = ph_comb2<F, MAX>(keep, ph_dpp<0x128>(send));
-  x = ph_comb2<F, MAX>(x, ph_dpp<0x141>(x));
-  x = ph_comb2<F, MAX>(x, ph_dpp<0x4E>(x));
-  x = ph_comb2<F, MAX>(x, ph_dpp<0xB1>(x));
+  F x = ph_comb2<F, MAX, RAW>(keep, ph_dpp<0x128>(send));
+  x = ph_comb2<F, MAX, RAW>(x, ph_dpp<0x141>(x));
+  x = ph_comb2<F, MAX, RAW>(x, ph_dpp<0x4E>(x));
+  x = ph_comb2<F, MAX, RAW>(x, ph_dpp<0xB1>(x));
   return x;
 }
 __device__ __forceinline__ void st_add(SI &a, i64 v) {
@@ -988,6 +1040,15 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
   const int V = P.vec;
   const bool sd = std::string(P.state) == "SD";
   const char *M = (sd || p->out_dtype == PH_F64) ? "double" : "float";
+  // The maxima take quiet operands only: a value that may be a signalling
+  // NaN (a load passed through) is quieted once, by canonicalisation, where
+  // it is computed (the compiler's max leaves the result for a signalling
+  // operand unspecified, and a row's chain then lost its earlier values).
+  // The column maxima are then taken by a v_max instruction in an asm
+  // statement, which also keeps each column's chain in row order; the
+  // compiler's own max would re-quiet a value coming out of an asm
+  // statement, one extra instruction per pinned maximum.
+  const bool quiet = max_operand_quiet(p, std::string(M) == "double" && p->out_dtype != PH_F64);
   const bool full_cols = P.cols % (256 * V) == 0;
   const bool full_rows = P.rows % P.tr == 0;
   const std::string C = lit_i64(P.cols);
@@ -1073,9 +1134,11 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
     o += fmt("#pragma unroll\n        for (int v = 0; v < %d; ++v) {\n          const i64 c = %s;\n", V, colv("v").c_str());
     o += "          {\n";
     o += body;
-    o += fmt("            const double val = (double)(%s); const M mval = (M)(%s);\n", res.c_str(), res.c_str());
+    o += fmt(quiet ? "            const double val = (double)(%s); const M mval = (M)(%s);\n"
+                   : "            const double val = (double)(%s); const M mval = ph_quiet((M)(%s));\n",
+             res.c_str(), res.c_str());
     // (columns past the row's end accumulate values that are never written)
-    if (P.do_cols) o += "            csum[v] += val; cmax[v] = ph_fmax(cmax[v], mval);\n";
+    if (P.do_cols) o += "            csum[v] += val; ph_amax(cmax[v], mval);\n";
     if (P.do_rows) {
       // with every column present the lane's first column starts the row
       // (the row's state still starts from +0.0 where it is combined);
@@ -1089,17 +1152,17 @@ std::string red_tile_float(const ph_prog *p, int n_in, const int *in_dt, const i
     if (P.do_rows && lane_cols) o += "      ls = lane_in ? ls : 0.0; lm = lane_in ? lm : (M)-__builtin_inf();\n";
     if (P.do_rows) o += "      rsv[k] = ls; rmv[k] = lm;\n";
     if (P.do_cols) {
-      // The column sums and maxima are brought up to date row by row (sums
-      // every row, maxima every second row): an empty asm that takes the
-      // running value as in-out operand.  Left alone, the compiler defers
-      // the column chains to the end of the group and holds every element's
-      // value of 8 rows until then, which does not fit in registers.
+      // The column sums are brought up to date row by row: an empty asm
+      // that takes the running value as in-out operand (the maxima are
+      // updated by their own asm statement).  Left alone, the compiler
+      // defers the column chains to the end of the group and holds every
+      // element's value of 8 rows until then, which does not fit in
+      // registers.
       o += fmt("#pragma unroll\n      for (int v = 0; v < %d; ++v) __asm__ volatile(\"\" : \"+v\"(csum[v]));\n", V);
-      o += fmt("      if (k & 1) {\n#pragma unroll\n        for (int v = 0; v < %d; ++v) __asm__ volatile(\"\" : \"+v\"(cmax[v]));\n      }\n", V);
     }
     o += "    }\n";
     if (P.do_rows) {
-      o += "    const double fs = tfold8<double, false>(rsv);\n    const M fm = tfold8<M, true>(rmv);\n";
+      o += "    const double fs = tfold8<double, false>(rsv);\n    const M fm = tfold8<M, true, true>(rmv);\n";
       o += "    const int rr = (lane >> 3) & 7;\n";
       o += "    if ((lane & 7) == 0 && " + g + " + rr < r1) { rsum_sh[" + g + " - r0 + rr][wave] = fs; rmax_sh[" + g +
            " - r0 + rr][wave] = fm; }\n";

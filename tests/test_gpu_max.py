@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_api as O
+from crystal import trace
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +99,32 @@ def test_max_of_rank3_slices(ph):
     for k in range(3):
         _same(pv.slices_max(k), O.slices_reduce(h, k, 1))
     _same([pv.max()], [O.max_seq(h)])
+
+
+@pytest.mark.parametrize("dt", ["F64", "F32"])
+def test_signalling_nan_inputs(ph, dt):
+    """A signalling NaN in the data (arithmetic never makes one) is skipped
+    like any NaN that is not first.  The blocks cover plain loads,
+    identities the compiler folds into a pass-through (x * 1.0, x - 0.0),
+    whose maxima must re-quiet their operands, and arithmetic (x * 2.0,
+    (x + 4) ** 2 * 0.5 - x), whose maxima take the instruction as is.  Each
+    signalling NaN sits after its column's and its row's largest element,
+    so a maximum that let it through would lose those."""
+    rng = np.random.default_rng(97)
+    npdt, ubits, snan = ((np.float64, np.uint64, 0x7FF0000000000001) if dt == "F64"
+                         else (np.float32, np.uint32, 0x7F800001))
+    h = rng.standard_normal((300, 500)).astype(npdt)
+    for r, c in [(5, 7), (120, 33), (299, 499), (0, 3)]:
+        h[r - 3 if r >= 3 else r + 1, c] = 50 + r
+        h[r, c - 2] = 60 + r
+        h.view(ubits)[r, c] = ubits(snan)
+    h.view(ubits)[100, 0] = ubits(snan)  # first element of row 100
+    x = ph.NArray.from_numpy(h)
+    f64 = h.astype(np.float64)
+    blocks = [("load", None), ("x * 1.0", lambda v: v * 1.0), ("x - 0.0", lambda v: v - 0.0),
+              ("x * 2.0", lambda v: v * 2.0), ("C3", lambda v: (v + 4) ** 2 * 0.5 - v)]
+    for name, fn in blocks:
+        print(f"signalling NaN, {dt}, block {name}", flush=True)
+        pv = x.view() if fn is None else x.process(fn)
+        f = f64 if fn is None else O.eval_expr(trace(fn), [h])[0].astype(np.float64)
+        _check_all(pv, f)
