@@ -1768,8 +1768,13 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 //    unique and is the sequential result, and an element whose dependency
 //    chain is d deep is exact from T_d on.  A pass that reproduces T bit for
 //    bit has reached it, and its flags are the sequential run's.  Past
-//    kOrdMaxIter passes (a deep chain, such as a shift by one element), one
-//    device thread runs the loop in order, kSeqChunk elements per launch.
+//    kOrdMaxIter passes (a deep chain, such as a shift by one element):
+//    with one input on the output's buffer, every element depends on at
+//    most one earlier element, and when none has two dependents the
+//    dependencies are chains; a thread per chain root then runs its chain
+//    in order, carrying each result to the next element (a recurrence down
+//    the rows of a matrix runs a thread per column).  Otherwise one device
+//    thread runs the loop in order, kSeqChunk elements per launch.
 // A plain copy takes pointer jumping instead of step 3: element L's value is
 // the old value at the read address of its chain's root, and
 // ceil(log2(depth)) passes over the chain links find every root.
@@ -1959,6 +1964,17 @@ struct OrdSpec {
   bool any_index() const { return indexed != 0 || out_indexed; }
 };
 
+// The one input of S that reads the output's buffer, or -1 (none, or more).
+int single_hazard(const OrdSpec &S) {
+  int h = -1;
+  for (int j = 0; j < S.n_in; ++j)
+    if (S.hazard >> j & 1) {
+      if (h >= 0) return -1;
+      h = j;
+    }
+  return h;
+}
+
 std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
   const int64_t n = ph::desc_count(S.out);
   const std::string N = lit_i64(n);
@@ -2081,6 +2097,55 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
     s += emit_coords(S.out->view, "R", "") + in_addr(0, "R");
     s += fmt("      ((%s *)A.tn)[L] = ((const %s *)A.in[0])[a_0];\n    }\n}\n", OST.c_str(), OST.c_str());
   }
+  const int hz1 = single_hazard(S);
+  if (S.prog && hz1 >= 0) {
+    // Chains of a map with one input on the output's buffer: element L
+    // depends on at most the one earlier element that wrote the address
+    // it reads (q0[L], or -1).  When no element has two dependents the
+    // dependencies form chains, and a thread per chain root runs its chain
+    // in order, carrying each result to the next element as the value it
+    // reads; the results go to T (tn), indexed by L.
+    const std::string sx = fmt("_%d", hz1);
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_mlink(OrdArgs A) {\n";
+    s += loop + emit_coords(S.out->view, "L", "") + in_addr(hz1, "L") + inverse(hz1);
+    s += fmt("      A.q1[L] = (ok%s && lp%s < L) ? lp%s : -1;\n    }\n}\n", sx.c_str(), sx.c_str(), sx.c_str());
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_unset(OrdArgs A) {\n" + loop;
+    s += "      A.q1[L] = -1;\n    }\n}\n";
+    // next pointers: q1[q0[L]] = L; a second dependent of one element is a
+    // conflict (the dependencies then form a tree)
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_next(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    s += "      const i64 p = A.q0[L];\n      if (p >= 0) {\n";
+    s += "        const unsigned long long old = atomicCAS((unsigned long long *)(A.q1 + p), ~0ull, (unsigned long long)L);\n";
+    s += "        if (old != ~0ull) ctl |= 1;\n      }\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    // walk: q0 the links, q1 the next pointers; every element lies on the
+    // chain of exactly one root, and next pointers only increase, so each
+    // walk ends
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_walk(OrdArgs A) {\n  int fl = 0;\n" + loop;
+    s += "      if (A.q0[L] >= 0) continue;\n";
+    s += fmt("      %s carry = %s(0);\n      bool from_chain = false;\n", vtype(hz1).c_str(), vtype(hz1).c_str());
+    s += "      for (i64 E = L; E >= 0; E = A.q1[E]) {\n";
+    std::string body_e = emit_coords(S.out->view, "E", "");
+    for (int j = 0; j < S.n_in; ++j) {
+      body_e += in_addr(j, "E");
+      if (j == hz1)
+        body_e += fmt("      const %s x%d = from_chain ? carry : %s;\n", vtype(j).c_str(), j,
+                      load(j, fmt("A.in[%d]", j), fmt("a_%d", j)).c_str());
+      else
+        body_e += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), fmt("a_%d", j)).c_str());
+    }
+    // the program reads the element index as L: give it E
+    std::string b;
+    {
+      std::vector<std::string> inexpr;
+      for (int j = 0; j < S.n_in; ++j) inexpr.push_back(fmt("x%d", j));
+      const std::string r = emit_prog(S.prog, inexpr, &b, "");
+      body_e += "      const i64 ph_lidx = E; (void)ph_lidx;\n" + b + fmt("      const auto res = %s;\n", r.c_str());
+    }
+    s += body_e;
+    s += fmt("      ph_stp((%s *)A.tn, E, res);\n", OST.c_str());
+    s += fmt("      carry = (%s)res;\n      from_chain = true;\n      }\n    }\n", vtype(hz1).c_str());
+    s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+  }
   if (S.any_index()) {
     // check: every index inside its buffer, and the span the writes cover
     // (mm preset to {extent, -1} by ph_ord_mm_init)
@@ -2136,6 +2201,7 @@ struct OrdKernels {
   hipFunction_t link = nullptr, jump = nullptr, gather = nullptr;
   hipFunction_t check = nullptr, direct = nullptr, mm_init = nullptr, inv_clear = nullptr, inv_fill = nullptr,
                 inv_check = nullptr;
+  hipFunction_t mlink = nullptr, unset = nullptr, next = nullptr, walk = nullptr;
 };
 
 std::map<std::string, OrdKernels> &ord_cache() {
@@ -2170,6 +2236,11 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_clear", &K->inv_clear);
   if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_fill", &K->inv_fill);
   if (!rc && S.out_indexed) rc = ph::jit_kernel2(src, "ph_ord_inv_check", &K->inv_check);
+  const bool chains = S.prog && single_hazard(S) >= 0;
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_mlink", &K->mlink);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_unset", &K->unset);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_next", &K->next);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_walk", &K->walk);
   if (rc) return rc;
   ord_cache()[key] = *K;
   return PH_OK;
@@ -2343,6 +2414,29 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
       if ((it == 0 && !(words[1] & kCtlRaw)) || (it > 0 && !(words[1] & kCtlChanged))) {
         exact = true;
         break;
+      }
+    }
+    if (!exact && K.walk) {
+      // chains deeper than the passes reach: one thread per chain, unless
+      // some element has two dependents
+      void *q[2];
+      rc = ph::scratch((size_t)n * sizeof(int64_t), &q[0], 6);
+      if (!rc) rc = ph::scratch((size_t)n * sizeof(int64_t), &q[1], 7);
+      if (rc) return rc;
+      A.q1 = (int64_t *)q[0];
+      rc = ord_launch(K.mlink, A, n, false, "ordered map (links)");
+      if (rc) return rc;
+      A.q0 = (const int64_t *)q[0];
+      A.q1 = (int64_t *)q[1];
+      rc = ord_launch(K.unset, A, n, false, "ordered map (next)");
+      if (!rc) rc = ord_pass(K.next, A, n, words, "ordered map (next)");
+      if (rc) return rc;
+      if (!(words[1] & kCtlConflict)) {
+        A.tn = t[cur];
+        rc = ord_pass(K.walk, A, n, words, "ordered map (chains)");
+        if (rc) return rc;
+        flags = words[0];
+        exact = true;
       }
     }
     if (!exact) return ord_sequential(K, A, n, flags_out);

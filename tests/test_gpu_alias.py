@@ -226,3 +226,52 @@ def test_random_aliased_chains_against_the_oracle(ph):
         assert np.array_equal(z.to_numpy(), ref), (shape, wchain, rchain, use_map)
         done += 1
     assert done >= 25
+
+
+def test_recurrence_down_the_rows_runs_per_column(ph):
+    """z[1.., ..] = z[..-2, ..].process { |v| v + 1 }: each column is a chain
+    of n - 1 elements, each reading the one just written above it, so
+    z[i, j] = z0[0, j] + i.  The chains are deeper than the relaxation
+    passes reach, and each runs on a thread of its own (one thread for
+    the whole array took seconds at 4096^2)."""
+    rng = np.random.default_rng(41)
+    for n, m in [(2048, 2048), (4096, 4096)]:
+        h = _i32(rng, (n, m))
+        z = ph.NArray.from_numpy(h)
+        ph.sync()
+        t0 = time.perf_counter()
+        z[["1..", ".."]] = z.view("..-2", "..").process(lambda v: v + 1)
+        ph.sync()
+        dt = time.perf_counter() - t0
+        exp = h[0][None, :].astype(np.int64) + np.arange(n)[:, None]
+        assert np.array_equal(z.to_numpy(), exp.astype(np.int32)), n
+        print(f"row recurrence {n}x{m}: {dt * 1e3:.1f} ms")
+        assert dt < 2.0
+    # a block that is not affine, against the oracle's loop
+    h = _i32(rng, (300, 257), -10 ** 6, 10 ** 6)
+    fn = lambda v: (v ^ 21) - 3  # noqa: E731
+    z = ph.NArray.from_numpy(h)
+    z[["1..", ".."]] = z.view("..-2", "..").process(fn)
+    ref = h.copy()
+    dst = O.region(P(["1..", ".."]), [300, 257])
+    src = O.region(P(["..-2", ".."]), [300, 257])
+    O.set_chunk_lazy(ref, [], dst, trace(fn), [(ref, [("region", src)])], src.shape)
+    assert np.array_equal(z.to_numpy(), ref)
+
+
+def test_broadcast_recurrence_forms_a_tree(ph):
+    """z[1.., ..] = z[..-2, 0..0].tile(1, m).process { ... }: every element
+    of row i reads z[i - 1, 0], so that element has m dependents (a tree,
+    not chains) and the loop runs in order; row i ends up f^i(z0[0, 0])."""
+    rng = np.random.default_rng(43)
+    n, m = 40, 33
+    h = _i32(rng, (n, m))
+    fn = lambda v: (v ^ 21) - 3  # noqa: E731
+    z = ph.NArray.from_numpy(h)
+    z[["1..", ".."]] = z.view("..-2", "0..0").tile(1, m).process(fn)
+    exp = h.copy()
+    v = int(h[0, 0])
+    for i in range(1, n):
+        v = (v ^ 21) - 3
+        exp[i, :] = v
+    assert np.array_equal(z.to_numpy(), exp)
