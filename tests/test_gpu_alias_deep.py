@@ -316,3 +316,67 @@ def test_map_indexed_rejects_bad_indexes(ph):
     rc = _copy_indexed(lib, buf, 100, _dev_i64(ph, np.arange(100)), _dev_i64(ph, w))
     assert rc == -4 and b"twice" in lib.ph_last_error()
     assert np.array_equal(buf.to_numpy(), h)  # nothing written
+
+
+class _Insn(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint8), ("dt", ctypes.c_uint8), ("dst", ctypes.c_uint8), ("a", ctypes.c_uint8),
+                ("b", ctypes.c_uint8), ("c", ctypes.c_uint8), ("pad_", ctypes.c_uint16), ("imm", ctypes.c_int64)]
+
+
+class _Prog(ctypes.Structure):
+    _fields_ = [("n_insn", ctypes.c_int32), ("n_in", ctypes.c_int32), ("out_reg", ctypes.c_int32),
+                ("out_dtype", ctypes.c_int32), ("insn", _Insn * 48)]
+
+
+def _prog(ph, e, n_in):
+    """A ph_prog from the host's lowering of expression `e`."""
+    insns, out_reg, out_dt = ph.program(e, n_in)
+    p = _Prog()
+    p.n_insn, p.n_in, p.out_reg, p.out_dtype = len(insns), n_in, out_reg, int(out_dt)
+    for k, (op, dt, dst, a, b, c, imm) in enumerate(insns):
+        I = p.insn[k]
+        I.op, I.dt, I.dst, I.a, I.b, I.c = op, dt, dst, a, b, c
+        I.imm = (np.array([imm], np.float64).view(np.int64)[0] if isinstance(imm, float) else int(imm))
+    return p
+
+
+def test_map_indexed_programs_follow_the_loop(ph):
+    """ph_map_indexed with a program, input 0 read from the output's buffer
+    through an index array: buf[w[k]] = f(buf[r[k]], y[k]) for k in order.
+    One chain of 6000 dependent elements (a shift by one), 64 interleaved
+    chains (a shift by 64: a thread per chain), the same with one element
+    read by two later ones (a tree: the in-order loop), and a second input
+    y read through a plain descriptor from another buffer."""
+    lib = _lib(ph)
+    lib.ph_map_indexed.argtypes = [ctypes.POINTER(_Prog), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                   ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    I = ph.DType.I32
+    e = ((ph.input(0, I) ^ 21) - 3) + ph.input(1, I)
+    prog = _prog(ph, e, 2)
+    rng = np.random.default_rng(11)
+    n = 6000
+    cases = {"one chain": np.r_[0, np.arange(n - 1)], "64 chains": np.r_[np.arange(64), np.arange(n - 64)]}
+    tree = cases["64 chains"].copy()
+    tree[200] = 72  # element 72 is read by 136 and by 200
+    cases["tree"] = tree
+    for name, r in cases.items():
+        print(f"indexed program: {name}", flush=True)
+        h = rng.integers(-1000, 1000, n).astype(np.int32)
+        yh = rng.integers(-3, 4, n).astype(np.int32)
+        w = np.arange(n)
+        buf = ph.NArray.from_numpy(h)
+        y = ph.NArray.from_numpy(yh)
+        ri, wi = _dev_i64(ph, r), _dev_i64(ph, w)
+        ins = (ctypes.c_void_p * 2)(buf.device_ptr, y.device_ptr)
+        ds = (_Desc * 2)(_desc(0, n, [n]), _desc(0, n, [n]))
+        ix = (ctypes.c_void_p * 2)(ri.device_ptr, None)
+        od = _desc(0, n, [n])
+        fl = ctypes.c_int32(0)
+        rc = lib.ph_map_indexed(ctypes.byref(prog), 2, ins, ds, ix, buf.device_ptr, ctypes.byref(od), wi.device_ptr,
+                                ctypes.byref(fl))
+        assert rc == 0 and fl.value == 0, lib.ph_last_error()
+        ref = h.astype(np.int64)
+        for k in range(n):
+            ref[w[k]] = ((ref[r[k]] ^ 21) - 3) + yh[k]
+        assert np.array_equal(buf.to_numpy(), ref.astype(np.int32)), name
