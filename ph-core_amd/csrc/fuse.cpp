@@ -1393,8 +1393,15 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
       const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "0", "c", "_cf", &s);
       s += "      f0 = " + e_ + ";\n    }\n";
     }
-    s += fmt("    if (c < %s) {\n#pragma unroll 4\n      for (i64 t = t0; t < t1; ++t) a = st_comb(a, ((const S *)A.colpart)[t * %s + c]);\n    }\n",
-             lit_i64(P.cols).c_str(), lit_i64(P.cols).c_str());
+    // (the states are loaded eight at a time before they are folded in
+    // order, so the loads of a group are in flight together)
+    const int64_t per = (P.ntr_fin + 3) / 4;
+    s += fmt("    if (c < %s) {\n#pragma unroll\n      for (int g = 0; g < %s; g += 8) {\n        S v[8];\n"
+             "#pragma unroll\n        for (int u = 0; u < 8; ++u) {\n"
+             "          const i64 t = t0 + g + u;\n          if (g + u < %s && t < t1) v[u] = ((const S *)A.colpart)[t * %s + c];\n        }\n"
+             "#pragma unroll\n        for (int u = 0; u < 8; ++u) if (g + u < %s && t0 + g + u < t1) a = st_comb(a, v[u]);\n      }\n    }\n",
+             lit_i64(P.cols).c_str(), lit_i64(per).c_str(), lit_i64(per).c_str(), lit_i64(P.cols).c_str(),
+             lit_i64(per).c_str());
     s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
     s += fmt("    if (part == 0 && c < %s) {\n", lit_i64(P.cols).c_str());
     s += "      S r = sh[lane];\n      for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
@@ -1435,8 +1442,11 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
         s += "      ff0 = " + e2 + ";\n    }\n";
       }
     }
-    s += fmt("    if (i < %s) {\n#pragma unroll 4\n      for (i64 t = 0; t < %s; ++t) a = st_comb(a, ((const S *)A.rowpart)[t * %s + i]);\n",
-             lit_i64(P.rows).c_str(), lit_i64(P.ntc_fin).c_str(), lit_i64(P.rows).c_str());
+    s += fmt("    if (i < %s) {\n#pragma unroll\n      for (int g = 0; g < %s; g += 8) {\n        S v[8];\n"
+             "#pragma unroll\n        for (int u = 0; u < 8; ++u) if (g + u < %s) v[u] = ((const S *)A.rowpart)[(g + u) * %s + i];\n"
+             "#pragma unroll\n        for (int u = 0; u < 8; ++u) if (g + u < %s) a = st_comb(a, v[u]);\n      }\n",
+             lit_i64(P.rows).c_str(), lit_i64(P.ntc_fin).c_str(), lit_i64(P.ntc_fin).c_str(), lit_i64(P.rows).c_str(),
+             lit_i64(P.ntc_fin).c_str());
     s += "      if (A.row_sum) { A.row_sum[i] = st_sum(a); fl |= st_ovf(a); }\n";
     if (fix) {
       // first element and, when the row's maximum is a zero, the first zero
