@@ -1403,22 +1403,37 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
              lit_i64(P.cols).c_str(), lit_i64(per).c_str(), lit_i64(per).c_str(), lit_i64(P.cols).c_str(),
              lit_i64(per).c_str());
     s += "    sh[threadIdx.x] = a;\n    __syncthreads();\n";
-    s += fmt("    if (part == 0 && c < %s) {\n", lit_i64(P.cols).c_str());
-    s += "      S r = sh[lane];\n      for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
-    s += "      if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n";
     if (fix) {
       // Enumerable#max keeps the slice's first element unless a later one is
-      // greater: a leading NaN stays, and of equal zeros the first one's sign
-      s += "      if (A.col_max) {\n        double mx = st_max(r);\n";
-      s += "        if (f0 != f0) mx = f0;\n";
-      s += fmt("        else if (mx == 0.0) {\n          for (i64 rr = 0; rr < %s; ++rr) {\n", lit_i64(P.rows).c_str());
+      // greater: a leading NaN stays, and of equal zeros the first one's
+      // sign.  Columns whose maximum is a zero are scanned by the whole
+      // wavefront in turn, 64 rows at a time, for their first zero.
+      s += "    if (part == 0) {\n      double mx = 0.0;\n      bool mzero = false;\n";
+      s += fmt("      if (c < %s) {\n", lit_i64(P.cols).c_str());
+      s += "        S r = sh[lane];\n        for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
+      s += "        if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n";
+      s += "        mx = st_max(r);\n        mzero = A.col_max && f0 == f0 && mx == 0.0;\n      }\n";
+      s += "      unsigned long long need = __ballot(mzero);\n";
+      s += "      while (need) {\n        const int l = __builtin_ctzll(need);\n        need &= need - 1;\n";
+      s += "        const i64 cl = (i64)blockIdx.x * 64 + l;\n";
+      s += fmt("        bool found = false;\n        for (i64 r1 = 0; r1 < %s && !found; r1 += 512) {\n          double g[8];\n"
+               "#pragma unroll\n          for (int k = 0; k < 8; ++k) {\n            const i64 rr = r1 + 64 * k + lane;\n"
+               "            g[k] = 1.0;\n            if (rr < %s) {\n", lit_i64(P.rows).c_str(), lit_i64(P.rows).c_str());
       {
-        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "rr", "c", "_cz", &s);
-        s += "            const double g = " + e_ + ";\n";
+        std::string ev;
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "rr", "cl", "_cz", &ev);
+        s += ev + "              g[k] = " + e_ + ";\n            }\n          }\n";
       }
-      s += "            if (g == 0.0) { mx = g; break; }\n          }\n        }\n";
-      s += "        A.col_max[c] = mx;\n      }\n";
+      s += "#pragma unroll\n          for (int k = 0; k < 8; ++k) {\n";
+      s += "            const unsigned long long hit = __ballot(g[k] == 0.0);\n            if (hit && !found) {\n";
+      s += "              const double gz = __shfl(g[k], __builtin_ctzll(hit));\n";
+      s += "              if (lane == l) mx = gz;\n              found = true;\n            }\n          }\n        }\n      }\n";
+      s += fmt("      if (c < %s && A.col_max) {\n        if (f0 != f0) mx = f0;\n        A.col_max[c] = mx;\n      }\n",
+               lit_i64(P.cols).c_str());
     } else {
+      s += fmt("    if (part == 0 && c < %s) {\n", lit_i64(P.cols).c_str());
+      s += "      S r = sh[lane];\n      for (int q = 1; q < 4; ++q) r = st_comb(r, sh[q * 64 + lane]);\n";
+      s += "      if (A.col_sum) { A.col_sum[c] = st_sum(r); fl |= st_ovf(r); }\n";
       s += "      if (A.col_max) A.col_max[c] = st_max(r);\n";
     }
     s += "    }\n  } else {\n";
@@ -1442,6 +1457,7 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
         s += "      ff0 = " + e2 + ";\n    }\n";
       }
     }
+    if (fix) s += "    double mx = 0.0;\n    bool mzero = false;\n";
     s += fmt("    if (i < %s) {\n#pragma unroll\n      for (int g = 0; g < %s; g += 8) {\n        S v[8];\n"
              "#pragma unroll\n        for (int u = 0; u < 8; ++u) if (g + u < %s) v[u] = ((const S *)A.rowpart)[(g + u) * %s + i];\n"
              "#pragma unroll\n        for (int u = 0; u < 8; ++u) if (g + u < %s) a = st_comb(a, v[u]);\n      }\n",
@@ -1451,17 +1467,33 @@ std::string red_source(const ph_prog *p, int n_in, const int *in_dt, const int64
     if (fix) {
       // first element and, when the row's maximum is a zero, the first zero
       // (its view-order index and sign also serve the full result)
-      s += "      const double m0 = st_max(a);\n      double mx = m0;\n";
-      s += fmt("      if (m0 == 0.0) {\n        for (i64 cc = 0; cc < %s; ++cc) {\n", lit_i64(P.cols).c_str());
+      s += "      mx = st_max(a);\n      mzero = mx == 0.0;\n    }\n";
+      // Rows of this wavefront whose maximum is a zero: the wavefront scans
+      // each in turn, 64 consecutive columns at a time, and the lowest lane
+      // holding a zero has the row's first one.  (A zero maximum is one of
+      // the row's values, so the scan ends within the row.)
+      s += "    {\n      const int lane = threadIdx.x & 63;\n      unsigned long long need = __ballot(mzero);\n";
+      s += "      while (need) {\n        const int l = __builtin_ctzll(need);\n        need &= need - 1;\n";
+      s += "        const i64 ri = i - lane + l;\n";
+      // (eight chunks of 64 columns are loaded per round, then checked in order)
+      s += fmt("        bool found = false;\n        for (i64 c1 = 0; c1 < %s && !found; c1 += 512) {\n          double g[8];\n"
+               "#pragma unroll\n          for (int k = 0; k < 8; ++k) {\n            const i64 cc = c1 + 64 * k + lane;\n"
+               "            g[k] = 1.0;\n            if (cc < %s) {\n", lit_i64(P.cols).c_str(), lit_i64(P.cols).c_str());
       {
-        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "i", "cc", "_rz", &s);
-        s += "          const double g = " + e_ + ";\n";
+        std::string ev;
+        const std::string e_ = emit_elem_at(p, n_in, in_dt, rs, cs, "ri", "cc", "_rz", &ev);
+        s += ev + "              g[k] = " + e_ + ";\n            }\n          }\n";
       }
-      const std::string key = P.colex ? fmt("cc * %s + i", lit_i64(P.rows).c_str()) : fmt("i * %s + cc", lit_i64(P.cols).c_str());
-      s += "          if (g == 0.0) { mx = g; zkey = ((" + key + ") << 1) | (__builtin_signbit(g) ? 1 : 0); break; }\n";
-      s += "        }\n      }\n      if (f0 != f0) mx = f0;\n";
-      s += "      if (A.row_max) A.row_max[i] = mx;\n";
-      s += "    }\n";
+      const std::string key =
+          P.colex ? fmt("zc * %s + ri", lit_i64(P.rows).c_str()) : fmt("ri * %s + zc", lit_i64(P.cols).c_str());
+      s += "#pragma unroll\n          for (int k = 0; k < 8; ++k) {\n";
+      s += "            const unsigned long long hit = __ballot(g[k] == 0.0);\n            if (hit && !found) {\n";
+      s += "              const int hl = __builtin_ctzll(hit);\n              const double gz = __shfl(g[k], hl);\n";
+      s += "              const i64 zc = c1 + 64 * k + hl;\n";
+      s += "              if (lane == l) { mx = gz; zkey = ((" + key + ") << 1) | (__builtin_signbit(gz) ? 1 : 0); }\n";
+      s += "              found = true;\n            }\n          }\n        }\n      }\n    }\n";
+      s += fmt("    if (i < %s) {\n      if (f0 != f0) mx = f0;\n      if (A.row_max) A.row_max[i] = mx;\n    }\n",
+               lit_i64(P.rows).c_str());
     } else {
       s += "      if (A.row_max) A.row_max[i] = st_max(a);\n    }\n";
     }

@@ -155,3 +155,39 @@ def test_tall_columns_per_slice(ph):
     _close(r["col_sum"], [O.sum_exact(h[:, c]) for c in range(cols)])
     _same(r["col_max"], O.slices_reduce(h, 1, 1))
     _close([r["full_sum"]], [O.sum_exact(h)])
+
+
+def test_zero_maximum_found_far_along_long_series(ph):
+    """Series whose maximum is a zero first reached near their end: the
+    first zero's sign is the result (Enumerable#max keeps it), found by a
+    wavefront scanning 512 elements per round.  A prime-length rank-1
+    array (one long row), a column of a tall array and a row of a wide
+    one."""
+    rng = np.random.default_rng(26)
+    n = _prime_at_least((1 << 24) + 99)
+    h = -np.abs(rng.standard_normal(n)) - 0.5
+    h[n - 3], h[n - 1] = -0.0, 0.0
+    x = ph.NArray.from_numpy(h)
+    x.max()
+    ph.sync()
+    t0 = time.perf_counter()
+    got = x.max()
+    dt = time.perf_counter() - t0
+    assert got == 0.0 and np.signbit(got)
+    print(f"zero maximum found at {n - 3} of {n}: {dt * 1e3:.1f} ms")
+    assert dt < 1.0
+    rows, cols = (1 << 21) + 5, 3
+    t = -np.abs(rng.standard_normal((rows, cols))) - 0.5
+    t[rows - 2, 1], t[rows - 1, 1] = -0.0, 0.0
+    t[rows - 1, 2] = 0.0
+    xt = ph.NArray.from_numpy(t)
+    _same(xt.slices_max(1), O.slices_reduce(t, 1, 1))
+    _same(xt.reduce2d_all()["col_max"], O.slices_reduce(t, 1, 1))
+    w = -np.abs(rng.standard_normal((3, (1 << 22) + 3))) - 0.5
+    w[2, -5], w[2, -2] = 0.0, -0.0
+    w[0, -1] = -0.0
+    xw = ph.NArray.from_numpy(w)
+    _same(xw.slices_max(0), O.slices_reduce(w, 0, 1))
+    r = xw.reduce2d_all()
+    _same(r["row_max"], O.slices_reduce(w, 0, 1))
+    _same([r["full_max"]], [O.max_seq(w)])
