@@ -383,22 +383,39 @@ std::string batch_source(const GBatch &B) {
   s += "      default: break;\n    }\n";
   s += "    if (A.dynamic) {\n      int t = 0;\n      if (lane == 0) t = atomicAdd(A.counter, 1);\n"
        "      q = __builtin_amdgcn_readfirstlane(t);\n    } else {\n      q += nwaves;\n    }\n  }\n}\n";
-  // per array: its items' partial results folded in item order
+  // per array (a wavefront each): lane l folds a run of consecutive items
+  // in order, then the lanes' results are folded in lane order (a fixed
+  // tree), so each result depends only on the item partials
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_gbatch_final(GArgs A) {\n";
-  s += "  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;\n  if (k >= A.n) return;\n";
+  s += "  const int k = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);\n";
+  s += "  if (k >= A.n) return;\n";
   s += "  const int ops = A.rops[k];\n  const int b = A.first[k], e = A.first[k + 1];\n";
-  s += "  if ((ops & 5) == 5 && A.sums) {  // Int32: checked running totals, items in order\n"
-       "    long long t = 0; bool ov = false;\n"
-       "    for (int i = b; i < e; ++i) {\n"
-       "      if (t + A.pint[3 * i + 1] < -2147483648LL || t + A.pint[3 * i + 2] > 2147483647LL) ov = true;\n"
-       "      t += A.pint[3 * i];\n    }\n"
-       "    A.sums[k] = (double)t;\n    if (ov) atomicOr(A.counter + 1, 1);\n"
-       "  } else if ((ops & 1) && A.sums) {\n    double t = 0.0;\n    for (int i = b; i < e; ++i) t += A.psum[i];\n"
-       "    A.sums[k] = t;\n  }\n";
-  s += "  if ((ops & 6) == 6 && A.maxes && e > b) {\n    double v = A.pmv[b];\n"
-       "    for (int i = b + 1; i < e; ++i) v = A.pmv[i] > v ? A.pmv[i] : v;\n    A.maxes[k] = v;\n"
-       "  } else if ((ops & 2) && A.maxes && e > b) {\n    double v = A.pmv[b]; unsigned ix = A.pmi[b];\n"
-       "    for (int i = b + 1; i < e; ++i) g_max_comb(v, ix, A.pmv[i], A.pmi[i]);\n    A.maxes[k] = v;\n  }\n}\n";
+  s += "  const int per = (e - b + 63) / 64;\n";
+  s += "  const int lb = b + lane * per < e ? b + lane * per : e, le = lb + per < e ? lb + per : e;\n";
+  s += "  if ((ops & 5) == 5 && A.sums) {  // Int32: total with the running total's lowest and highest prefix\n"
+       "    long long t = 0, lo = 1LL << 61, hi = -(1LL << 61);\n"
+       "    for (int i = lb; i < le; ++i) {\n"
+       "      const long long l2 = t + A.pint[3 * i + 1], h2 = t + A.pint[3 * i + 2];\n"
+       "      lo = l2 < lo ? l2 : lo; hi = h2 > hi ? h2 : hi; t += A.pint[3 * i];\n    }\n"
+       "    for (int o = 1; o < 64; o <<= 1) {\n"
+       "      const long long t2 = __shfl_down(t, o, 64), l2 = __shfl_down(lo, o, 64), h2 = __shfl_down(hi, o, 64);\n"
+       "      if ((lane & (2 * o - 1)) == 0 && lane + o < 64) {\n"
+       "        lo = t + l2 < lo ? t + l2 : lo; hi = t + h2 > hi ? t + h2 : hi; t += t2;\n      }\n    }\n"
+       "    if (lane == 0) {\n      A.sums[k] = (double)t;\n"
+       "      if (lo < -2147483648LL || hi > 2147483647LL) atomicOr(A.counter + 1, 1);\n    }\n"
+       "  } else if ((ops & 1) && A.sums) {\n    double t = 0.0;\n    for (int i = lb; i < le; ++i) t += A.psum[i];\n"
+       "    for (int o = 1; o < 64; o <<= 1) {\n      const double t2 = __shfl_down(t, o, 64);\n"
+       "      if ((lane & (2 * o - 1)) == 0 && lane + o < 64) t += t2;\n    }\n"
+       "    if (lane == 0) A.sums[k] = t;\n  }\n";
+  s += "  if ((ops & 6) == 6 && A.maxes && e > b) {\n    double v = -__builtin_inf();\n"
+       "    for (int i = lb; i < le; ++i) v = A.pmv[i] > v ? A.pmv[i] : v;\n"
+       "    for (int o = 32; o > 0; o >>= 1) { const double v2 = __shfl_xor(v, o, 64); v = v2 > v ? v2 : v; }\n"
+       "    if (lane == 0) A.maxes[k] = v;\n"
+       "  } else if ((ops & 2) && A.maxes && e > b) {\n    double v = -__builtin_inf(); unsigned ix = 0xffffffffu;\n"
+       "    for (int i = lb; i < le; ++i) g_max_comb(v, ix, A.pmv[i], A.pmi[i]);\n"
+       "    for (int o = 32; o > 0; o >>= 1) {\n      const double v2 = __shfl_xor(v, o, 64);\n"
+       "      const unsigned i2 = __shfl_xor(ix, o, 64);\n      g_max_comb(v, ix, v2, i2);\n    }\n"
+       "    if (lane == 0) A.maxes[k] = v;\n  }\n}\n";
   s += "extern \"C\" __global__ void ph_gbatch_reset(GArgs A) {\n"
        "  if (threadIdx.x == 0 && blockIdx.x == 0) { A.counter[0] = 0; A.counter[1] = 0; }\n}\n";
   return s;
@@ -515,7 +532,7 @@ int launch_batch(GBatch *B) {
     if (rc) return rc;
   }
   if (B->any_red) {
-    rc = launch_module(B->k_final, (unsigned)((B->n + 255) / 256), &B->args, "ph_gbatch_final");
+    rc = launch_module(B->k_final, (unsigned)((B->n + 3) / 4), &B->args, "ph_gbatch_final");
     if (rc) return rc;
   }
   return PH_OK;
@@ -608,7 +625,7 @@ int run_per_array(GBatch *B) {
     rc = launch_module(B->k_main, (unsigned)((e - b + 3) / 4), &a, "ph_gbatch (one array)");
     if (rc) return rc;
   }
-  if (B->any_red) return launch_module(B->k_final, (unsigned)((B->n + 255) / 256), &B->args, "ph_gbatch_final");
+  if (B->any_red) return launch_module(B->k_final, (unsigned)((B->n + 3) / 4), &B->args, "ph_gbatch_final");
   return PH_OK;
 }
 
