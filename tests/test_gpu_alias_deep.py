@@ -55,9 +55,11 @@ def _deep(view, shape, chain, rng=None, steps=5):
     return view, list(shape)
 
 
-def test_deep_source_copied_onto_itself(ph):
+@pytest.mark.parametrize("dtype", [np.int32, np.float64, np.bool_])
+def test_deep_source_copied_onto_itself(ph, dtype):
+    """A copy moves the elements' bytes as they are, whatever their size."""
     rng = np.random.default_rng(1)
-    h = _i32(rng, (24, 16))
+    h = _i32(rng, (24, 16)).astype(dtype) if dtype != np.bool_ else rng.random((24, 16)) < 0.5
     z = ph.NArray.from_numpy(h)
     chain = []
     v, vs = _deep(z.view(), [24, 16], chain)
