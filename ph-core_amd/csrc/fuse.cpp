@@ -1851,13 +1851,20 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 //    most one earlier element, and when none has two dependents the
 //    dependencies are chains; a thread per chain root then runs its chain
 //    in order, carrying each result to the next element (a recurrence down
-//    the rows of a matrix runs a thread per column).  Otherwise one device
-//    thread runs the loop in order, kSeqChunk elements per launch.
+//    the rows of a matrix runs a thread per column).  When some element has
+//    several dependents the dependencies form a tree: pointer jumping finds
+//    each element's depth, the elements are ordered by depth, and a pass
+//    per level evaluates them, each reading its parent's result from T (a
+//    broadcast of column 0 down the rows runs a pass per row).  Otherwise
+//    (several inputs on the buffer, or a tree too deep for passes per level)
+//    one device thread runs the loop in order, kSeqChunk elements per launch.
 // A plain copy takes pointer jumping instead of step 3: element L's value is
 // the old value at the read address of its chain's root, and
 // ceil(log2(depth)) passes over the chain links find every root.
 constexpr int kOrdMaxIter = 12;
 constexpr int64_t kSeqChunk = 1 << 16;
+constexpr int64_t kOrdMaxLevels = 1 << 16;
+constexpr int64_t kOrdLevelWidth = 16;  // elements per level, on average, below which the loop runs in order
 constexpr int kCtlConflict = 1, kCtlRaw = 2, kCtlChanged = 4;
 
 // W^-1 for an affine output whose axes nest: sorted by decreasing |stride|,
@@ -2006,7 +2013,27 @@ const char *kOrdPrelude = R"PH(
 struct OrdArgs {
   const void *in[8]; void *out; const void *tp; void *tn; const i64 *q0; i64 *q1; int *flags; i64 iter; i64 l0; i64 l1;
   const i64 *ix[8]; const i64 *wx; i64 *inv; i64 *mm; i64 ibase; i64 ispan;
+  i64 *dp; const i64 *dq; i64 *cnt; i64 *lv;
 };
+// Adds 1 to c[key] for every active lane, one atomic per distinct key in
+// the wavefront; returns the lane's slot: the counter's old value plus the
+// lane's rank among the lanes with its key.
+__device__ __forceinline__ i64 ph_agg_add(i64 *c, i64 key) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pend = __ballot(1);
+  i64 pos = 0;
+  while (pend) {
+    const int lead = __builtin_ctzll(pend);
+    const i64 k = __shfl(key, lead);
+    const unsigned long long grp = __ballot(key == k) & pend;
+    i64 base = 0;
+    if (lane == lead) base = (i64)atomicAdd((unsigned long long *)(c + k), (unsigned long long)__popcll(grp));
+    base = __shfl(base, lead);
+    if (key == k) pos = base + __popcll(grp & ((1ull << lane) - 1));
+    pend &= ~grp;
+  }
+  return pos;
+}
 __device__ __forceinline__ bool ph_same(bool a, bool b) { return a == b; }
 __device__ __forceinline__ bool ph_same(int a, int b) { return a == b; }
 __device__ __forceinline__ bool ph_same(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
@@ -2199,30 +2226,61 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
     // chain of exactly one root, and next pointers only increase, so each
     // walk ends
     s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_walk(OrdArgs A) {\n  int fl = 0;\n" + loop;
+    // element E's result into T, the input on the buffer reading `from`
+    // where E depends on an earlier element (the program reads the element
+    // index as L: give it E)
+    auto elem = [&](const std::string &dep, const std::string &from) {
+      std::string e = emit_coords(S.out->view, "E", "");
+      for (int j = 0; j < S.n_in; ++j) {
+        e += in_addr(j, "E");
+        const std::string ld = load(j, fmt("A.in[%d]", j), fmt("a_%d", j));
+        if (j == hz1)
+          e += fmt("      const %s x%d = %s ? %s : %s;\n", vtype(j).c_str(), j, dep.c_str(), from.c_str(), ld.c_str());
+        else
+          e += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, ld.c_str());
+      }
+      std::vector<std::string> inexpr;
+      for (int j = 0; j < S.n_in; ++j) inexpr.push_back(fmt("x%d", j));
+      std::string b;
+      const std::string r = emit_prog(S.prog, inexpr, &b, "");
+      e += "      const i64 ph_lidx = E; (void)ph_lidx;\n" + b + fmt("      const auto res = %s;\n", r.c_str());
+      return e + fmt("      ph_stp((%s *)A.tn, E, res);\n", OST.c_str());
+    };
     s += "      if (A.q0[L] >= 0) continue;\n";
     s += fmt("      %s carry = %s(0);\n      bool from_chain = false;\n", vtype(hz1).c_str(), vtype(hz1).c_str());
     s += "      for (i64 E = L; E >= 0; E = A.q1[E]) {\n";
-    std::string body_e = emit_coords(S.out->view, "E", "");
-    for (int j = 0; j < S.n_in; ++j) {
-      body_e += in_addr(j, "E");
-      if (j == hz1)
-        body_e += fmt("      const %s x%d = from_chain ? carry : %s;\n", vtype(j).c_str(), j,
-                      load(j, fmt("A.in[%d]", j), fmt("a_%d", j)).c_str());
-      else
-        body_e += fmt("      const %s x%d = %s;\n", vtype(j).c_str(), j, load(j, fmt("A.in[%d]", j), fmt("a_%d", j)).c_str());
-    }
-    // the program reads the element index as L: give it E
-    std::string b;
-    {
-      std::vector<std::string> inexpr;
-      for (int j = 0; j < S.n_in; ++j) inexpr.push_back(fmt("x%d", j));
-      const std::string r = emit_prog(S.prog, inexpr, &b, "");
-      body_e += "      const i64 ph_lidx = E; (void)ph_lidx;\n" + b + fmt("      const auto res = %s;\n", r.c_str());
-    }
-    s += body_e;
-    s += fmt("      ph_stp((%s *)A.tn, E, res);\n", OST.c_str());
+    s += elem("from_chain", "carry");
     s += fmt("      carry = (%s)res;\n      from_chain = true;\n      }\n    }\n", vtype(hz1).c_str());
     s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+
+    // Trees (an element read by several later ones).  Each element's depth
+    // by pointer jumping over {ancestor, distance} pairs (dq -> dp); the
+    // elements counted and placed by depth (cnt: counts, then cursors); then
+    // a pass per level, each element reading its parent's result from T.
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dinit(OrdArgs A) {\n" + loop;
+    s += "      const i64 p = A.q0[L]; A.dp[2 * L] = p; A.dp[2 * L + 1] = p >= 0 ? 1 : 0;\n    }\n}\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_djump(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    s += "      i64 a = A.dq[2 * L], d = A.dq[2 * L + 1];\n";
+    s += "      if (a >= 0) { d += A.dq[2 * a + 1]; a = A.dq[2 * a]; if (a >= 0) ctl |= 4; }\n";
+    s += "      A.dp[2 * L] = a; A.dp[2 * L + 1] = d;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dmax(OrdArgs A) {\n  i64 m = 0;\n" + loop;
+    s += "      const i64 d = A.dq[2 * L + 1]; m = d > m ? d : m;\n    }\n";
+    s += "  for (int o = 32; o > 0; o >>= 1) { const i64 m2 = __shfl_xor(m, o); m = m2 > m ? m2 : m; }\n";
+    s += "  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax((unsigned long long *)(A.mm + 2), (unsigned long long)m);\n}\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lcount(OrdArgs A) {\n" + loop;
+    s += "      ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n    }\n}\n";
+    // (lv preset to -1: a slot left unplaced, or placed outside lv, raises
+    // the conflict bit instead of addressing out of bounds)
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lplace(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    s += "      const i64 pos = ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n";
+    s += "      if (pos >= 0 && pos < " + N + ") A.lv[pos] = L; else ctl |= 1;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    // one level: the elements lv[l0 .. l1)
+    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_level(OrdArgs A) {\n  int fl = 0, ctl = 0;\n";
+    s += "    for (i64 ph_li = A.l0 + (i64)blockIdx.x * 256 + threadIdx.x; ph_li < A.l1; ph_li += (i64)gridDim.x * 256) {\n";
+    s += "      const i64 E = A.lv[ph_li];\n      if (E < 0 || E >= " + N + ") { ctl |= 1; continue; }\n";
+    s += "      const i64 pE = A.q0[E];\n";
+    s += elem("pE >= 0", load(hz1, "A.tn", "pE"));
+    s += "    }\n  if (fl) atomicOr(A.flags, fl);\n  ph_ctl(A.flags + 1, ctl);\n}\n";
   }
   if (S.any_index()) {
     // check: every index inside its buffer, and the span the writes cover
@@ -2280,6 +2338,7 @@ struct OrdKernels {
   hipFunction_t check = nullptr, direct = nullptr, mm_init = nullptr, inv_clear = nullptr, inv_fill = nullptr,
                 inv_check = nullptr;
   hipFunction_t mlink = nullptr, unset = nullptr, next = nullptr, walk = nullptr;
+  hipFunction_t dinit = nullptr, djump = nullptr, dmax = nullptr, lcount = nullptr, lplace = nullptr, level = nullptr;
 };
 
 std::map<std::string, OrdKernels> &ord_cache() {
@@ -2319,6 +2378,12 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_unset", &K->unset);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_next", &K->next);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_walk", &K->walk);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_dinit", &K->dinit);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_djump", &K->djump);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_dmax", &K->dmax);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_lcount", &K->lcount);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_lplace", &K->lplace);
+  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_level", &K->level);
   if (rc) return rc;
   ord_cache()[key] = *K;
   return PH_OK;
@@ -2338,6 +2403,10 @@ struct OrdArgsHost {
   int64_t *inv;
   int64_t *mm;  // the write index's least and greatest address
   int64_t ibase, ispan;  // the addresses inv covers: [ibase, ibase + ispan)
+  int64_t *dp;           // {ancestor, distance} pairs written by a depth pass
+  const int64_t *dq;     // the pairs it reads
+  int64_t *cnt;          // elements per depth, then the placement cursors
+  int64_t *lv;           // the elements ordered by depth
 };
 
 int ord_launch(hipFunction_t fn, OrdArgsHost &A, int64_t n, bool one_thread, const char *what) {
@@ -2385,6 +2454,82 @@ int ord_direct(const OrdKernels &K, OrdArgsHost &A, int64_t n, int32_t *flags_ou
   int rc = ord_pass(K.direct, A, n, words, "indexed map");
   if (rc) return rc;
   if (flags_out) *flags_out = words[0];
+  return PH_OK;
+}
+
+// A tree of dependents (A.q0: each element's parent, or -1) evaluated level
+// by level into T, the elements ordered by depth in `order` (n slots).
+// *exact false, and nothing evaluated, when the tree is deeper than
+// kOrdMaxLevels or than n / kOrdLevelWidth (a pass per level would then
+// cost more than the loop in order).
+int ord_levels(const OrdKernels &K, OrdArgsHost &A, int64_t n, void *T, void *order, int *flags, bool *exact) {
+  *exact = false;
+  void *p[2];
+  int rc = ph::scratch((size_t)n * 2 * sizeof(int64_t), &p[0], 9);
+  if (!rc) rc = ph::scratch((size_t)n * 2 * sizeof(int64_t), &p[1], 10);
+  if (rc) return rc;
+  A.dp = (int64_t *)p[0];
+  rc = ord_launch(K.dinit, A, n, false, "ordered map (depths)");
+  if (rc) return rc;
+  int words[2] = {0, 0};
+  int pc = 0;
+  bool settled = false;
+  for (int round = 0; round < 64 && !settled; ++round) {
+    A.dq = (const int64_t *)p[pc];
+    A.dp = (int64_t *)p[pc ^ 1];
+    rc = ord_pass(K.djump, A, n, words, "ordered map (depths)");
+    if (rc) return rc;
+    pc ^= 1;
+    settled = !(words[1] & kCtlChanged);
+  }
+  if (!settled) return PH_OK;
+  A.dq = (const int64_t *)p[pc];
+  rc = ph::clear_i32((int *)(A.mm + 2), 2);
+  if (!rc) rc = ord_launch(K.dmax, A, n, false, "ordered map (depths)");
+  if (rc) return rc;
+  int64_t depth = 0;
+  PH_HIP(hipMemcpyAsync(&depth, A.mm + 2, sizeof(depth), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  if (depth > kOrdMaxLevels || depth > n / kOrdLevelWidth) return PH_OK;
+  const int64_t nl = depth + 1;
+  void *c;
+  rc = ph::scratch((size_t)nl * 2 * sizeof(int64_t), &c, 11);
+  if (rc) return rc;
+  A.cnt = (int64_t *)c;
+  rc = ph::clear_i32((int *)c, (int)(nl * 4));
+  if (!rc) rc = ord_launch(K.lcount, A, n, false, "ordered map (levels)");
+  if (rc) return rc;
+  std::vector<int64_t> cnt((size_t)nl), off((size_t)nl);
+  PH_HIP(hipMemcpyAsync(cnt.data(), c, (size_t)nl * sizeof(int64_t), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  int64_t total = 0;
+  for (int64_t k = 0; k < nl; ++k) {
+    off[(size_t)k] = total;
+    total += cnt[(size_t)k];
+  }
+  if (total != n) return ph::set_error(PH_E_HIP, "ordered map: %lld elements placed in levels of %lld",
+                                       (long long)total, (long long)n);
+  A.cnt = (int64_t *)c + nl;  // cursors, from the offsets
+  A.lv = (int64_t *)order;
+  PH_HIP(hipMemcpyAsync(A.cnt, off.data(), (size_t)nl * sizeof(int64_t), hipMemcpyHostToDevice, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  A.q1 = (int64_t *)order;
+  rc = ord_launch(K.unset, A, n, false, "ordered map (levels)");
+  if (!rc) rc = ph::clear_i32(A.flags, 2);
+  if (!rc) rc = ord_launch(K.lplace, A, n, false, "ordered map (levels)");
+  if (rc) return rc;
+  A.tn = T;
+  for (int64_t k = 0; k < nl; ++k) {
+    A.l0 = off[(size_t)k];
+    A.l1 = off[(size_t)k] + cnt[(size_t)k];
+    rc = ord_launch(K.level, A, cnt[(size_t)k], false, "ordered map (level)");
+    if (rc) return rc;
+  }
+  PH_HIP(hipMemcpyAsync(words, A.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, ph::stream()));
+  PH_HIP(hipStreamSynchronize(ph::stream()));
+  if (words[1] & kCtlConflict) return ph::set_error(PH_E_HIP, "ordered map: an element left out of its level");
+  *flags = words[0];
+  *exact = true;
   return PH_OK;
 }
 
@@ -2515,6 +2660,9 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
         if (rc) return rc;
         flags = words[0];
         exact = true;
+      } else {
+        rc = ord_levels(K, A, n, t[cur], q[1], &flags, &exact);
+        if (rc) return rc;
       }
     }
     if (!exact) return ord_sequential(K, A, n, flags_out);

@@ -19,7 +19,7 @@ struct Scratch {
   void *ptr = nullptr;
   size_t bytes = 0;
 };
-Scratch g_scratch[9];
+Scratch g_scratch[12];
 long g_launches = 0;
 long g_fail_after = -2;  // -2: not read yet, -1: disabled
 }  // namespace

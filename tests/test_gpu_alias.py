@@ -262,16 +262,43 @@ def test_recurrence_down_the_rows_runs_per_column(ph):
 def test_broadcast_recurrence_forms_a_tree(ph):
     """z[1.., ..] = z[..-2, 0..0].tile(1, m).process { ... }: every element
     of row i reads z[i - 1, 0], so that element has m dependents (a tree,
-    not chains) and the loop runs in order; row i ends up f^i(z0[0, 0])."""
+    not chains), evaluated a pass per row; row i ends up f^i(z0[0, 0]).
+    A block of 3 rows (no more than 16 elements per level: the loop in
+    order) and 3000 x 1024 (3000 passes: the loop in order took about a
+    second)."""
     rng = np.random.default_rng(43)
-    n, m = 40, 33
-    h = _i32(rng, (n, m))
     fn = lambda v: (v ^ 21) - 3  # noqa: E731
+    for n, m in [(40, 33), (3, 5), (3000, 1024)]:
+        h = _i32(rng, (n, m))
+        z = ph.NArray.from_numpy(h)
+        ph.sync()
+        t0 = time.perf_counter()
+        z[["1..", ".."]] = z.view("..-2", "0..0").tile(1, m).process(fn)
+        ph.sync()
+        dt = time.perf_counter() - t0
+        exp = h.copy()
+        v = int(h[0, 0])
+        for i in range(1, n):
+            v = (v ^ 21) - 3
+            exp[i, :] = v
+        assert np.array_equal(z.to_numpy(), exp), (n, m)
+        print(f"broadcast recurrence {n}x{m}: {dt * 1e3:.1f} ms")
+    assert dt < 0.5
+
+
+def test_tree_along_the_first_row(ph):
+    """z[.., 1..] = z[0..0, ..-2].tile(n, 1).process { ... }: element
+    (i, j) reads z[0, j - 1], which element (0, j - 1) wrote, so the
+    elements of row 0 form a chain and every column hangs off it (a tree
+    with a level per column), against the loop itself."""
+    rng = np.random.default_rng(44)
+    n, m = 100, 300
+    h = _i32(rng, (n, m))
+    fn = lambda v: (v * 3 + 7) % 1000  # noqa: E731
     z = ph.NArray.from_numpy(h)
-    z[["1..", ".."]] = z.view("..-2", "0..0").tile(1, m).process(fn)
-    exp = h.copy()
-    v = int(h[0, 0])
-    for i in range(1, n):
-        v = (v ^ 21) - 3
-        exp[i, :] = v
-    assert np.array_equal(z.to_numpy(), exp)
+    z[["..", "1.."]] = z.view("0..0", "..-2").tile(n, 1).process(fn)
+    ref = [[int(v) for v in row] for row in h]
+    for i in range(n):
+        for j in range(m - 1):
+            ref[i][j + 1] = (ref[0][j] * 3 + 7) % 1000
+    assert np.array_equal(z.to_numpy(), np.array(ref, np.int32))

@@ -347,8 +347,10 @@ def test_map_indexed_programs_follow_the_loop(ph):
     through an index array: buf[w[k]] = f(buf[r[k]], y[k]) for k in order.
     One chain of 6000 dependent elements (a shift by one), 64 interleaved
     chains (a shift by 64: a thread per chain), the same with one element
-    read by two later ones (a tree: the in-order loop), and a second input
-    y read through a plain descriptor from another buffer."""
+    read by two later ones (a tree: a pass per depth level), a random tree
+    (each element reads one of the 63 written before it), a tree as deep as
+    the chain (the in-order loop), and a second input y read through a
+    plain descriptor from another buffer."""
     lib = _lib(ph)
     lib.ph_map_indexed.argtypes = [ctypes.POINTER(_Prog), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
                                    ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
@@ -362,6 +364,11 @@ def test_map_indexed_programs_follow_the_loop(ph):
     tree = cases["64 chains"].copy()
     tree[200] = 72  # element 72 is read by 136 and by 200
     cases["tree"] = tree
+    rt = np.arange(n) - rng.integers(1, 64, n)
+    cases["random tree"] = np.where(rt < 0, np.arange(n), rt)
+    deep = cases["one chain"].copy()
+    deep[3000] = 17  # element 17 is read by 18 and by 3000
+    cases["deep tree"] = deep
     for name, r in cases.items():
         print(f"indexed program: {name}", flush=True)
         h = rng.integers(-1000, 1000, n).astype(np.int32)
