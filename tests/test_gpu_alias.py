@@ -286,6 +286,35 @@ def test_broadcast_recurrence_forms_a_tree(ph):
     assert dt < 0.5
 
 
+def test_tree_overflow_and_float_values(ph):
+    """Trees run a pass per level: an Int32 `* 2` down the rows raises
+    OverflowError once row 31 leaves Int32 (the reference's checked `*`),
+    and stays exact below that; a Float64 tree is bit for bit the loop's."""
+    n, m = 40, 33
+    h = np.zeros((n, m), np.int32)
+    h[0, 0] = 1
+    z = ph.NArray.from_numpy(h)
+    with pytest.raises(OverflowError):
+        z[["1..", ".."]] = z.view("..-2", "0..0").tile(1, m).process(lambda v: v * 2)
+    z = ph.NArray.from_numpy(h[:31].copy())
+    z[["1..", ".."]] = z.view("..-2", "0..0").tile(1, m).process(lambda v: v * 2)
+    exp = h[:31].copy()
+    exp[1:, :] = (2 ** np.arange(1, 31))[:, None]
+    assert np.array_equal(z.to_numpy(), exp)
+    rng = np.random.default_rng(45)
+    n, m = 200, 64
+    hf = rng.standard_normal((n, m))
+    zf = ph.NArray.from_numpy(hf)
+    zf[["1..", ".."]] = zf.view("..-2", "0..0").tile(1, m).process(lambda v: v * 1.5 - 0.25)
+    ref = hf.copy()
+    v = float(hf[0, 0])
+    for i in range(1, n):
+        v = v * 1.5 - 0.25
+        ref[i, :] = v
+    got = zf.to_numpy()
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
 def test_tree_along_the_first_row(ph):
     """z[.., 1..] = z[0..0, ..-2].tile(n, 1).process { ... }: element
     (i, j) reads z[0, j - 1], which element (0, j - 1) wrote, so the
