@@ -618,7 +618,7 @@ def other_workloads(phase, ev, args):
         phase.NArray.from_json(j, D.F64)
         t2 = time.perf_counter()
         out["json_f64_1024x1024"] = {"to_json_ms": round((t1 - t0) * 1e3, 1), "from_json_ms": round((t2 - t1) * 1e3, 1),
-                                     "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing, 1 thread"}
+                                     "text_mb": round(len(j) / 1e6, 2), "note": "host formatting / parsing in consecutive chunks, up to 16 threads"}
     if want("path"):
         out["path_ops_i32_8192"] = path_ops(phase, ev, args)
     return out
