@@ -126,3 +126,30 @@ def test_json_yaml_round_trip(ph, dt):
         assert b.dtype == a.dtype and np.array_equal(b, a), text[:200]
     y = d.to_yaml()
     assert all(len(line) <= 80 + 40 for line in y.splitlines())  # flow sequences wrap near column 80
+
+
+def test_json_of_many_elements_in_chunks(ph):
+    """Documents long enough to be written and read in chunks on several
+    threads: the text is Python's own JSON of the same values, in order, and
+    reads back bit for bit; the first non-finite or invalid element in
+    order is the one reported, though later chunks hold others."""
+    rng = np.random.default_rng(10)
+    n = 300007
+    a = rng.standard_normal(n) * 10.0 ** rng.integers(-30, 30, n)
+    d = ph.NArray.from_numpy(a)
+    j = d.to_json()
+    doc = json.loads(j)
+    assert doc["shape"] == [n] and np.array_equal(np.array(doc["elements"]).view(np.uint64), a.view(np.uint64))
+    assert np.array_equal(ph.NArray.from_json(j, ph.DType.F64).to_numpy().view(np.uint64), a.view(np.uint64))
+    ai = rng.integers(-2 ** 31, 2 ** 31, n, dtype=np.int64).astype(np.int32)
+    ji = ph.NArray.from_numpy(ai).to_json()
+    assert json.loads(ji)["elements"] == ai.tolist()
+    b = a.copy()
+    b[n - 5], b[200003], b[150001] = np.nan, np.inf, -np.inf
+    with pytest.raises(ph.ParseError, match="^-Infinity not allowed in JSON"):
+        ph.NArray.from_numpy(b).to_json()
+    parts = j.split(",")
+    parts[250000] = "1.5x"
+    parts[120000] = "2.5y"
+    with pytest.raises(ph.ParseError, match="invalid Float64 element '2.5y'"):
+        ph.NArray.from_json(",".join(parts), ph.DType.F64)
