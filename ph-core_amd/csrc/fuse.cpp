@@ -485,6 +485,20 @@ struct MapArgs {
 
 constexpr int kBlock = 256;
 
+// Elements per lane of the map kernel: the most (up to 8) that keep a
+// lane's loads within 48 bytes.  Measured on 16384^2 Float64
+// (a + b.tile) * c: 2 per lane (48 bytes) 1.02 ms, 4 1.10 ms, 8 1.17 ms; on
+// 8192^2 Int32 maps: x.eq(y) 0.100 ms at 4 against 0.106 at 8, z[mask] = y
+// 0.143 against 0.155, and z[mask] = 7 (5 bytes) 0.098 ms at 8 against 0.124
+// at 4.
+int map_ept(int n_in, const int *in_dt) {
+  int in_bytes = 0;
+  for (int j = 0; j < n_in; ++j) in_bytes += (int)ph::dtype_size(in_dt[j]);
+  int ept = 8;
+  while (ept > 1 && ept * in_bytes > 48) ept /= 2;
+  return ept;
+}
+
 std::string map_source(const ph_prog *p, int n_in, const int *in_dt, const ph::Layout &L,
                        int out_dt, int ept, bool flat, int64_t *nblocks) {
   // operands 0..n_in-1 inputs, n_in output
@@ -2846,7 +2860,7 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
     }
     ph::Layout L;
     ph::collapse(dp, n_in + 1, &L);
-    const int ept = ph::dtype_size(prog->out_dtype) >= 8 ? 4 : 8;
+    const int ept = map_ept(n_in, in_dt);
     const int64_t c = L.rank > 0 ? L.shape[L.rank - 1] : 1;
     int64_t nb;
     std::string src = map_source(prog, n_in, in_dt, L, prog->out_dtype, ept, L.rank > 1 && c < 256, &nb);
@@ -3067,15 +3081,7 @@ int ph_map(const ph_prog *prog, int32_t n_in, const void *const *ins, const ph_v
   descs[n_in] = out_desc;
   ph::Layout L;
   ph::collapse(descs, n_in + 1, &L);
-  // elements per lane: the most (up to 8) that keep a lane's loads within
-  // 48 bytes.  Measured on 16384^2 Float64 (a + b.tile) * c: 2 per lane
-  // (48 bytes) 1.02 ms, 4 1.10 ms, 8 1.17 ms; on 8192^2 Int32 maps: x.eq(y)
-  // 0.100 ms at 4 against 0.106 at 8, z[mask] = y 0.143 against 0.155,
-  // and z[mask] = 7 (5 bytes) 0.098 ms at 8 against 0.124 at 4.
-  int in_bytes = 0;
-  for (int j = 0; j < n_in; ++j) in_bytes += (int)ph::dtype_size(in_dt[j]);
-  int ept = 8;
-  while (ept > 1 && ept * in_bytes > 48) ept /= 2;
+  int ept = map_ept(n_in, in_dt);
   if (const char *e = std::getenv("PH_MAP_EPT")) ept = std::min(16, std::max(1, std::atoi(e)));  // tuning override
   const int64_t cols = L.rank > 0 ? L.shape[L.rank - 1] : 1;
   const bool flat = L.rank > 1 && cols < 256;

@@ -242,3 +242,55 @@ def test_middle_axis_reduction_of_reshaped_view(ph):
         others = tuple(i for i in range(3) if i != ax)
         assert np.array_equal(np.array(v.slices_sum(ax)), hv.astype(np.int64).sum(axis=others)), ax
         assert np.array_equal(np.array(v.slices_max(ax)), hv.max(axis=others)), ax
+
+
+def test_tiled_map_more_layouts(ph):
+    """The LDS-tiled map over extents that are multiples of 4 but not of 64
+    (partial tiles at the edges), three inputs of mixed element types with
+    Float32 specials, a rank-3 view, regions with aligned and unaligned
+    offsets, and an Int32 division by a zero that raises.  One launch each,
+    bit-exact against the oracle."""
+    from crystal import trace
+    rng = np.random.default_rng(62)
+    i32 = lambda s: rng.integers(-2 ** 31, 2 ** 31, s, dtype=np.int64).astype(np.int32)  # noqa: E731
+    h, c = i32((300, 256)), i32((256, 300))
+    fn2 = lambda a, b: a.wadd(b) ^ (a & 4095)  # noqa: E731
+    exp, _ = O.eval_expr(trace(fn2, 2), [np.ascontiguousarray(h.T), c])
+    got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(h).view().permute(1, 0).map_with(
+        ph.NArray.from_numpy(c), block=fn2))
+    assert n == 1 and np.array_equal(got.to_numpy(), exp)
+    # Float32 transposed, Int32 and Float32 contiguous: 200 x 132
+    f = rng.standard_normal((132, 200)).astype(np.float32)
+    f[3, 5], f[7, 9], f[100, 0] = np.inf, -0.0, np.nan
+    gi = rng.integers(-50, 50, (200, 132)).astype(np.int32)
+    gf = rng.standard_normal((200, 132)).astype(np.float32)
+    fn3 = lambda a, b, d: (a * b - d) / 2  # noqa: E731 (Float32 throughout)
+    exp, _ = O.eval_expr(trace(fn3, 3), [np.ascontiguousarray(f.T), gi, gf])
+    got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(f).view().permute(1, 0).map_with(
+        ph.NArray.from_numpy(gi), ph.NArray.from_numpy(gf), block=fn3))
+    assert n == 1 and exp.dtype == np.float32 and got.to_numpy().dtype == exp.dtype
+    assert np.array_equal(got.to_numpy().view(np.uint32), exp.view(np.uint32))
+    # rank 3: (3, 64, 72) viewed as (3, 72, 64)
+    h3, c3 = i32((3, 64, 72)), i32((3, 72, 64))
+    exp, _ = O.eval_expr(trace(fn2, 2), [np.ascontiguousarray(h3.transpose(0, 2, 1)), c3])
+    got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(h3).view().permute(0, 2, 1).map_with(
+        ph.NArray.from_numpy(c3), block=fn2))
+    assert n == 1 and np.array_equal(got.to_numpy(), exp)
+    # regions: rows 4.. (offset 4 x 256, aligned) and columns 1.. (offset 1)
+    for lits, sub in [(("4..", ".."), h[4:, :]), (("..", "1..-4"), h[:, 1:-3])]:
+        cs = i32((sub.shape[1], sub.shape[0]))
+        exp, _ = O.eval_expr(trace(fn2, 2), [np.ascontiguousarray(sub.T), cs])
+        got, n = _one_launch(ph, lambda: ph.NArray.from_numpy(h).view(*lits).permute(1, 0).map_with(
+            ph.NArray.from_numpy(cs), block=fn2))
+        assert n == 1 and np.array_equal(got.to_numpy(), exp), lits
+    # Int32 // with a zero read through the transposed operand
+    num = rng.integers(-1000, 1000, (128, 200)).astype(np.int32)
+    den = rng.integers(1, 50, (200, 128)).astype(np.int32)
+    exp, fl = O.eval_expr(trace(lambda a, b: a // b, 2), [num, np.ascontiguousarray(den.T)])
+    assert fl == 0
+    xden = ph.NArray.from_numpy(den).view().permute(1, 0)
+    assert np.array_equal(ph.NArray.from_numpy(num).map_with(xden, block=lambda a, b: a // b).to_numpy(), exp)
+    den[77, 13] = 0
+    xden = ph.NArray.from_numpy(den).view().permute(1, 0)
+    with pytest.raises(ph.DivisionByZeroError):
+        ph.NArray.from_numpy(num).map_with(xden, block=lambda a, b: a // b).to_numpy()

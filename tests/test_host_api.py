@@ -237,13 +237,18 @@ def test_element_programs_compile_for_gfx950(host):
 
 def test_tiled_map_programs_compile_for_gfx950(host):
     """The map that stages a transposed operand through LDS is generated
-    and compiled for a few blocks and element types (no launch, no device);
-    the element index is refused there."""
+    and compiled for a few blocks and element types, mixed Int32 and
+    Float32 operands among them (no launch, no device); the element index is
+    refused there."""
     for dt, fn in BLOCKS[:3] + BLOCKS[8:10]:
         d = getattr(host.DType, DT[dt][0])
         host.compile_check(fn(host.placeholder(d)), [d], 64, 96, 2)
     two = host.input(0, host.DType.BF16) * host.input(1, host.DType.F32)
     host.compile_check(two, [host.DType.BF16, host.DType.F32], 300, 257, 2)
+    I, F = host.DType.I32, host.DType.F32
+    host.compile_check(host.input(0, I).wadd(host.input(1, I)), [I, I], 200, 132, 2)
+    mixed = host.input(0, F) * host.input(1, I) - host.input(2, F)
+    host.compile_check(mixed, [F, I, F], 64, 128, 2)
     with pytest.raises(host.ArgumentError):
         host.compile_check(host.input(0, host.DType.I32) + host.element_index(), [host.DType.I32], 64, 64, 2)
 
