@@ -616,9 +616,10 @@ def test_concatenate_vs_numpy():
 
 def test_max_follows_enumerable_greater_than():
     """Enumerable#max keeps the first element and replaces it only by a
-    greater one (`>`): a leading NaN is kept (nothing compares
-    greater than it), a later NaN is skipped, and of equal zeros the first
-    one's sign survives.  Known answers from that definition."""
+    greater one (`>`, src/multi_indexable.cr:18 includes Enumerable): a
+    leading NaN is kept (nothing compares greater than it), a later NaN is
+    skipped, and of equal zeros the first one's sign survives.  Known
+    answers from that definition."""
     nan = float("nan")
     assert math.isnan(O.max_seq(np.array([nan, 1.0, 5.0])))
     assert O.max_seq(np.array([1.0, nan, 2.0])) == 2.0
