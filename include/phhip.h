@@ -274,9 +274,10 @@ int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
  * output the map runs in one parallel pass; otherwise the order is resolved
  * by repeated passes to a fixed point.  Past 12 dependent elements, when
  * only one input reads the buffer: a device thread per chain when no
- * element has two dependents, else a pass per depth level of the tree of
- * dependents (when the levels average 16 elements or more and number at
- * most 65536).  Other deep maps run a sequential loop on one device
+ * element has two dependents; else, for the tree of dependents, a thread
+ * per chain of the elements that others read (when those form chains) and
+ * one pass for the rest, or a pass per depth level (when the levels
+ * average 16 elements or more and number at most 65536).  Other deep maps run a sequential loop on one device
  * thread.  An input that overlaps the output through another base pointer
  * is PH_E_ARGUMENT.
  * A program with PH_OP_INDEX (NArray#map_with_index / map_with_index! /

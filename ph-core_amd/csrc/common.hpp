@@ -27,8 +27,8 @@ int device();
 // all work on it is ordered by the single library stream.  Slots: 0-1
 // gathered map and reduction inputs, 2 reduction and heat partials, 3 flag
 // words, 4-7 the buffers of maps whose inputs share the output's buffer,
-// 8 the inverse of an indexed map's write index, 9-11 the depths and level
-// counts of an ordered map's tree of dependents.
+// 8 the inverse of an indexed map's write index, 9-11 the marks, depths and
+// level counts of an ordered map's tree of dependents.
 int scratch(size_t bytes, void **out, int slot = 0);
 
 // ph_strided_copy when source and destination share a buffer: the copy in

@@ -1866,10 +1866,13 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 //    dependencies are chains; a thread per chain root then runs its chain
 //    in order, carrying each result to the next element (a recurrence down
 //    the rows of a matrix runs a thread per column).  When some element has
-//    several dependents the dependencies form a tree: pointer jumping finds
-//    each element's depth, the elements are ordered by depth, and a pass
-//    per level evaluates them, each reading its parent's result from T (a
-//    broadcast of column 0 down the rows runs a pass per row).  Otherwise
+//    several dependents the dependencies form a tree.  If the elements that
+//    others read still form chains, those chains are walked the same way
+//    and one pass then computes the rest from their parents' results (a
+//    broadcast of column 0 down the rows walks column 0).  Otherwise
+//    pointer jumping finds each element's depth, the elements are ordered
+//    by depth, and a pass per level evaluates them, each reading its
+//    parent's result from T.  Otherwise
 //    (several inputs on the buffer, or a tree too deep for passes per level)
 //    one device thread runs the loop in order, kSeqChunk elements per launch.
 // A plain copy takes pointer jumping instead of step 3: element L's value is
@@ -2027,7 +2030,7 @@ const char *kOrdPrelude = R"PH(
 struct OrdArgs {
   const void *in[8]; void *out; const void *tp; void *tn; const i64 *q0; i64 *q1; int *flags; i64 iter; i64 l0; i64 l1;
   const i64 *ix[8]; const i64 *wx; i64 *inv; i64 *mm; i64 ibase; i64 ispan;
-  i64 *dp; const i64 *dq; i64 *cnt; i64 *lv;
+  i64 *dp; const i64 *dq; i64 *cnt; i64 *lv; int *mk;
 };
 // Adds 1 to c[key] for every active lane, one atomic per distinct key in
 // the wavefront; returns the lane's slot: the counter's old value plus the
@@ -2094,7 +2097,8 @@ int single_hazard(const OrdSpec &S) {
   return h;
 }
 
-std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
+// part 0: every kernel but those of trees of dependents; part 1: those.
+std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
   const int64_t n = ph::desc_count(S.out);
   const std::string N = lit_i64(n);
   const bool copy = S.prog == nullptr;
@@ -2217,6 +2221,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
     s += fmt("      ((%s *)A.tn)[L] = ((const %s *)A.in[0])[a_0];\n    }\n}\n", OST.c_str(), OST.c_str());
   }
   const int hz1 = single_hazard(S);
+  std::string t;  // the kernels of trees of dependents (part 1)
   if (S.prog && hz1 >= 0) {
     // Chains of a map with one input on the output's buffer: element L
     // depends on at most the one earlier element that wrote the address
@@ -2267,35 +2272,53 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w) {
     s += fmt("      carry = (%s)res;\n      from_chain = true;\n      }\n    }\n", vtype(hz1).c_str());
     s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
 
+    // Trees whose inner elements (those another element reads) form
+    // chains: the walk above runs the chains (q1 then links each inner
+    // element to its one inner dependent), then one pass computes every
+    // other element from its parent's result in T.
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_mclear(OrdArgs A) {\n" + loop;
+    t += "      A.mk[L] = 0;\n    }\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_mark(OrdArgs A) {\n" + loop;
+    t += "      const i64 p = A.q0[L]; if (p >= 0) A.mk[p] = 1;\n    }\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_inext(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    t += "      const i64 p = A.q0[L];\n      if (p >= 0 && A.mk[L]) {\n";
+    t += "        const unsigned long long old = atomicCAS((unsigned long long *)(A.q1 + p), ~0ull, (unsigned long long)L);\n";
+    t += "        if (old != ~0ull) ctl |= 1;\n      }\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_leaves(OrdArgs A) {\n  int fl = 0;\n" + loop;
+    t += "      if (A.mk[L]) continue;\n      const i64 E = L, pE = A.q0[E];\n";
+    t += elem("pE >= 0", load(hz1, "A.tn", "pE"));
+    t += "    }\n  if (fl) atomicOr(A.flags, fl);\n}\n";
+
     // Trees (an element read by several later ones).  Each element's depth
     // by pointer jumping over {ancestor, distance} pairs (dq -> dp); the
     // elements counted and placed by depth (cnt: counts, then cursors); then
     // a pass per level, each element reading its parent's result from T.
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dinit(OrdArgs A) {\n" + loop;
-    s += "      const i64 p = A.q0[L]; A.dp[2 * L] = p; A.dp[2 * L + 1] = p >= 0 ? 1 : 0;\n    }\n}\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_djump(OrdArgs A) {\n  int ctl = 0;\n" + loop;
-    s += "      i64 a = A.dq[2 * L], d = A.dq[2 * L + 1];\n";
-    s += "      if (a >= 0) { d += A.dq[2 * a + 1]; a = A.dq[2 * a]; if (a >= 0) ctl |= 4; }\n";
-    s += "      A.dp[2 * L] = a; A.dp[2 * L + 1] = d;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dmax(OrdArgs A) {\n  i64 m = 0;\n" + loop;
-    s += "      const i64 d = A.dq[2 * L + 1]; m = d > m ? d : m;\n    }\n";
-    s += "  for (int o = 32; o > 0; o >>= 1) { const i64 m2 = __shfl_xor(m, o); m = m2 > m ? m2 : m; }\n";
-    s += "  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax((unsigned long long *)(A.mm + 2), (unsigned long long)m);\n}\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lcount(OrdArgs A) {\n" + loop;
-    s += "      ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n    }\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dinit(OrdArgs A) {\n" + loop;
+    t += "      const i64 p = A.q0[L]; A.dp[2 * L] = p; A.dp[2 * L + 1] = p >= 0 ? 1 : 0;\n    }\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_djump(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    t += "      i64 a = A.dq[2 * L], d = A.dq[2 * L + 1];\n";
+    t += "      if (a >= 0) { d += A.dq[2 * a + 1]; a = A.dq[2 * a]; if (a >= 0) ctl |= 4; }\n";
+    t += "      A.dp[2 * L] = a; A.dp[2 * L + 1] = d;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_dmax(OrdArgs A) {\n  i64 m = 0;\n" + loop;
+    t += "      const i64 d = A.dq[2 * L + 1]; m = d > m ? d : m;\n    }\n";
+    t += "  for (int o = 32; o > 0; o >>= 1) { const i64 m2 = __shfl_xor(m, o); m = m2 > m ? m2 : m; }\n";
+    t += "  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax((unsigned long long *)(A.mm + 2), (unsigned long long)m);\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lcount(OrdArgs A) {\n" + loop;
+    t += "      ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n    }\n}\n";
     // (lv preset to -1: a slot left unplaced, or placed outside lv, raises
     // the conflict bit instead of addressing out of bounds)
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lplace(OrdArgs A) {\n  int ctl = 0;\n" + loop;
-    s += "      const i64 pos = ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n";
-    s += "      if (pos >= 0 && pos < " + N + ") A.lv[pos] = L; else ctl |= 1;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_lplace(OrdArgs A) {\n  int ctl = 0;\n" + loop;
+    t += "      const i64 pos = ph_agg_add(A.cnt, A.dq[2 * L + 1]);\n";
+    t += "      if (pos >= 0 && pos < " + N + ") A.lv[pos] = L; else ctl |= 1;\n    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
     // one level: the elements lv[l0 .. l1)
-    s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_level(OrdArgs A) {\n  int fl = 0, ctl = 0;\n";
-    s += "    for (i64 ph_li = A.l0 + (i64)blockIdx.x * 256 + threadIdx.x; ph_li < A.l1; ph_li += (i64)gridDim.x * 256) {\n";
-    s += "      const i64 E = A.lv[ph_li];\n      if (E < 0 || E >= " + N + ") { ctl |= 1; continue; }\n";
-    s += "      const i64 pE = A.q0[E];\n";
-    s += elem("pE >= 0", load(hz1, "A.tn", "pE"));
-    s += "    }\n  if (fl) atomicOr(A.flags, fl);\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+    t += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_level(OrdArgs A) {\n  int fl = 0, ctl = 0;\n";
+    t += "    for (i64 ph_li = A.l0 + (i64)blockIdx.x * 256 + threadIdx.x; ph_li < A.l1; ph_li += (i64)gridDim.x * 256) {\n";
+    t += "      const i64 E = A.lv[ph_li];\n      if (E < 0 || E >= " + N + ") { ctl |= 1; continue; }\n";
+    t += "      const i64 pE = A.q0[E];\n";
+    t += elem("pE >= 0", load(hz1, "A.tn", "pE"));
+    t += "    }\n  if (fl) atomicOr(A.flags, fl);\n  ph_ctl(A.flags + 1, ctl);\n}\n";
   }
+  if (part == 1) return std::string(kPrelude) + kOrdPrelude + t;
   if (S.any_index()) {
     // check: every index inside its buffer, and the span the writes cover
     // (mm preset to {extent, -1} by ph_ord_mm_init)
@@ -2353,6 +2376,7 @@ struct OrdKernels {
                 inv_check = nullptr;
   hipFunction_t mlink = nullptr, unset = nullptr, next = nullptr, walk = nullptr;
   hipFunction_t dinit = nullptr, djump = nullptr, dmax = nullptr, lcount = nullptr, lplace = nullptr, level = nullptr;
+  hipFunction_t mclear = nullptr, mark = nullptr, inext = nullptr, leaves = nullptr;
 };
 
 std::map<std::string, OrdKernels> &ord_cache() {
@@ -2360,7 +2384,7 @@ std::map<std::string, OrdKernels> &ord_cache() {
   return c;
 }
 
-int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
+std::string ord_key(const OrdSpec &S) {
   std::string key = S.prog ? "O" + prog_key(S.prog) : fmt("C%zu", S.esize);
   key_add(&key, S.in_dt, sizeof(int) * (size_t)S.n_in);
   key_add(&key, &S.hazard, sizeof(S.hazard));
@@ -2368,6 +2392,11 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   key_add(&key, &S.out_indexed, sizeof(S.out_indexed));
   key_add(&key, S.out, sizeof(ph_view_desc));
   for (int j = 0; j < S.n_in; ++j) key_add(&key, S.in[j], sizeof(ph_view_desc));
+  return key;
+}
+
+int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
+  const std::string key = ord_key(S);
   auto hit = ord_cache().find(key);
   if (hit != ord_cache().end()) {
     *K = hit->second;
@@ -2392,14 +2421,32 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_unset", &K->unset);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_next", &K->next);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_walk", &K->walk);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_dinit", &K->dinit);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_djump", &K->djump);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_dmax", &K->dmax);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_lcount", &K->lcount);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_lplace", &K->lplace);
-  if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_level", &K->level);
   if (rc) return rc;
   ord_cache()[key] = *K;
+  return PH_OK;
+}
+
+// The kernels of trees of dependents, a module of their own that is
+// compiled the first time S meets a tree (the chains' kernels are already in
+// *K).
+int ord_tree_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
+  if (K->leaves) return PH_OK;
+  const std::string src = ord_source(S, w, 1);
+  int rc = ph::jit_kernel(src, "ph_ord_dinit", &K->dinit);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_djump", &K->djump);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_dmax", &K->dmax);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_lcount", &K->lcount);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_lplace", &K->lplace);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_level", &K->level);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_mclear", &K->mclear);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_mark", &K->mark);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_inext", &K->inext);
+  if (!rc) rc = ph::jit_kernel2(src, "ph_ord_leaves", &K->leaves);
+  if (rc) {
+    K->leaves = nullptr;
+    return rc;
+  }
+  ord_cache()[ord_key(S)] = *K;
   return PH_OK;
 }
 
@@ -2421,6 +2468,7 @@ struct OrdArgsHost {
   const int64_t *dq;     // the pairs it reads
   int64_t *cnt;          // elements per depth, then the placement cursors
   int64_t *lv;           // the elements ordered by depth
+  int *mk;               // 1 for an element that another reads
 };
 
 int ord_launch(hipFunction_t fn, OrdArgsHost &A, int64_t n, bool one_thread, const char *what) {
@@ -2468,6 +2516,35 @@ int ord_direct(const OrdKernels &K, OrdArgsHost &A, int64_t n, int32_t *flags_ou
   int rc = ord_pass(K.direct, A, n, words, "indexed map");
   if (rc) return rc;
   if (flags_out) *flags_out = words[0];
+  return PH_OK;
+}
+
+// A tree of dependents (A.q0: each element's parent, or -1) whose inner
+// elements (those another element reads) form chains: the chains walked into
+// T (next pointers in A.q1, n slots), then the other elements in one pass.
+// *exact false, and nothing evaluated, when an inner element has two inner
+// dependents.
+int ord_inner_chains(const OrdKernels &K, OrdArgsHost &A, int64_t n, void *T, int *flags, bool *exact) {
+  *exact = false;
+  void *m;
+  int rc = ph::scratch((size_t)n * sizeof(int), &m, 9);
+  if (rc) return rc;
+  A.mk = (int *)m;
+  rc = ord_launch(K.mclear, A, n, false, "ordered map (inner elements)");
+  if (!rc) rc = ord_launch(K.mark, A, n, false, "ordered map (inner elements)");
+  if (!rc) rc = ord_launch(K.unset, A, n, false, "ordered map (inner elements)");
+  int words[2] = {0, 0};
+  if (!rc) rc = ord_pass(K.inext, A, n, words, "ordered map (inner elements)");
+  if (rc) return rc;
+  if (words[1] & kCtlConflict) return PH_OK;
+  A.tn = T;
+  rc = ord_pass(K.walk, A, n, words, "ordered map (chains)");
+  if (rc) return rc;
+  const int fw = words[0];
+  rc = ord_pass(K.leaves, A, n, words, "ordered map (leaves)");
+  if (rc) return rc;
+  *flags = fw | words[0];
+  *exact = true;
   return PH_OK;
 }
 
@@ -2675,7 +2752,9 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
         flags = words[0];
         exact = true;
       } else {
-        rc = ord_levels(K, A, n, t[cur], q[1], &flags, &exact);
+        rc = ord_tree_kernels(S, w, &K);
+        if (!rc) rc = ord_inner_chains(K, A, n, t[cur], &flags, &exact);
+        if (!rc && !exact) rc = ord_levels(K, A, n, t[cur], q[1], &flags, &exact);
         if (rc) return rc;
       }
     }
@@ -2782,7 +2861,9 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
     S.esize = ph::dtype_size(prog->out_dtype);
     S.indexed = 1;
     S.out_indexed = true;
-    return ph::jit_compile_only(ord_source(S, WriteInverse()));
+    int rc6 = ph::jit_compile_only(ord_source(S, WriteInverse()));
+    if (!rc6 && single_hazard(S) >= 0) rc6 = ph::jit_compile_only(ord_source(S, WriteInverse(), 1));
+    return rc6;
   }
   if (kind == 3 || kind == 4) {
     // the kernels of a map whose input 0 reads the output's buffer
@@ -2819,7 +2900,10 @@ int ph_compile_check(const ph_prog *prog, int32_t n_in, const int32_t *in_dtypes
     S.esize = ph::dtype_size(in_dt[0]);
     if (kind == 3 && in_dt[0] != prog->out_dtype)
       return ph::set_error(PH_E_ARGUMENT, "kind 3: input 0 must have the output's dtype");
-    return ph::jit_compile_only(ord_source(S, write_inverse(*S.out)));
+    const WriteInverse wi = write_inverse(*S.out);
+    int rc3 = ph::jit_compile_only(ord_source(S, wi));
+    if (!rc3 && S.prog && single_hazard(S) >= 0) rc3 = ph::jit_compile_only(ord_source(S, wi, 1));
+    return rc3;
   }
   if (kind == 2) {
     if (prog_uses_index(prog)) return ph::set_error(PH_E_ARGUMENT, "the tiled map has no element index");

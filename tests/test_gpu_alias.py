@@ -262,13 +262,13 @@ def test_recurrence_down_the_rows_runs_per_column(ph):
 def test_broadcast_recurrence_forms_a_tree(ph):
     """z[1.., ..] = z[..-2, 0..0].tile(1, m).process { ... }: every element
     of row i reads z[i - 1, 0], so that element has m dependents (a tree,
-    not chains), evaluated a pass per row; row i ends up f^i(z0[0, 0]).
-    A block of 3 rows (no more than 16 elements per level: the loop in
-    order) and 3000 x 1024 (3000 passes: the loop in order took about a
-    second)."""
+    not chains); row i ends up f^i(z0[0, 0]).  The elements read by others
+    (column 0) form one chain, walked by one thread, and one pass then
+    computes the rest from their parents.  A block of 3 rows, 20000 x 3
+    and 3000 x 1024 (the loop in order took about a second)."""
     rng = np.random.default_rng(43)
     fn = lambda v: (v ^ 21) - 3  # noqa: E731
-    for n, m in [(40, 33), (3, 5), (3000, 1024)]:
+    for n, m in [(40, 33), (3, 5), (20000, 3), (3000, 1024)]:
         h = _i32(rng, (n, m))
         z = ph.NArray.from_numpy(h)
         ph.sync()

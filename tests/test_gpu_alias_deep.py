@@ -347,10 +347,11 @@ def test_map_indexed_programs_follow_the_loop(ph):
     through an index array: buf[w[k]] = f(buf[r[k]], y[k]) for k in order.
     One chain of 6000 dependent elements (a shift by one), 64 interleaved
     chains (a shift by 64: a thread per chain), the same with one element
-    read by two later ones (a tree: a pass per depth level), a random tree
-    (each element reads one of the 63 written before it), a tree as deep as
-    the chain (the in-order loop), and a second input y read through a
-    plain descriptor from another buffer."""
+    read by two later ones (a tree whose inner elements still form chains),
+    a random tree (each element reads one of the 63 written before it: a
+    pass per depth level), a tree as deep as the chain (the in-order loop),
+    and a second input y read through a plain descriptor from another
+    buffer."""
     lib = _lib(ph)
     lib.ph_map_indexed.argtypes = [ctypes.POINTER(_Prog), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
                                    ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
