@@ -13,6 +13,7 @@ and ph_map_indexed directly, against the loop they state."""
 import ctypes
 import os
 import random
+import time
 
 import numpy as np
 import pytest
@@ -390,3 +391,45 @@ def test_map_indexed_programs_follow_the_loop(ph):
         for k in range(n):
             ref[w[k]] = ((ref[r[k]] ^ 21) - 3) + yh[k]
         assert np.array_equal(buf.to_numpy(), ref.astype(np.int32)), name
+
+
+def test_map_indexed_random_tree_by_levels(ph):
+    """A random tree of 200000 elements, each reading one of the 63 written
+    just before it (about 6000 depth levels, so a pass per level), with the
+    output written through a permutation: the device result is the loop's,
+    buf[w[k]] = f(buf[r[k]], y[k]) for k in order."""
+    lib = _lib(ph)
+    lib.ph_map_indexed.argtypes = [ctypes.POINTER(_Prog), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                   ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    I = ph.DType.I32
+    prog = _prog(ph, ((ph.input(0, I) ^ 21) - 3) + ph.input(1, I), 2)
+    rng = np.random.default_rng(12)
+    n = 200000
+    w = rng.permutation(n)
+    back = rng.integers(1, 64, n)
+    k = np.arange(n)
+    r = np.where(k - back >= 0, w[np.maximum(k - back, 0)], rng.integers(0, n, n))  # read what element k - back wrote
+    h = rng.integers(-1000, 1000, n).astype(np.int32)
+    yh = rng.integers(-3, 4, n).astype(np.int32)
+    buf, y = ph.NArray.from_numpy(h), ph.NArray.from_numpy(yh)
+    ri, wi = _dev_i64(ph, r), _dev_i64(ph, w)
+    ins = (ctypes.c_void_p * 2)(buf.device_ptr, y.device_ptr)
+    ds = (_Desc * 2)(_desc(0, n, [n]), _desc(0, n, [n]))
+    ix = (ctypes.c_void_p * 2)(ri.device_ptr, None)
+    od = _desc(0, n, [n])
+    fl = ctypes.c_int32(0)
+    ph.sync()
+    t0 = time.perf_counter()
+    rc = lib.ph_map_indexed(ctypes.byref(prog), 2, ins, ds, ix, buf.device_ptr, ctypes.byref(od), wi.device_ptr,
+                            ctypes.byref(fl))
+    ph.sync()
+    dt = time.perf_counter() - t0
+    assert rc == 0 and fl.value == 0, lib.ph_last_error()
+    ref = h.astype(np.int64)
+    rl, wl, yl = r.tolist(), w.tolist(), yh.tolist()
+    refl = ref.tolist()
+    for j in range(n):
+        refl[wl[j]] = ((refl[rl[j]] ^ 21) - 3) + yl[j]
+    assert np.array_equal(buf.to_numpy(), np.array(refl, np.int64).astype(np.int32))
+    print(f"random tree of {n} elements by levels: {dt * 1e3:.1f} ms with compilation")
