@@ -637,8 +637,9 @@ def _cpu_leg(fn, by, what, size="full 8192^2"):
 def path_ops(phase, ev, args):
     """The other operations of the path on Int32 8192 x 8192 arrays:
     transposed gather and scatter (permuted View / MutableView), mask
-    assignment with a scalar and with a source array, elementwise equality
-    and map_with_coord.  Bytes are algorithmic: each element read or written
+    assignment with a scalar and with a source array, elementwise equality,
+    map_with_coord, a map over a transposed operand, and a recurrence whose
+    source is a ProcView of the array it writes.  Bytes are algorithmic: each element read or written
     once, Bool masks and results 1 byte."""
     import numpy as np
     D = phase.DType
@@ -716,6 +717,26 @@ def path_ops(phase, ev, args):
                 "sample": f"map_with over x.permute(1, 0) and y, its first {rows} rows, {dt * 1e3:.0f} ms"}
     entry("map_transposed", "ph_map_tiled (transposed operand staged through LDS)",
           lambda: xt.map_with(y, block=lambda a, b: a.wadd(b)).materialize(), 12 * nn, cpu_tmap)
+    # an aliased source (a12): each row becomes the row above it plus one,
+    # row 1 first, as ph-core's lazy loop gives; every column is a chain
+    zr = phase.NArray.random([n, n], D.I32, seed=7, a=-1000, b=1000)
+    zrh = zr.to_numpy()[:rows].copy() if O else None
+
+    def rec():
+        zr[["1..", ".."]] = zr.view("..-2", "..").process(lambda v: v + 1)
+
+    def cpu_rec():
+        from crystal import parse_literal, trace
+        dst = O.region([parse_literal("1.."), parse_literal("..")], [rows, n])
+        src = O.region([parse_literal("..-2"), parse_literal("..")], [rows, n])
+        h = zrh.copy()
+        t0 = time.perf_counter()
+        O.set_chunk_lazy(h, [], dst, trace(lambda v: v + 1), [(h, [("region", src)])], src.shape)
+        dt = time.perf_counter() - t0
+        return {"value": round(8 * (rows - 1) * n / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                "sample": f"set_chunk of a ProcView of the same array, its first {rows} rows, {dt * 1e3:.0f} ms"}
+    entry("aliased_recurrence", "ordered map: 3 passes, chain links, a walk per column, the write",
+          rec, 8 * (n - 1) * n, cpu_rec)
     return res
 
 

@@ -1860,7 +1860,8 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 //    unique and is the sequential result, and an element whose dependency
 //    chain is d deep is exact from T_d on.  A pass that reproduces T bit for
 //    bit has reached it, and its flags are the sequential run's.  Past
-//    kOrdMaxIter passes (a deep chain, such as a shift by one element):
+//    kOrdMaxIter passes (kOrdMaxIterWalk with one input on the output's
+//    buffer; a deep chain, such as a shift by one element):
 //    with one input on the output's buffer, every element depends on at
 //    most one earlier element, and when none has two dependents the
 //    dependencies are chains; a thread per chain root then runs its chain
@@ -1879,6 +1880,12 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 // the old value at the read address of its chain's root, and
 // ceil(log2(depth)) passes over the chain links find every root.
 constexpr int kOrdMaxIter = 12;
+// With one input on the output's buffer the chain walk and the tree passes
+// are exact at any depth, so the passes stop sooner: three still settle a map
+// whose elements depend at most on elements that depend on none (a
+// transposed self-assignment), and a deep chain pays 3 passes before its
+// walk, not 13.
+constexpr int kOrdMaxIterWalk = 2;
 constexpr int64_t kSeqChunk = 1 << 16;
 constexpr int64_t kOrdMaxLevels = 1 << 16;
 constexpr int64_t kOrdLevelWidth = 16;  // elements per level, on average, below which the loop runs in order
@@ -2717,7 +2724,8 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
   } else {
     int flags = 0;
     bool exact = false;
-    for (int it = 0; it <= kOrdMaxIter; ++it) {
+    const int max_iter = K.walk ? kOrdMaxIterWalk : kOrdMaxIter;
+    for (int it = 0; it <= max_iter; ++it) {
       A.iter = it;
       A.tp = t[cur];
       A.tn = t[cur ^ 1];
