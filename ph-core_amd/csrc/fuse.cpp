@@ -1866,7 +1866,9 @@ int map_tiled(const ph_prog *prog, int n_in, const void *const *ins, const int *
 //    most one earlier element, and when none has two dependents the
 //    dependencies are chains; a thread per chain root then runs its chain
 //    in order, carrying each result to the next element (a recurrence down
-//    the rows of a matrix runs a thread per column).  When some element has
+//    the rows of a matrix runs a thread per column).  When that input reads
+//    each address at most once through an affine view, its chains follow
+//    in closed form (ph_ord_walkc) and the passes before are skipped.  When some element has
 //    several dependents the dependencies form a tree.  If the elements that
 //    others read still form chains, those chains are walked the same way
 //    and one pass then computes the rest from their parents' results (a
@@ -2278,6 +2280,27 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
     s += elem("from_chain", "carry");
     s += fmt("      carry = (%s)res;\n      from_chain = true;\n      }\n    }\n", vtype(hz1).c_str());
     s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+    const WriteInverse ri = (S.in[hz1]->nstage == 0 && !(S.indexed >> hz1 & 1) && !S.out_indexed && w.ok)
+                                ? write_inverse(*S.in[hz1])
+                                : WriteInverse();
+    if (ri.ok) {
+      // walkc: the same walk when the input on the buffer reads each
+      // address at most once (an injective affine map): the dependencies
+      // are then chains, an element is a root when no earlier element
+      // wrote the address it reads, and its successor is the element that
+      // reads the address it writes (the read map inverted), if that comes
+      // later.  No link passes and no pointer loads.
+      s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_walkc(OrdArgs A) {\n  int fl = 0;\n" + loop;
+      s += emit_coords(S.out->view, "L", "") + in_addr(hz1, "L") + inverse(hz1);
+      s += fmt("      if (ok_%d && lp_%d < L) continue;\n", hz1, hz1);
+      s += fmt("      %s carry = %s(0);\n      bool from_chain = false;\n", vtype(hz1).c_str(), vtype(hz1).c_str());
+      s += "      for (i64 E = L;;) {\n";
+      s += elem("from_chain", "carry");
+      s += fmt("      carry = (%s)res;\n      from_chain = true;\n", vtype(hz1).c_str());
+      s += out_addr("E") + emit_inverse(ri, "w_", "_s");
+      s += "      if (!ok_s || lp_s <= E) break;\n      E = lp_s;\n      }\n    }\n";
+      s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+    }
 
     // Trees whose inner elements (those another element reads) form
     // chains: the walk above runs the chains (q1 then links each inner
@@ -2381,7 +2404,7 @@ struct OrdKernels {
   hipFunction_t link = nullptr, jump = nullptr, gather = nullptr;
   hipFunction_t check = nullptr, direct = nullptr, mm_init = nullptr, inv_clear = nullptr, inv_fill = nullptr,
                 inv_check = nullptr;
-  hipFunction_t mlink = nullptr, unset = nullptr, next = nullptr, walk = nullptr;
+  hipFunction_t mlink = nullptr, unset = nullptr, next = nullptr, walk = nullptr, walkc = nullptr;
   hipFunction_t dinit = nullptr, djump = nullptr, dmax = nullptr, lcount = nullptr, lplace = nullptr, level = nullptr;
   hipFunction_t mclear = nullptr, mark = nullptr, inext = nullptr, leaves = nullptr;
 };
@@ -2428,6 +2451,7 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_unset", &K->unset);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_next", &K->next);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_walk", &K->walk);
+  if (!rc && chains && src.find("ph_ord_walkc") != std::string::npos) rc = ph::jit_kernel2(src, "ph_ord_walkc", &K->walkc);
   if (rc) return rc;
   ord_cache()[key] = *K;
   return PH_OK;
@@ -2722,6 +2746,15 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
     rc = ord_launch(K.gather, A, n, false, "ordered copy (gather)");
     if (rc) return rc;
   } else {
+    if (K.walkc && (words[1] & kCtlRaw)) {
+      // chains in closed form: one walk into T, then the write
+      A.tn = t[0];
+      rc = ord_pass(K.walkc, A, n, words, "ordered map (chains)");
+      if (rc) return rc;
+      if (flags_out) *flags_out = words[0];
+      A.tp = t[0];
+      return ord_launch(K.write, A, n, false, "ordered map (write)");
+    }
     int flags = 0;
     bool exact = false;
     const int max_iter = K.walk ? kOrdMaxIterWalk : kOrdMaxIter;
