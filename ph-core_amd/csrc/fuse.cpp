@@ -30,6 +30,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <map>
+#include <regex>
 #include <string>
 #include <vector>
 
@@ -2068,11 +2069,13 @@ __device__ __forceinline__ void ph_stp(unsigned char *p, i64 i, bool v) { p[i] =
 __device__ __forceinline__ void ph_stp(int *p, i64 i, int v) { p[i] = v; }
 __device__ __forceinline__ void ph_stp(float *p, i64 i, float v) { p[i] = v; }
 __device__ __forceinline__ void ph_stp(double *p, i64 i, double v) { p[i] = v; }
-// control bits of a wavefront, one atomic
+// control bits of a wavefront: one atomic, skipped when its bits are
+// already set (they only ever go from 0 to 1; atomics on the one word
+// serialise, about 11 ns each, where every wavefront of a pass sets a bit)
 __device__ __forceinline__ void ph_ctl(int *w, int ctl) {
   const unsigned long long b1 = __ballot(ctl & 1), b2 = __ballot(ctl & 2), b4 = __ballot(ctl & 4);
   const int v = (b1 ? 1 : 0) | (b2 ? 2 : 0) | (b4 ? 4 : 0);
-  if ((threadIdx.x & 63) == 0 && v) atomicOr(w, v);
+  if ((threadIdx.x & 63) == 0 && v && (*(volatile int *)w & v) != v) atomicOr(w, v);
 }
 )PH";
 
@@ -2104,6 +2107,26 @@ int single_hazard(const OrdSpec &S) {
       h = j;
     }
   return h;
+}
+
+// A kernel's source with its index arithmetic in 32 bits: a local
+// `typedef int i64` and integer literals without their LL suffix (the caller
+// has checked that every index, address and extent fits).  Division by a
+// constant then takes a 32-bit multiply-high instead of a 64-bit sequence.
+std::string narrow_index(const std::string &k) {
+  const size_t open = k.find("{\n");
+  if (open == std::string::npos) return k;
+  std::string r = k.substr(0, open + 2) + "  typedef int i64;\n" + k.substr(open + 2);
+  return std::regex_replace(r, std::regex("([0-9])LL"), "$1");
+}
+
+// Whether every index and address of S fits in 30 bits (no index arrays).
+bool ord_small(const OrdSpec &S) {
+  const int64_t lim = int64_t(1) << 30;
+  if (S.any_index() || ph::desc_count(S.out) >= lim || S.out->extent >= lim) return false;
+  for (int j = 0; j < S.n_in; ++j)
+    if (S.in[j]->extent >= lim) return false;
+  return true;
 }
 
 // part 0: every kernel but those of trees of dependents; part 1: those.
@@ -2155,7 +2178,9 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
     *code += fmt("      const auto res = %s;\n", r.c_str());
   };
 
+  const bool small = ord_small(S);
   // classify: conflicts and reads after writes
+  size_t mark = s.size();
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_classify(OrdArgs A) {\n  int ctl = 0;\n";
   s += loop + emit_coords(S.out->view, "L", "");
   for (int j = 0; j < S.n_in; ++j) {
@@ -2166,6 +2191,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
              sx.c_str(), sx.c_str());
   }
   s += "    }\n  ph_ctl(A.flags + 1, ctl);\n}\n";
+  if (small) s = s.substr(0, mark) + narrow_index(s.substr(mark));
 
   // relax: one Jacobi pass into tn, reading tp where an earlier element wrote
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_relax(OrdArgs A) {\n  int fl = 0, ctl = 0;\n";
@@ -2194,9 +2220,11 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
   s += "    }\n  if (fl) atomicOr(A.flags, fl);\n  ph_ctl(A.flags + 1, ctl);\n}\n";
 
   // write: out[W(L)] = tp[L]
+  mark = s.size();
   s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_write(OrdArgs A) {\n";
   s += loop + emit_coords(S.out->view, "L", "") + out_addr("L");
   s += fmt("      ((%s *)A.out)[w_] = ((const %s *)A.tp)[L];\n    }\n}\n", OST.c_str(), OST.c_str());
+  if (small) s = s.substr(0, mark) + narrow_index(s.substr(mark));
 
   // seq: the loop itself, in order, over [l0, l1), one thread
   s += "extern \"C\" __global__ void __launch_bounds__(64) ph_ord_seq(OrdArgs A) {\n";
@@ -2290,6 +2318,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
       // wrote the address it reads, and its successor is the element that
       // reads the address it writes (the read map inverted), if that comes
       // later.  No link passes and no pointer loads.
+      const size_t mk = s.size();
       s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_walkc(OrdArgs A) {\n  int fl = 0;\n" + loop;
       s += emit_coords(S.out->view, "L", "") + in_addr(hz1, "L") + inverse(hz1);
       s += fmt("      if (ok_%d && lp_%d < L) continue;\n", hz1, hz1);
@@ -2300,6 +2329,7 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
       s += out_addr("E") + emit_inverse(ri, "w_", "_s");
       s += "      if (!ok_s || lp_s <= E) break;\n      E = lp_s;\n      }\n    }\n";
       s += "  if (fl) atomicOr(A.flags, fl);\n}\n";
+      if (small) s = s.substr(0, mk) + narrow_index(s.substr(mk));
     }
 
     // Trees whose inner elements (those another element reads) form
