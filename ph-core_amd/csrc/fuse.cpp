@@ -2256,6 +2256,23 @@ std::string ord_source(const OrdSpec &S, const WriteInverse &w, int part = 0) {
     s += loop + "      const i64 q = A.iter ? A.q0[L] : -1; const i64 R = q >= 0 ? q : L;\n";
     s += emit_coords(S.out->view, "R", "") + in_addr(0, "R");
     s += fmt("      ((%s *)A.tn)[L] = ((const %s *)A.in[0])[a_0];\n    }\n}\n", OST.c_str(), OST.c_str());
+    const WriteInverse ri = (S.in[0]->nstage == 0 && !S.any_index() && w.ok) ? write_inverse(*S.in[0]) : WriteInverse();
+    if (ri.ok) {
+      // walkc: a source that reads each address at most once makes the
+      // links chains in closed form (roots: no earlier element wrote the
+      // address read; successor: the element reading the address written,
+      // if later); every element of a chain takes its root's old value
+      const size_t mk = s.size();
+      s += "extern \"C\" __global__ void __launch_bounds__(256) ph_ord_walkc(OrdArgs A) {\n" + loop;
+      s += emit_coords(S.out->view, "L", "") + in_addr(0, "L") + inverse(0);
+      s += "      if (ok_0 && lp_0 < L) continue;\n";
+      s += fmt("      const %s v = ((const %s *)A.in[0])[a_0];\n", OST.c_str(), OST.c_str());
+      s += "      for (i64 E = L;;) {\n";
+      s += fmt("      ((%s *)A.tn)[E] = v;\n", OST.c_str());
+      s += emit_coords(S.out->view, "E", "") + out_addr("E") + emit_inverse(ri, "w_", "_s");
+      s += "      if (!ok_s || lp_s <= E) break;\n      E = lp_s;\n      }\n    }\n}\n";
+      if (small) s = s.substr(0, mk) + narrow_index(s.substr(mk));
+    }
   }
   const int hz1 = single_hazard(S);
   std::string t;  // the kernels of trees of dependents (part 1)
@@ -2481,7 +2498,7 @@ int ord_kernels(const OrdSpec &S, const WriteInverse &w, OrdKernels *K) {
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_unset", &K->unset);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_next", &K->next);
   if (!rc && chains) rc = ph::jit_kernel2(src, "ph_ord_walk", &K->walk);
-  if (!rc && chains && src.find("ph_ord_walkc") != std::string::npos) rc = ph::jit_kernel2(src, "ph_ord_walkc", &K->walkc);
+  if (!rc && src.find("ph_ord_walkc") != std::string::npos) rc = ph::jit_kernel2(src, "ph_ord_walkc", &K->walkc);
   if (rc) return rc;
   ord_cache()[key] = *K;
   return PH_OK;
@@ -2750,8 +2767,16 @@ int run_ordered(const OrdSpec &S, const void *const *ins, void *out, int32_t *fl
   const bool raw = (words[1] & kCtlRaw) != 0;
   int cur = 0;  // T holding the result
   if (!S.prog) {
-    // a copy: chain links, pointer jumping to the roots, one gather
+    // a copy: chain links, pointer jumping to the roots, one gather (or,
+    // with an injective affine source, the chains walked in closed form)
     A.iter = 0;
+    if (raw && K.walkc) {
+      A.tn = t[0];
+      rc = ord_launch(K.walkc, A, n, false, "ordered copy (chains)");
+      if (rc) return rc;
+      A.tp = t[0];
+      return ord_launch(K.write, A, n, false, "ordered copy (write)");
+    }
     if (raw) {
       void *q[2];
       rc = ph::scratch((size_t)n * sizeof(int64_t), &q[0], 6);
