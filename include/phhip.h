@@ -272,14 +272,16 @@ int ph_fill_scatter(void *dst, int32_t dtype, const int64_t *dst_shape,
  * do when a source or mask is a view of the array being written
  * (src/n_array.cr:484-492, :523-536).  When no element reads another's
  * output the map runs in one parallel pass; otherwise the order is resolved
- * by repeated passes to a fixed point.  Past 12 dependent elements, when
- * only one input reads the buffer: a device thread per chain when no
- * element has two dependents; else, for the tree of dependents, a thread
- * per chain of the elements that others read (when those form chains) and
- * one pass for the rest, or a pass per depth level (when the levels
- * average 16 elements or more and number at most 65536).  Other deep maps run a sequential loop on one device
- * thread.  An input that overlaps the output through another base pointer
- * is PH_E_ARGUMENT.
+ * by repeated passes to a fixed point (at most 12; 3 when only one input
+ * reads the buffer).  With one input on the buffer, deeper dependencies
+ * run a device thread per chain when no element has two dependents (with
+ * no passes at all when that input's affine view reads each address at
+ * most once); else, for the tree of dependents, a thread per chain of the
+ * elements that others read (when those form chains) and one pass for the
+ * rest, or a pass per depth level (when the levels average 16 elements or
+ * more and number at most 65536).  Other deep maps run a sequential loop
+ * on one device thread.  An input that overlaps the output through another
+ * base pointer is PH_E_ARGUMENT.
  * A program with PH_OP_INDEX (NArray#map_with_index / map_with_index! /
  * map_with_coord, src/n_array.cr:605-650) sees each element's lexicographic
  * position in out_desc's shape as an Int32; a shape of more than 2^31 - 1
